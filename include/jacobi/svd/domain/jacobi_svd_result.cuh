@@ -1,0 +1,19 @@
+#pragma once
+// Host result of one solve (reference include/jacobi/svd/domain/jacobi_svd_result.cuh:11-42):
+// U m x n row-major, sigma n values in final column order (unsorted),
+// V n x n row-major, and the sweep count including the verifying sweep.
+#include <cstddef>
+#include <vector>
+
+namespace jacobi::svd
+{
+    struct JacobiSvdResult final
+    {
+        std::size_t rows = 0;
+        std::size_t columns = 0;
+        std::vector<double> u;
+        std::vector<double> sigma;
+        std::vector<double> v;
+        int sweeps = 0;
+    };
+} // namespace jacobi::svd

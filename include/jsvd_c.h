@@ -1,0 +1,148 @@
+/*
+ * jsvd_c.h -- C-ABI of the MI355X (gfx950) one-sided Jacobi SVD.
+ *
+ * Plain C types only (pointers, sizes, status codes); no HIP or torch types
+ * appear in any signature.  The C++ API in include/jacobi/svd/ wraps these
+ * entry points and rethrows the reference's exception types and messages.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the root of kleedaisuki/jacobi-svd-cuda-opt).
+ *
+ * Layout conventions (identical to the reference's JacobiSvdResult,
+ * include/jacobi/svd/domain/jacobi_svd_result.cuh:26-36):
+ *   A  m x n row-major (m >= n)        U  m x n row-major
+ *   sigma  n values, in final column order (not sorted)
+ *   V  n x n row-major
+ * Packed batches store matrix k right after matrix k-1 with no padding:
+ * A and U use m_k*n_k elements, sigma n_k, V n_k*n_k, sweeps one int32.
+ *
+ * Threading: every function is thread-safe.  Calls on the library's own
+ * stream are serialised per device; plan executions run on the caller's
+ * stream and may overlap with other plans.
+ */
+#ifndef JSVD_C_H
+#define JSVD_C_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum jsvd_status
+{
+    JSVD_OK = 0,
+    JSVD_EINVAL = 1,  /* invalid argument (reference: std::invalid_argument) */
+    JSVD_EDEVICE = 2, /* HIP runtime failure (reference: jacobi::svd::CudaError) */
+    JSVD_ENOMEM = 3   /* device allocation failed */
+} jsvd_status;
+
+/* Arithmetic of a solve. */
+enum
+{
+    JSVD_PRECISION_FP64 = 0,  /* plain FP64 Jacobi in the reference's pair order */
+    JSVD_PRECISION_MIXED = 1  /* FP32 sweeps + FP64 rotation parameters + FP64 refinement */
+};
+
+/* Layout policy values (reference LayoutTransposeMode,
+ * include/jacobi/svd/domain/layout_transpose.cuh:10-26). */
+enum
+{
+    JSVD_LAYOUT_AUTO = 0,
+    JSVD_LAYOUT_ON = 1,
+    JSVD_LAYOUT_OFF = 2
+};
+
+/* Mirrors JacobiSvdConfig (include/jacobi/svd/domain/jacobi_svd_config.cuh:12-58)
+ * field for field, then appends the knobs the reference lacks. */
+typedef struct jsvd_config
+{
+    double epsilon;              /* default 1e-9: rotate pair when |pq| > eps*sqrt(pp*qq) */
+    int32_t max_sweeps;          /* default 128 */
+    int32_t threads_per_block;   /* default 256; rounded to a multiple of 64 in [64,1024] */
+    int32_t layout_mode;         /* JSVD_LAYOUT_*; accepted and validated, no numeric effect */
+    int32_t layout_min_columns;  /* default 16 */
+    uint64_t layout_min_elements;/* default 4096 */
+    int32_t layout_auto_tune;    /* default 0 */
+    int32_t layout_bench_reps;   /* default 2 */
+    int32_t layout_bench_sweeps; /* default 8 */
+    /* --- no reference counterpart --- */
+    int32_t precision;           /* JSVD_PRECISION_*; default FP64 */
+    int32_t block_width;         /* 0 = plain pairs; 16 or 32 = blocked Jacobi for large matrices */
+    int32_t refine_sweeps;       /* mixed: FP64 sweep cap after the FP32 phase (default 4) */
+} jsvd_config;
+
+/* Fills *cfg with the reference defaults (jacobi_svd_config.cuh:17-57). */
+void jsvd_config_default(jsvd_config *cfg);
+
+/* Message of the last failing call on this thread ("" if none).  Same text
+ * the C++ API throws. */
+const char *jsvd_last_error(void);
+
+/* Number of the library's ABI revision. */
+int32_t jsvd_abi_version(void);
+
+/* Single solve from host memory, synchronous.
+ * Replaces jacobi::svd::one_sided_jacobi_svd
+ * (include/jacobi/svd/domain/jacobi_svd.cuh:25-28, src/domain/jacobi_svd.cu:402-412). */
+jsvd_status jsvd_solve_f64(const double *a_rm, uint64_t m, uint64_t n, const jsvd_config *cfg, double *u_rm,
+                           double *sigma, double *v_rm, int32_t *sweeps);
+
+/* Batched solve from host memory: count independent matrices, pointer per
+ * matrix.  Synchronous.  Replaces the per-testcase loop of
+ * src/application/pipeline.cu:59-83 -> src/application/kernel_stage.cuh:30-58. */
+jsvd_status jsvd_solve_batched_f64(int64_t count, const double *const *a_rm, const uint64_t *m, const uint64_t *n,
+                                   const jsvd_config *cfg, double *const *u_rm, double *const *sigma,
+                                   double *const *v_rm, int32_t *sweeps);
+
+/* ---- device-resident batches (no reference counterpart: the reference has no
+ * batched or asynchronous entry point; this is the work-list launch the
+ * batched pipeline above is built on) ---- */
+typedef struct jsvd_plan jsvd_plan;
+
+typedef struct jsvd_plan_info
+{
+    int64_t count;             /* matrices in the plan */
+    int64_t lds_matrices;      /* solved by the LDS-resident work-list kernel */
+    int64_t large_matrices;    /* solved by the persistent grid-barrier kernel */
+    int64_t a_elements;        /* packed A / U elements */
+    int64_t v_elements;        /* packed V elements */
+    int64_t sigma_elements;    /* packed sigma elements */
+    int32_t workgroup_size;    /* threads per workgroup of the work-list kernel */
+    int32_t grid_size;         /* workgroups of the work-list kernel */
+    int32_t lds_bytes;         /* dynamic LDS per workgroup */
+    int32_t precision;
+} jsvd_plan_info;
+
+/* Builds the device work-list for count matrices of shapes m[k] x n[k].
+ * Allocates all device memory the plan needs; execute allocates nothing. */
+jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, const jsvd_config *cfg,
+                             jsvd_plan **out);
+
+/* Runs the plan on device buffers (packed layout above) on `stream`
+ * (a hipStream_t, or NULL for the null stream).  Asynchronous.  d_sweeps may
+ * be NULL.  For JSVD_PRECISION_MIXED the inputs and outputs are still FP64. */
+jsvd_status jsvd_plan_execute_f64(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                                  int32_t *d_sweeps, void *stream);
+
+jsvd_status jsvd_plan_get_info(const jsvd_plan *plan, jsvd_plan_info *info);
+void jsvd_plan_destroy(jsvd_plan *plan);
+
+/* Layout-threshold autotune.  Replaces jacobi::svd::auto_tune_layout_transpose_threshold
+ * (include/jacobi/svd/domain/jacobi_svd.cuh:36-37, src/domain/layout_transpose_autotune.cu:78-126). */
+typedef struct jsvd_autotune_report
+{
+    int32_t executed;
+    int32_t recommended_min_columns;
+    uint64_t recommended_min_elements;
+    double estimated_best_speedup;
+    uint64_t sample_count;
+} jsvd_autotune_report;
+
+jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JSVD_C_H */

@@ -1,0 +1,65 @@
+// jsvd_kernels.hpp -- launch-side declarations of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace jsvd
+{
+    // One entry of the device work-list: a matrix and where its operands live
+    // in the packed batch buffers (offsets in elements; U shares A's offset).
+    struct WorkItem
+    {
+        int32_t m;
+        int32_t n;
+        int32_t index; // position in the caller's batch (sweeps output slot)
+        int32_t pad;
+        int64_t a_off;
+        int64_t v_off;
+        int64_t s_off;
+    };
+
+    // Dynamic LDS bytes the LDS-resident FP64 kernel needs for an m x n matrix.
+    inline int64_t lds_bytes_f64(int64_t m, int64_t n)
+    {
+        return (m * n + n * n + n) * 8;
+    }
+    // Same for the mixed-precision kernel (FP32 A and V, FP64 sigma scratch).
+    inline int64_t lds_bytes_mixed(int64_t m, int64_t n)
+    {
+        return (m * n + n * n) * 4 + n * 8;
+    }
+
+    constexpr int64_t kMaxLdsBytes = 160 * 1024 - 256;
+
+    hipError_t launch_lds_batch_f64(int block, int grid, int lds_bytes, const WorkItem *items, int count,
+                                    const double *a, double *u, double *sigma, double *v, int32_t *sweeps,
+                                    double eps, int max_sweeps, unsigned int *counters, hipStream_t stream);
+    const void *lds_batch_f64_symbol(int block);
+
+    hipError_t launch_lds_batch_mixed(int block, int grid, int lds_bytes, const WorkItem *items, int count,
+                                      const double *a, double *u, double *sigma, double *v, int32_t *sweeps,
+                                      double eps, int max_sweeps, int refine_sweeps, unsigned int *counters,
+                                      hipStream_t stream);
+    const void *lds_batch_mixed_symbol(int block);
+
+    // Persistent plain-FP64 sweep kernel over one column-major matrix.
+    hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
+                                            double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
+                                            int32_t *sweeps_out, hipStream_t stream);
+    const void *persistent_sweeps_f64_symbol(int block);
+
+    // out (cols x rows, row-major) = transpose of in (rows x cols, row-major);
+    // when row_scale is given, input row r is divided by row_scale[r] (and
+    // written as 0 when row_scale[r] <= eps).
+    hipError_t launch_transpose_f64(const double *in, double *out, int64_t rows, int64_t cols,
+                                    const double *row_scale, double eps, hipStream_t stream);
+
+    // sigma[j] = ||column j|| of a column-major m x n matrix.
+    hipError_t launch_column_norms_f64(const double *a_cm, double *sigma, int64_t m, int n, hipStream_t stream);
+
+    // V <- I (n x n), flags[0..count) <- 0, sweeps slot <- 0.
+    hipError_t launch_identity_f64(double *v, int n, hipStream_t stream);
+    hipError_t launch_fill_i32(int32_t *p, int32_t value, int64_t count, hipStream_t stream);
+} // namespace jsvd

@@ -1,0 +1,547 @@
+// jsvd_kernels_f64.hip -- plain FP64 one-sided Jacobi kernels for gfx950.
+//
+// Two sweep kernels cover every shape:
+//  * lds_batch_f64: a persistent work-list kernel.  Each workgroup claims
+//    whole matrices from a device work-list (sorted by cost, largest first)
+//    and keeps A and V resident in LDS for the entire solve: load once,
+//    all sweeps with workgroup barriers only, convergence decided on-device,
+//    U / Sigma / V written once.  Pairs of a round are spread over aligned
+//    lane groups; each group reduces its three dots with wave64 shuffles and
+//    applies the rotation to the rows it holds in registers.
+//  * persistent_sweeps_f64: matrices too large for LDS.  A cooperative
+//    persistent kernel with a grid barrier per round; one workgroup per
+//    column pair, columns held in registers between the dot and the update,
+//    and the sweep loop with its convergence flag kept on the device.
+//
+// Arithmetic per pair follows the reference kernel
+// (src/domain/jacobi_rotation_kernels.cuh:276-353): dots over the rows,
+// Rutishauser rotation with the |pq| > eps*sqrt(pp*qq) test, then
+// a_p <- c a_p - s a_q, a_q <- s a_p + c a_q on A and on V.  Pairs come in
+// the reference's circle order, so the final column order (and with it the
+// unsorted sigma) is the reference's.
+#include "jsvd_device.hpp"
+#include "jsvd_kernels.hpp"
+
+namespace jsvd
+{
+    namespace
+    {
+        __device__ __forceinline__ int pow2_floor(int x)
+        {
+            return x <= 1 ? 1 : (1 << (31 - __clz(x)));
+        }
+        __device__ __forceinline__ int pow2_ceil(int x)
+        {
+            return x <= 1 ? 1 : (1 << (32 - __clz(x - 1)));
+        }
+
+        // One column pair held in LDS: G lanes share the rows.  Returns 1 when
+        // the pair counts as rotated.
+        template <int RMAX>
+        __device__ __forceinline__ int pair_lds_f64(double *__restrict__ ap, double *__restrict__ aq,
+                                                    double *__restrict__ vp, double *__restrict__ vq, int m, int n,
+                                                    int G, int lane, double eps)
+        {
+            double pp = 0.0, qq = 0.0, pq = 0.0;
+            const int rows = (m + G - 1) / G; // uniform over the group
+            if (rows <= RMAX)
+            {
+                double xp[RMAX], xq[RMAX];
+#pragma unroll
+                for (int j = 0; j < RMAX; ++j)
+                {
+                    if (j < rows)
+                    {
+                        const int i = lane + j * G;
+                        const bool in = i < m;
+                        xp[j] = in ? ap[i] : 0.0;
+                        xq[j] = in ? aq[i] : 0.0;
+                        pp += xp[j] * xp[j];
+                        qq += xq[j] * xq[j];
+                        pq += xp[j] * xq[j];
+                    }
+                }
+                pp = dev::group_sum(pp, G);
+                qq = dev::group_sum(qq, G);
+                pq = dev::group_sum(pq, G);
+                double c, s;
+                const bool rot = dev::rotation(pp, qq, pq, eps, c, s);
+                if (s != 0.0)
+                {
+#pragma unroll
+                    for (int j = 0; j < RMAX; ++j)
+                    {
+                        if (j < rows)
+                        {
+                            const int i = lane + j * G;
+                            if (i < m)
+                            {
+                                ap[i] = c * xp[j] - s * xq[j];
+                                aq[i] = s * xp[j] + c * xq[j];
+                            }
+                        }
+                    }
+                    for (int i = lane; i < n; i += G)
+                    {
+                        const double x = vp[i], y = vq[i];
+                        vp[i] = c * x - s * y;
+                        vq[i] = s * x + c * y;
+                    }
+                }
+                return rot ? 1 : 0;
+            }
+            for (int i = lane; i < m; i += G)
+            {
+                const double x = ap[i], y = aq[i];
+                pp += x * x;
+                qq += y * y;
+                pq += x * y;
+            }
+            pp = dev::group_sum(pp, G);
+            qq = dev::group_sum(qq, G);
+            pq = dev::group_sum(pq, G);
+            double c, s;
+            const bool rot = dev::rotation(pp, qq, pq, eps, c, s);
+            if (s != 0.0)
+            {
+                for (int i = lane; i < m; i += G)
+                {
+                    const double x = ap[i], y = aq[i];
+                    ap[i] = c * x - s * y;
+                    aq[i] = s * x + c * y;
+                }
+                for (int i = lane; i < n; i += G)
+                {
+                    const double x = vp[i], y = vq[i];
+                    vp[i] = c * x - s * y;
+                    vq[i] = s * x + c * y;
+                }
+            }
+            return rot ? 1 : 0;
+        }
+
+        template <int BS, int RMAX>
+        __device__ void solve_lds_f64(const WorkItem &w, double *lds, const double *__restrict__ a,
+                                      double *__restrict__ u, double *__restrict__ sigma, double *__restrict__ v,
+                                      int32_t *__restrict__ sweeps, double eps, int max_sweeps)
+        {
+            const int m = w.m, n = w.n, tid = threadIdx.x;
+            double *A = lds;                         // column-major, leading dimension m
+            double *V = lds + (int64_t)m * n;        // column-major n x n
+            double *S = V + (int64_t)n * n;          // sigma scratch
+            const double *ag = a + w.a_off;
+            const int mn = m * n, nn = n * n;
+            for (int idx = tid; idx < mn; idx += BS)
+            {
+                const int i = idx / n, j = idx - (idx / n) * n;
+                A[j * m + i] = ag[idx];
+            }
+            for (int idx = tid; idx < nn; idx += BS)
+                V[idx] = (idx % (n + 1) == 0) ? 1.0 : 0.0;
+            __syncthreads();
+
+            const int even = n + (n & 1);
+            const int half = even >> 1;
+            const int rounds = n < 2 ? 0 : even - 1;
+            int G = pow2_floor(BS / (half > 0 ? half : 1));
+            G = min(G, 64);
+            G = min(G, pow2_ceil(m));
+            const int groups = BS / G;
+            const int gid = tid / G, lane = tid & (G - 1);
+
+            int executed = 0;
+            for (int sweep = 0; sweep < max_sweeps; ++sweep)
+            {
+                int any = 0;
+                for (int r = 0; r < rounds; ++r)
+                {
+                    for (int k = gid; k < half; k += groups)
+                    {
+                        const int p = dev::column_at(k, r, n, even);
+                        const int q = dev::column_at(even - 1 - k, r, n, even);
+                        if (p < 0 || q < 0)
+                            continue;
+                        any |= pair_lds_f64<RMAX>(A + p * m, A + q * m, V + p * n, V + q * n, m, n, G, lane, eps);
+                    }
+                    __syncthreads();
+                }
+                executed = sweep + 1;
+                if (!__syncthreads_or(any))
+                    break;
+            }
+
+            // U / Sigma epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
+            // sigma_j = ||a_j||, U_ij = a_ij / sigma_j if sigma_j > eps else 0.
+            for (int j = gid; j < n; j += groups)
+            {
+                double acc = 0.0;
+                for (int i = lane; i < m; i += G)
+                {
+                    const double x = A[j * m + i];
+                    acc += x * x;
+                }
+                acc = dev::group_sum(acc, G);
+                if (lane == 0)
+                {
+                    const double sj = sqrt(fmax(acc, 0.0));
+                    S[j] = sj;
+                    sigma[w.s_off + j] = sj;
+                }
+            }
+            __syncthreads();
+            double *ug = u + w.a_off;
+            for (int idx = tid; idx < mn; idx += BS)
+            {
+                const int i = idx / n, j = idx - (idx / n) * n;
+                const double sj = S[j];
+                ug[idx] = (sj > eps) ? (A[j * m + i] / sj) : 0.0;
+            }
+            double *vg = v + w.v_off;
+            for (int idx = tid; idx < nn; idx += BS)
+            {
+                const int i = idx / n, j = idx - (idx / n) * n;
+                vg[idx] = V[j * n + i];
+            }
+            if (tid == 0 && sweeps)
+                sweeps[w.index] = executed;
+        }
+
+        // counters[0]: next work-list entry; counters[1]: workgroups finished.
+        // The last workgroup to finish resets both, so the next launch on the
+        // stream starts from a clean work-list without a memset.
+        template <int BS, int RMAX>
+        __global__ __launch_bounds__(BS) void lds_batch_f64(const WorkItem *__restrict__ items, int count,
+                                                            const double *__restrict__ a, double *__restrict__ u,
+                                                            double *__restrict__ sigma, double *__restrict__ v,
+                                                            int32_t *__restrict__ sweeps, double eps, int max_sweeps,
+                                                            unsigned int *__restrict__ counters)
+        {
+            extern __shared__ double lds[];
+            __shared__ int s_item;
+            for (;;)
+            {
+                if (threadIdx.x == 0)
+                    s_item = (int)atomicAdd(&counters[0], 1u);
+                __syncthreads();
+                const int item = s_item;
+                if (item >= count)
+                    break;
+                const WorkItem w = items[item];
+                solve_lds_f64<BS, RMAX>(w, lds, a, u, sigma, v, sweeps, eps, max_sweeps);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0)
+            {
+                const unsigned int done = atomicAdd(&counters[1], 1u);
+                if (done == gridDim.x - 1)
+                {
+                    __hip_atomic_store(&counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&counters[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+
+        // Persistent sweeps over one column-major matrix too large for LDS.
+        // flags[s] is set when sweep s rotated any pair; it is read by every
+        // workgroup after the sweep's last grid barrier, so all workgroups
+        // leave the sweep loop together.
+        template <int BS, int RMAX>
+        __global__ __launch_bounds__(BS) void persistent_sweeps_f64(double *__restrict__ A, double *__restrict__ V,
+                                                                    int64_t m, int n, double eps, int max_sweeps,
+                                                                    int32_t *__restrict__ flags,
+                                                                    unsigned int *__restrict__ bar,
+                                                                    int32_t *__restrict__ sweeps_out)
+        {
+            constexpr int WAVES = BS / 64;
+            __shared__ double red[3][WAVES];
+            const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+            const int even = n + (n & 1);
+            const int half = even >> 1;
+            const int rounds = n < 2 ? 0 : even - 1;
+            const bool in_regs = m <= (int64_t)BS * RMAX;
+            int executed = 0;
+            for (int sweep = 0; sweep < max_sweeps; ++sweep)
+            {
+                int any = 0;
+                for (int r = 0; r < rounds; ++r)
+                {
+                    for (int k = blockIdx.x; k < half; k += gridDim.x)
+                    {
+                        const int p = dev::column_at(k, r, n, even);
+                        const int q = dev::column_at(even - 1 - k, r, n, even);
+                        if (p < 0 || q < 0)
+                            continue;
+                        double *ap = A + (int64_t)p * m;
+                        double *aq = A + (int64_t)q * m;
+                        double xp[RMAX], xq[RMAX];
+                        double pp = 0.0, qq = 0.0, pq = 0.0;
+                        if (in_regs)
+                        {
+#pragma unroll
+                            for (int j = 0; j < RMAX; ++j)
+                            {
+                                const int64_t i = tid + (int64_t)j * BS;
+                                const bool inb = i < m;
+                                xp[j] = inb ? ap[i] : 0.0;
+                                xq[j] = inb ? aq[i] : 0.0;
+                                pp += xp[j] * xp[j];
+                                qq += xq[j] * xq[j];
+                                pq += xp[j] * xq[j];
+                            }
+                        }
+                        else
+                        {
+                            for (int64_t i = tid; i < m; i += BS)
+                            {
+                                const double x = ap[i], y = aq[i];
+                                pp += x * x;
+                                qq += y * y;
+                                pq += x * y;
+                            }
+                        }
+                        pp = dev::wave_sum(pp);
+                        qq = dev::wave_sum(qq);
+                        pq = dev::wave_sum(pq);
+                        if (lane == 0)
+                        {
+                            red[0][wave] = pp;
+                            red[1][wave] = qq;
+                            red[2][wave] = pq;
+                        }
+                        __syncthreads();
+                        pp = red[0][0];
+                        qq = red[1][0];
+                        pq = red[2][0];
+#pragma unroll
+                        for (int w2 = 1; w2 < WAVES; ++w2)
+                        {
+                            pp += red[0][w2];
+                            qq += red[1][w2];
+                            pq += red[2][w2];
+                        }
+                        double c, s;
+                        any |= dev::rotation(pp, qq, pq, eps, c, s) ? 1 : 0;
+                        if (s != 0.0)
+                        {
+                            if (in_regs)
+                            {
+#pragma unroll
+                                for (int j = 0; j < RMAX; ++j)
+                                {
+                                    const int64_t i = tid + (int64_t)j * BS;
+                                    if (i < m)
+                                    {
+                                        ap[i] = c * xp[j] - s * xq[j];
+                                        aq[i] = s * xp[j] + c * xq[j];
+                                    }
+                                }
+                            }
+                            else
+                            {
+                                for (int64_t i = tid; i < m; i += BS)
+                                {
+                                    const double x = ap[i], y = aq[i];
+                                    ap[i] = c * x - s * y;
+                                    aq[i] = s * x + c * y;
+                                }
+                            }
+                            double *vp = V + (int64_t)p * n;
+                            double *vq = V + (int64_t)q * n;
+                            for (int i = tid; i < n; i += BS)
+                            {
+                                const double x = vp[i], y = vq[i];
+                                vp[i] = c * x - s * y;
+                                vq[i] = s * x + c * y;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    if (r == rounds - 1 && any && tid == 0)
+                        __hip_atomic_fetch_or(&flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    dev::grid_barrier(bar, gridDim.x);
+                }
+                executed = sweep + 1;
+                const int f = __hip_atomic_load(&flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (f == 0)
+                    break;
+            }
+            if (blockIdx.x == 0 && tid == 0 && sweeps_out)
+                *sweeps_out = executed;
+        }
+
+        // Tiled transpose through a padded LDS tile (32 x 33 doubles: the
+        // extra column puts the 32 elements of a tile column in distinct bank
+        // pairs).  Block 32 x 8.
+        __global__ __launch_bounds__(256) void transpose_f64(const double *__restrict__ in, double *__restrict__ out,
+                                                             int64_t rows, int64_t cols,
+                                                             const double *__restrict__ row_scale, double eps)
+        {
+            __shared__ double tile[32][33];
+            const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+            const int tx = threadIdx.x, ty = threadIdx.y;
+#pragma unroll
+            for (int k = 0; k < 32; k += 8)
+            {
+                const int64_t r = r0 + ty + k, c = c0 + tx;
+                if (r < rows && c < cols)
+                {
+                    double x = in[r * cols + c];
+                    if (row_scale)
+                    {
+                        const double sc = row_scale[r];
+                        x = (sc > eps) ? (x / sc) : 0.0;
+                    }
+                    tile[ty + k][tx] = x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < 32; k += 8)
+            {
+                const int64_t c = c0 + ty + k, r = r0 + tx;
+                if (c < cols && r < rows)
+                    out[c * rows + r] = tile[tx][ty + k];
+            }
+        }
+
+        template <int BS>
+        __global__ __launch_bounds__(BS) void column_norms_f64(const double *__restrict__ a, double *__restrict__ sigma,
+                                                               int64_t m)
+        {
+            constexpr int WAVES = BS / 64;
+            __shared__ double red[WAVES];
+            const double *col = a + (int64_t)blockIdx.x * m;
+            double acc = 0.0;
+            for (int64_t i = threadIdx.x; i < m; i += BS)
+            {
+                const double x = col[i];
+                acc += x * x;
+            }
+            acc = dev::wave_sum(acc);
+            if ((threadIdx.x & 63) == 0)
+                red[threadIdx.x >> 6] = acc;
+            __syncthreads();
+            if (threadIdx.x == 0)
+            {
+                double t = red[0];
+                for (int w = 1; w < WAVES; ++w)
+                    t += red[w];
+                sigma[blockIdx.x] = sqrt(fmax(t, 0.0));
+            }
+        }
+
+        __global__ void identity_f64(double *v, int n)
+        {
+            const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            const int64_t total = (int64_t)n * n;
+            if (idx < total)
+                v[idx] = (idx % (n + 1) == 0) ? 1.0 : 0.0;
+        }
+
+        __global__ void fill_i32(int32_t *p, int32_t value, int64_t count)
+        {
+            const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            if (idx < count)
+                p[idx] = value;
+        }
+    } // namespace
+
+    constexpr int kLdsRmax = 16;
+    constexpr int kPersistentRmax = 16;
+
+    const void *lds_batch_f64_symbol(int block)
+    {
+        switch (block)
+        {
+        case 64: return reinterpret_cast<const void *>(&lds_batch_f64<64, kLdsRmax>);
+        case 128: return reinterpret_cast<const void *>(&lds_batch_f64<128, kLdsRmax>);
+        case 256: return reinterpret_cast<const void *>(&lds_batch_f64<256, kLdsRmax>);
+        case 512: return reinterpret_cast<const void *>(&lds_batch_f64<512, kLdsRmax>);
+        default: return nullptr;
+        }
+    }
+
+    hipError_t launch_lds_batch_f64(int block, int grid, int lds_bytes, const WorkItem *items, int count,
+                                    const double *a, double *u, double *sigma, double *v, int32_t *sweeps,
+                                    double eps, int max_sweeps, unsigned int *counters, hipStream_t stream)
+    {
+        switch (block)
+        {
+        case 64:
+            lds_batch_f64<64, kLdsRmax><<<grid, 64, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
+                                                                         max_sweeps, counters);
+            break;
+        case 128:
+            lds_batch_f64<128, kLdsRmax><<<grid, 128, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
+                                                                           max_sweeps, counters);
+            break;
+        case 256:
+            lds_batch_f64<256, kLdsRmax><<<grid, 256, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
+                                                                           max_sweeps, counters);
+            break;
+        case 512:
+            lds_batch_f64<512, kLdsRmax><<<grid, 512, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
+                                                                           max_sweeps, counters);
+            break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+
+    const void *persistent_sweeps_f64_symbol(int block)
+    {
+        switch (block)
+        {
+        case 256: return reinterpret_cast<const void *>(&persistent_sweeps_f64<256, kPersistentRmax>);
+        case 512: return reinterpret_cast<const void *>(&persistent_sweeps_f64<512, kPersistentRmax>);
+        case 1024: return reinterpret_cast<const void *>(&persistent_sweeps_f64<1024, kPersistentRmax>);
+        default: return nullptr;
+        }
+    }
+
+    hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
+                                            double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
+                                            int32_t *sweeps_out, hipStream_t stream)
+    {
+        const void *fn = persistent_sweeps_f64_symbol(block);
+        if (!fn)
+            return hipErrorInvalidValue;
+        void *args[] = {&a_cm, &v_cm, &m, &n, &eps, &max_sweeps, &flags, &barrier, &sweeps_out};
+        return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+    }
+
+    hipError_t launch_transpose_f64(const double *in, double *out, int64_t rows, int64_t cols,
+                                    const double *row_scale, double eps, hipStream_t stream)
+    {
+        if (rows == 0 || cols == 0)
+            return hipSuccess;
+        dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+        transpose_f64<<<grid, dim3(32, 8), 0, stream>>>(in, out, rows, cols, row_scale, eps);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_column_norms_f64(const double *a_cm, double *sigma, int64_t m, int n, hipStream_t stream)
+    {
+        if (n == 0)
+            return hipSuccess;
+        column_norms_f64<256><<<n, 256, 0, stream>>>(a_cm, sigma, m);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_identity_f64(double *v, int n, hipStream_t stream)
+    {
+        const int64_t total = (int64_t)n * n;
+        if (total == 0)
+            return hipSuccess;
+        identity_f64<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(v, n);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_fill_i32(int32_t *p, int32_t value, int64_t count, hipStream_t stream)
+    {
+        if (count == 0)
+            return hipSuccess;
+        fill_i32<<<(unsigned)((count + 255) / 256), 256, 0, stream>>>(p, value, count);
+        return hipGetLastError();
+    }
+} // namespace jsvd

@@ -1,0 +1,125 @@
+"""MI355X one-sided Jacobi SVD: Python front end of libjsvd_amd.so.
+
+Host-array entry points mirror ``jacobi::svd::one_sided_jacobi_svd``
+(reference include/jacobi/svd/domain/jacobi_svd.cuh:25-28): row-major m x n
+input with m >= n, results U (m x n), sigma (n, unsorted, final column
+order), V (n x n) and the sweep count.  ``BatchPlan`` runs device-resident
+packed batches on the caller's HIP stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (JsvdConfig, JsvdError, JsvdInvalidArgument, LAYOUT_AUTO, LAYOUT_OFF, LAYOUT_ON,
+                   PRECISION_FP64, PRECISION_MIXED, default_config, load)
+
+__all__ = [
+    "JsvdConfig", "JsvdError", "JsvdInvalidArgument", "PRECISION_FP64", "PRECISION_MIXED", "LAYOUT_AUTO",
+    "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets",
+]
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _as_cfg(cfg: JsvdConfig | None, overrides) -> JsvdConfig:
+    if cfg is None:
+        return default_config(**overrides)
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
+    return cfg
+
+
+def svd(a, cfg: JsvdConfig | None = None, **overrides):
+    """One solve from host memory.  Returns (U, sigma, V, sweeps)."""
+    lib = load()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.ndim != 2:
+        raise ValueError("input must be a 2-D array")
+    m, n = a.shape
+    c = _as_cfg(cfg, overrides)
+    u = np.zeros((m, n))
+    s = np.zeros((n,))
+    v = np.zeros((n, n))
+    sw = ctypes.c_int32()
+    _lib.check(lib.jsvd_solve_f64(a.ctypes.data_as(_dp), m, n, ctypes.byref(c), u.ctypes.data_as(_dp),
+                                  s.ctypes.data_as(_dp), v.ctypes.data_as(_dp), ctypes.byref(sw)))
+    return u, s, v, sw.value
+
+
+def svd_batched(mats, cfg: JsvdConfig | None = None, **overrides):
+    """Many independent solves from host memory in one work-list launch."""
+    lib = load()
+    mats = [np.ascontiguousarray(x, dtype=np.float64) for x in mats]
+    count = len(mats)
+    c = _as_cfg(cfg, overrides)
+    outs = [(np.zeros(x.shape), np.zeros(x.shape[1]), np.zeros((x.shape[1], x.shape[1]))) for x in mats]
+    arr = lambda xs: (_dp * count)(*[x.ctypes.data_as(_dp) for x in xs])
+    ms = (ctypes.c_uint64 * count)(*[x.shape[0] for x in mats])
+    ns = (ctypes.c_uint64 * count)(*[x.shape[1] for x in mats])
+    sw = (ctypes.c_int32 * count)()
+    _lib.check(lib.jsvd_solve_batched_f64(count, arr(mats), ms, ns, ctypes.byref(c), arr([o[0] for o in outs]),
+                                          arr([o[1] for o in outs]), arr([o[2] for o in outs]), sw))
+    return [(o[0], o[1], o[2], int(sw[k])) for k, o in enumerate(outs)]
+
+
+def pack_offsets(ms, ns):
+    """Element offsets of the packed layout in include/jsvd_c.h: (a_off, v_off, s_off, totals)."""
+    ms = np.asarray(ms, dtype=np.int64)
+    ns = np.asarray(ns, dtype=np.int64)
+    a_sz, v_sz, s_sz = ms * ns, ns * ns, ns
+    off = lambda sz: np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64)
+    return off(a_sz), off(v_sz), off(s_sz), (int(a_sz.sum()), int(v_sz.sum()), int(s_sz.sum()))
+
+
+class BatchPlan:
+    """Device work-list for a fixed batch of shapes (``jsvd_plan``).
+
+    ``execute`` takes torch CUDA tensors (float64, packed layout) and runs on
+    the current torch stream without allocating.
+    """
+
+    def __init__(self, ms, ns, cfg: JsvdConfig | None = None, **overrides):
+        lib = load()
+        self.ms = np.asarray(ms, dtype=np.int64)
+        self.ns = np.asarray(ns, dtype=np.int64)
+        self.cfg = _as_cfg(cfg, overrides)
+        self._handle = ctypes.c_void_p()
+        _lib.check(lib.jsvd_plan_create(len(self.ms), self.ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        self.ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        ctypes.byref(self.cfg), ctypes.byref(self._handle)))
+        self.a_off, self.v_off, self.s_off, (self.a_elems, self.v_elems, self.s_elems) = pack_offsets(
+            self.ms, self.ns)
+
+    @property
+    def info(self) -> dict:
+        info = _lib.JsvdPlanInfo()
+        _lib.check(load().jsvd_plan_get_info(self._handle, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def execute(self, a, u, s, v, sweeps=None, stream=None):
+        import torch
+
+        for t, nel in ((a, self.a_elems), (u, self.a_elems), (s, self.s_elems), (v, self.v_elems)):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() >= nel):
+                raise ValueError("BatchPlan.execute needs contiguous float64 CUDA tensors of the packed sizes")
+        if sweeps is not None and not (sweeps.is_cuda and sweeps.dtype == torch.int32
+                                       and sweeps.numel() >= len(self.ms)):
+            raise ValueError("sweeps must be an int32 CUDA tensor with one slot per matrix")
+        st = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(load().jsvd_plan_execute_f64(self._handle, a.data_ptr(), u.data_ptr(), s.data_ptr(),
+                                                v.data_ptr(), sweeps.data_ptr() if sweeps is not None else None,
+                                                st.cuda_stream))
+
+    def close(self):
+        if self._handle:
+            load().jsvd_plan_destroy(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

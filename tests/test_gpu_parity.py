@@ -1,0 +1,227 @@
+"""GPU parity of the FP64 path against the CPU oracle, through the C-ABI.
+
+Tolerances (FP64 Jacobi in the reference's pair order; the dot products are
+reduced in a different order, so agreement is to rounding, not bitwise):
+  * sigma, compared position by position (the unsorted column order is part
+    of the result): max |s - s_oracle| <= 1e-12 * max(s_oracle);
+  * ||U^T U - I||_F, ||V^T V - I||_F and ||A - U S V^T||_F / ||A||_F
+    <= 1e-12 * (n / 64 + 1);
+  * sweep counts equal the oracle's within one sweep (rounding can move the
+    last threshold decision by one sweep).
+At eps = 1e-15 (the rounding floor of the cosine test) a rotation with
+zeta = (qq - pp) / (2 pq) within rounding of 0 takes t = +1 or t = -1
+depending on the last bit, which decides which of the two columns grows:
+the multiset of sigma is the same but two columns may trade places.  Batches
+at that floor therefore compare sigma sorted for every matrix and require
+the positional match for at least 90% of them.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import jsvd
+import oracle
+from jsvd import gen, metrics
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "golden_small.npz")
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _assert_parity(a, got, ref, eps, tag="", allow_permutation=False):
+    """Checks got against the oracle's ref; returns True when sigma matched position by position."""
+    u, s, v, sw = got
+    uo, so, vo, swo = ref
+    n = a.shape[1]
+    tol = 1e-12 * (n / 64 + 1)
+    scale = max(so.max(), 1e-300)
+    positional = bool(np.max(np.abs(s - so)) <= 1e-12 * scale)
+    if not allow_permutation:
+        assert positional, f"{tag} sigma (positional) differs"
+    assert metrics.sigma_errors(s, so)[0] <= 1e-12, f"{tag} sigma (sorted) differs"
+    mask = s > eps
+    assert metrics.orthogonality(u, mask) <= tol, f"{tag} U orthogonality"
+    assert metrics.orthogonality(v) <= tol, f"{tag} V orthogonality"
+    if np.linalg.norm(a) > 0:
+        assert metrics.residual(a, u, s, v) <= tol, f"{tag} residual"
+    assert abs(sw - swo) <= 1, f"{tag} sweeps {sw} vs oracle {swo}"
+    return positional
+
+
+def test_golden_fixtures():
+    d = np.load(GOLDEN)
+    for k in range(int(d["count"])):
+        a = d[f"a_{k}"]
+        _note(f"golden case {k} shape {a.shape}")
+        got = jsvd.svd(a, epsilon=1e-15, max_sweeps=30)
+        ref = (d[f"u_{k}"], d[f"s_{k}"], d[f"v_{k}"], int(d[f"sweeps_{k}"]))
+        _assert_parity(a, got, ref, 1e-15, f"golden{k}")
+        assert np.max(np.abs(np.sort(got[1])[::-1] - d[f"sigma_mp_{k}"])) <= 4e-15 * d[f"sigma_mp_{k}"][0]
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 3), (16, 8), (17, 9), (64, 33), (100, 1), (128, 64), (96, 64),
+                                   (250, 63), (40, 40)])
+def test_lds_path_against_oracle(shape):
+    _note(f"lds path shape {shape}")
+    rng = np.random.default_rng(100 + shape[0] + 1000 * shape[1])
+    a = rng.standard_normal(shape)
+    got = jsvd.svd(a, epsilon=1e-13, max_sweeps=30)
+    ref = oracle.solve(a, 1e-13, 30)
+    _assert_parity(a, got, ref, 1e-13, str(shape))
+
+
+@pytest.mark.parametrize("shape,threads", [((256, 256), 256), ((300, 150), 256), ((300, 151), 512),
+                                           ((5000, 40), 256), ((1024, 129), 1024)])
+def test_persistent_path_against_oracle(shape, threads):
+    # These shapes exceed the LDS budget, so they run the cooperative
+    # persistent kernel (5000 x 40 also exercises its row-streaming branch).
+    _note(f"persistent path shape {shape} threads {threads}")
+    rng = np.random.default_rng(7 + shape[0] + shape[1])
+    a = rng.standard_normal(shape)
+    got = jsvd.svd(a, epsilon=1e-13, max_sweeps=30, threads_per_block=threads)
+    ref = oracle.solve(a, 1e-13, 30, threads=8)
+    _assert_parity(a, got, ref, 1e-13, str(shape))
+
+
+def test_gaussian_256_at_floor_tolerance():
+    # The 256 x 256 N(0,1) workload with eps = 1e-15 and 30 sweeps.
+    _note("gaussian 256 eps 1e-15")
+    a = gen.gaussian_square(256)
+    got = jsvd.svd(a, epsilon=1e-15, max_sweeps=30)
+    ref = oracle.solve(a, 1e-15, 30, threads=8)
+    _assert_parity(a, got, ref, 1e-15, "gaussian256")
+
+
+def test_special_cases():
+    _note("special cases")
+    d = np.diag([2.0, -5.0, 1.0, 3.0])
+    u, s, v, sw = jsvd.svd(d, epsilon=1e-9, max_sweeps=30)
+    assert sw == 1
+    np.testing.assert_array_equal(s, [2.0, 5.0, 1.0, 3.0])
+    np.testing.assert_array_equal(v, np.eye(4))
+    np.testing.assert_array_equal(u, np.diag([1.0, -1.0, 1.0, 1.0]))
+
+    rng = np.random.default_rng(5)
+    z = rng.standard_normal((7, 4))
+    z[:, 1] = 0.0
+    u, s, v, sw = jsvd.svd(z, epsilon=1e-9, max_sweeps=30)
+    uo, so, vo, swo = oracle.solve(z, 1e-9, 30)
+    assert np.sum(s == 0.0) == 1 and np.all(u[:, s == 0.0] == 0.0)
+    np.testing.assert_allclose(s, so, rtol=0, atol=1e-13 * so.max())
+
+    one = np.array([[3.0], [4.0]])
+    u, s, v, sw = jsvd.svd(one)
+    assert sw == 1 and abs(s[0] - 5.0) < 1e-15 and v[0, 0] == 1.0
+
+    nan = np.eye(4, 3)
+    nan[0, 1] = np.nan
+    u, s, v, sw = jsvd.svd(nan, epsilon=1e-9, max_sweeps=5)
+    assert s[1] == 0.0 and np.all(u[:, 1] == 0.0) and s[0] == 1.0 and s[2] == 1.0
+
+    a = rng.standard_normal((30, 30))
+    assert jsvd.svd(a, epsilon=1e-15, max_sweeps=1)[3] == 1
+
+    tiny = np.vstack([np.diag([1.0, 5e-10]), np.zeros((1, 2))])
+    u, s, v, sw = jsvd.svd(tiny, epsilon=1e-9)
+    assert s[1] == 5e-10 and np.all(u[:, 1] == 0.0)
+
+
+def test_repeat_runs_are_bitwise_identical():
+    _note("determinism")
+    rng = np.random.default_rng(9)
+    for shape in [(90, 60), (300, 150)]:
+        a = rng.standard_normal(shape)
+        r1 = jsvd.svd(a, epsilon=1e-14, max_sweeps=30)
+        r2 = jsvd.svd(a, epsilon=1e-14, max_sweeps=30)
+        for x, y in zip(r1[:3], r2[:3]):
+            np.testing.assert_array_equal(x, y)
+        assert r1[3] == r2[3]
+
+
+def test_batched_host_api_against_oracle():
+    _note("batched host api, 256 uneven matrices")
+    mats, sig = gen.uneven_batch(256)
+    got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
+    # eps = 1e-13 is above the rounding floor: positions must agree exactly.
+    ref = oracle.solve_batch(mats, 1e-13, 30)
+    for k, (a, g, r) in enumerate(zip(mats, got, ref)):
+        _assert_parity(a, g, r, 1e-13, f"batch{k}")
+        assert metrics.sigma_errors(g[1], sig[k])[0] < 1e-13
+
+
+def test_batch_plan_device_full_uneven_batch():
+    # The 4096-matrix uneven batch on device buffers through jsvd_plan_*:
+    # every matrix against its constructed spectrum and the invariants, and a
+    # seeded subset against the oracle.
+    import torch
+
+    _note("device plan, 4096 uneven matrices")
+    mats, sig = gen.uneven_batch(4096)
+    flat, ms, ns = gen.pack(mats)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=1e-15, max_sweeps=30)
+    info = plan.info
+    assert info["lds_matrices"] == 4096 and info["large_matrices"] == 0
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(flat).to(dev)
+    u = torch.empty(plan.a_elems, dtype=torch.float64, device=dev)
+    s = torch.empty(plan.s_elems, dtype=torch.float64, device=dev)
+    v = torch.empty(plan.v_elems, dtype=torch.float64, device=dev)
+    sw = torch.zeros(len(ms), dtype=torch.int32, device=dev)
+    plan.execute(a, u, s, v, sw)
+    torch.cuda.synchronize()
+    u, s, v, sw = (t.cpu().numpy() for t in (u, s, v, sw))
+    worst = 0.0
+    for k in range(len(mats)):
+        m, n = int(ms[k]), int(ns[k])
+        uk = u[plan.a_off[k]:plan.a_off[k] + m * n].reshape(m, n)
+        sk = s[plan.s_off[k]:plan.s_off[k] + n]
+        vk = v[plan.v_off[k]:plan.v_off[k] + n * n].reshape(n, n)
+        rep = metrics.report(mats[k], uk, sk, vk, sig[k], 1e-15)
+        worst = max(worst, rep["sigma_err_norm"])
+        assert rep["sigma_err_norm"] < 1e-13, k
+        assert rep["orth_v"] < 1e-12 and rep["residual"] < 1e-12, k
+        assert 1 <= sw[k] <= 30
+    _note(f"worst normwise sigma error vs constructed spectrum: {worst:.3e}")
+    subset = list(range(0, 4096, 7))
+    ref = oracle.solve_batch([mats[k] for k in subset], 1e-15, 30)
+    positional = 0
+    for k, r in zip(subset, ref):
+        m, n = int(ms[k]), int(ns[k])
+        g = (u[plan.a_off[k]:plan.a_off[k] + m * n].reshape(m, n), s[plan.s_off[k]:plan.s_off[k] + n],
+             v[plan.v_off[k]:plan.v_off[k] + n * n].reshape(n, n), int(sw[k]))
+        positional += _assert_parity(mats[k], g, r, 1e-15, f"plan{k}", allow_permutation=True)
+    _note(f"positional sigma match at eps=1e-15: {positional}/{len(subset)}")
+    assert positional >= 0.9 * len(subset)
+    plan.close()
+
+
+def test_batch_plan_repeat_execution():
+    # The work-list counters reset themselves at the end of each launch: a
+    # second execution of the same plan must give identical results.
+    import torch
+
+    _note("device plan repeat")
+    mats, _ = gen.uneven_batch(300, seed=77)
+    flat, ms, ns = gen.pack(mats)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=1e-14, max_sweeps=30)
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(flat).to(dev)
+    outs = []
+    for _ in range(3):
+        u = torch.empty(plan.a_elems, dtype=torch.float64, device=dev)
+        s = torch.empty(plan.s_elems, dtype=torch.float64, device=dev)
+        v = torch.empty(plan.v_elems, dtype=torch.float64, device=dev)
+        plan.execute(a, u, s, v)
+        torch.cuda.synchronize()
+        outs.append((u.cpu(), s.cpu(), v.cpu()))
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert torch.equal(x, y)
+    plan.close()
