@@ -28,6 +28,6 @@ namespace jacobi::svd
         int layout_transpose_benchmark_sweeps = 8;
         JacobiPrecision precision = JacobiPrecision::fp64;
         int block_width = 0;
-        int refine_sweeps = 4;
+        int refine_sweeps = 0; // 0: FP64 refinement runs to convergence (at most max_sweeps)
     };
 } // namespace jacobi::svd

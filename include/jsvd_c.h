@@ -69,7 +69,7 @@ typedef struct jsvd_config
     /* --- no reference counterpart --- */
     int32_t precision;           /* JSVD_PRECISION_*; default FP64 */
     int32_t block_width;         /* 0 = plain pairs; 16 or 32 = blocked Jacobi for large matrices */
-    int32_t refine_sweeps;       /* mixed: FP64 sweep cap after the FP32 phase (default 4) */
+    int32_t refine_sweeps;       /* mixed: FP64 sweep cap after the FP32 phase; 0 (default) = max_sweeps */
 } jsvd_config;
 
 /* Fills *cfg with the reference defaults (jacobi_svd_config.cuh:17-57). */
@@ -108,10 +108,12 @@ typedef struct jsvd_plan_info
     int64_t a_elements;        /* packed A / U elements */
     int64_t v_elements;        /* packed V elements */
     int64_t sigma_elements;    /* packed sigma elements */
-    int32_t workgroup_size;    /* threads per workgroup of the work-list kernel */
-    int32_t grid_size;         /* workgroups of the work-list kernel */
-    int32_t lds_bytes;         /* dynamic LDS per workgroup */
+    int32_t workgroup_size;    /* threads per workgroup of the first (largest) bucket */
+    int32_t grid_size;         /* workgroups of that bucket's launch */
+    int32_t lds_bytes;         /* dynamic LDS per workgroup of that bucket */
     int32_t precision;
+    int32_t buckets;           /* work-list launches (shape class x LDS occupancy) */
+    int32_t reserved;
 } jsvd_plan_info;
 
 /* Builds the device work-list for count matrices of shapes m[k] x n[k].
@@ -140,6 +142,15 @@ typedef struct jsvd_autotune_report
 } jsvd_autotune_report;
 
 jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out);
+
+/* Shape-class table of the work-list kernel (no reference counterpart).
+ * Column counts are bucketed as n <= 8, 16, 32, 48, 64, 96, 128 (buckets
+ * 0..6).  jsvd_tune_class sets the class used for a bucket by plans created
+ * afterwards (-1 = default choice; an index that does not fit a shape falls
+ * back to the default) and returns the previous setting, or -2 for a bad
+ * bucket.  jsvd_autotune fills this table from measurements. */
+int32_t jsvd_tune_class(int32_t bucket, int32_t cls);
+int32_t jsvd_class_count(void);
 
 #ifdef __cplusplus
 }

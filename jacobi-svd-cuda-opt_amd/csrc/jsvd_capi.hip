@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -151,17 +152,84 @@ namespace
     };
 } // namespace
 
+namespace
+{
+    // Process-wide shape-class choices per column-count bucket, written by
+    // the autotuner (or jsvd_tune_set) and read at plan creation.  -1 keeps
+    // the default (first fitting class).
+    constexpr int kTuneBuckets = 7;
+    constexpr int64_t kTuneUpper[kTuneBuckets] = {8, 16, 32, 48, 64, 96, 128};
+    std::mutex g_tune_mu;
+    int g_tuned[kTuneBuckets] = {-1, -1, -1, -1, -1, -1, -1};
+
+    int tune_bucket(int64_t n)
+    {
+        for (int b = 0; b < kTuneBuckets; ++b)
+            if (n <= kTuneUpper[b])
+                return b;
+        return -1;
+    }
+
+    int preferred_class(int64_t m, int64_t n)
+    {
+        const int b = tune_bucket(n);
+        if (b >= 0)
+        {
+            std::lock_guard<std::mutex> lock(g_tune_mu);
+            const int c = g_tuned[b];
+            if (c >= 0 && jsvd::class_fits(c, m, n))
+                return c;
+        }
+        return jsvd::select_class_f64(m, n);
+    }
+} // namespace
+
+// Work-list counters per bucket: two per stage (FP32 sweeps,
+// orthonormalisation, B = A Q, FP64 sweeps); plain buckets use the last two.
+constexpr int kCountersPerBucket = 8;
+
+// A bucket is one persistent launch of the LDS-resident work-list kernel:
+// matrices of one shape class whose LDS needs allow the same number of
+// workgroups per CU.
+struct JsvdBucket
+{
+    int cls = -1;           // shape class, -1 = generic kernel
+    int block = 256;
+    int lds_bytes = 0;
+    int grid = 0;
+    int64_t first = 0;      // first work-list entry of the bucket
+    int64_t count = 0;
+    double work = 0.0;      // sum of n^2 (m + n), for launch order
+    int64_t max_n = 0;
+    // Rotation-log classes: one tangent log per resident workgroup.
+    double *vlog = nullptr;
+    int64_t vlog_stride = 0;
+    // Mixed precision: the FP32 sweep class and the launch shapes of the
+    // FP32, orthonormalisation and B = A Q stages; `cls` is then the class
+    // of the FP64 refinement.
+    int f32cls = -1;
+    int lds32 = 0, grid32 = 0;
+    int lds_ortho = 0, grid_ortho = 0, grid_gemm = 0;
+};
+
 struct jsvd_plan
 {
     int device = 0;
     jsvd_config cfg{};
     int64_t count = 0;
     std::vector<int64_t> m, n, a_off, v_off, s_off;
-    // LDS-resident work-list part.
+    // LDS-resident work-list part: entries grouped by bucket, largest first.
     std::vector<jsvd::WorkItem> items;
+    std::vector<JsvdBucket> buckets;
     DevBuf<jsvd::WorkItem> d_items;
-    DevBuf<unsigned int> d_counters;
-    int lds_block = 256, lds_grid = 0, lds_bytes = 0;
+    DevBuf<unsigned int> d_counters; // kCountersPerBucket per bucket
+    std::vector<std::unique_ptr<DevBuf<double>>> vlogs;
+    DevBuf<double> q_scratch;        // mixed precision: packed column-major Q
+    // Buckets beyond the first run on plan-owned streams forked from the
+    // caller's stream and joined back (event record / wait only).
+    std::vector<hipStream_t> aux;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
     // Large matrices: persistent kernel over a column-major workspace.
     std::vector<int64_t> large;
     DevBuf<double> ws_a, ws_v, ws_s;
@@ -169,6 +237,16 @@ struct jsvd_plan
     DevBuf<unsigned int> ws_bar;
     int big_block = 256;
     int64_t a_elems = 0, v_elems = 0, s_elems = 0;
+
+    ~jsvd_plan()
+    {
+        for (hipStream_t st : aux)
+            (void)hipStreamDestroy(st);
+        for (hipEvent_t e : join)
+            (void)hipEventDestroy(e);
+        if (fork)
+            (void)hipEventDestroy(fork);
+    }
 };
 
 namespace
@@ -251,7 +329,7 @@ void jsvd_config_default(jsvd_config *cfg)
     cfg->layout_bench_sweeps = 8;
     cfg->precision = JSVD_PRECISION_FP64;
     cfg->block_width = 0;
-    cfg->refine_sweeps = 4;
+    cfg->refine_sweeps = 0;
 }
 
 const char *jsvd_last_error(void)
@@ -295,7 +373,14 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     plan->v_off.resize(count);
     plan->s_off.resize(count);
     int64_t ao = 0, vo = 0, so = 0, big_mn = 0, big_nn = 0, big_n = 0;
-    int64_t max_lds = 0;
+    hipDeviceProp_t prop;
+    JSVD_HIP(hipGetDeviceProperties(&prop, plan->device));
+    const int generic_block = lds_kernel_block(cfg.threads_per_block);
+    // (class, workgroups per CU by LDS, FP32 class or -1) -> bucket
+    using GroupKey = std::pair<std::pair<int, int>, int>;
+    std::vector<std::pair<GroupKey, std::vector<jsvd::WorkItem>>> groups;
+    bool any_mixed = false;
+    static const int kSlots[] = {32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
     for (int64_t k = 0; k < count; ++k)
     {
         plan->a_off[k] = ao;
@@ -304,7 +389,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         ao += m[k] * n[k];
         vo += n[k] * n[k];
         so += n[k];
-        const int64_t need = jsvd::lds_bytes_f64(m[k], n[k]);
+        const int cls = preferred_class(m[k], n[k]);
+        const int64_t need = cls >= 0 ? jsvd::class_lds_bytes(cls, m[k], n[k]) : jsvd::lds_bytes_f64(m[k], n[k]);
         if (need <= jsvd::kMaxLdsBytes)
         {
             jsvd::WorkItem w{};
@@ -314,8 +400,30 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             w.a_off = plan->a_off[k];
             w.v_off = plan->v_off[k];
             w.s_off = plan->s_off[k];
-            plan->items.push_back(w);
-            max_lds = std::max(max_lds, need);
+            const int block = cls >= 0 ? jsvd::class_block(cls) : generic_block;
+            int slots = 1;
+            for (int sl : kSlots)
+                if (need * sl <= jsvd::kMaxLdsBytes && (block / 64) * sl <= 32)
+                {
+                    slots = sl;
+                    break;
+                }
+            // Mixed precision needs an FP32 class for the sweeps, the class
+            // kernel for the refinement and Q (n x n FP64) in LDS; other
+            // matrices of a mixed plan take the plain FP64 path.
+            int f32cls = -1;
+            if (cfg.precision == JSVD_PRECISION_MIXED && cls >= 0 &&
+                jsvd::orthonormalize_lds_bytes(n[k]) <= jsvd::kMaxLdsBytes)
+                f32cls = jsvd::select_f32_class(m[k], n[k]);
+            any_mixed = any_mixed || f32cls >= 0;
+            const GroupKey key{{cls, slots}, f32cls};
+            auto it = std::find_if(groups.begin(), groups.end(), [&](auto &g) { return g.first == key; });
+            if (it == groups.end())
+            {
+                groups.push_back({key, {}});
+                it = groups.end() - 1;
+            }
+            it->second.push_back(w);
         }
         else
         {
@@ -329,33 +437,114 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     plan->v_elems = vo;
     plan->s_elems = so;
 
-    // Longest-processing-time-first work-list: the cost of a sweep grows as
-    // n^2 (m + n), so the biggest matrices are claimed first and the small
-    // ones fill the tail.
-    std::stable_sort(plan->items.begin(), plan->items.end(), [](const jsvd::WorkItem &x, const jsvd::WorkItem &y) {
-        const double cx = double(x.n) * x.n * (x.m + x.n), cy = double(y.n) * y.n * (y.m + y.n);
-        return cx > cy;
-    });
-
-    if (!plan->items.empty())
+    // Longest-processing-time-first inside each bucket (a sweep costs about
+    // n^2 (m + n)), and the buckets with the most work are launched first.
+    auto cost = [](const jsvd::WorkItem &x) { return double(x.n) * x.n * (x.m + x.n); };
+    for (auto &g : groups)
     {
-        plan->lds_block = lds_kernel_block(cfg.threads_per_block);
-        plan->lds_bytes = static_cast<int>(max_lds);
-        const void *fn = jsvd::lds_batch_f64_symbol(plan->lds_block);
-        JSVD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, plan->lds_bytes));
+        std::stable_sort(g.second.begin(), g.second.end(),
+                         [&](const jsvd::WorkItem &x, const jsvd::WorkItem &y) { return cost(x) > cost(y); });
+        JsvdBucket b;
+        b.cls = g.first.first.first;
+        b.f32cls = g.first.second;
+        b.block = b.cls >= 0 ? jsvd::class_block(b.cls) : generic_block;
+        b.count = static_cast<int64_t>(g.second.size());
+        for (const auto &w : g.second)
+        {
+            const int64_t need = b.cls >= 0 ? jsvd::class_lds_bytes(b.cls, w.m, w.n) : jsvd::lds_bytes_f64(w.m, w.n);
+            b.lds_bytes = std::max<int>(b.lds_bytes, static_cast<int>(need));
+            b.work += cost(w);
+            b.max_n = std::max<int64_t>(b.max_n, w.n);
+            if (b.f32cls >= 0)
+            {
+                b.lds32 = std::max<int>(b.lds32, static_cast<int>(jsvd::f32_class_lds_bytes(b.f32cls, w.m, w.n)));
+                b.lds_ortho = std::max<int>(b.lds_ortho, static_cast<int>(jsvd::orthonormalize_lds_bytes(w.n)));
+            }
+        }
+        plan->buckets.push_back(b);
+    }
+    std::vector<size_t> order(groups.size());
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return plan->buckets[x].work > plan->buckets[y].work; });
+    {
+        std::vector<JsvdBucket> sorted;
+        for (size_t i : order)
+        {
+            JsvdBucket b = plan->buckets[i];
+            b.first = static_cast<int64_t>(plan->items.size());
+            plan->items.insert(plan->items.end(), groups[i].second.begin(), groups[i].second.end());
+            sorted.push_back(b);
+        }
+        plan->buckets.swap(sorted);
+    }
+
+    for (auto &b : plan->buckets)
+    {
+        const void *fn = b.cls >= 0 ? jsvd::lds_class_f64_symbol(b.cls) : jsvd::lds_batch_f64_symbol(b.block);
+        JSVD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_bytes));
         int occupancy = 0;
-        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, fn, plan->lds_block, plan->lds_bytes));
+        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, fn, b.block, b.lds_bytes));
         if (occupancy <= 0)
             return fail(JSVD_EDEVICE, "work-list kernel cannot be resident with the requested LDS size.");
-        hipDeviceProp_t prop;
-        JSVD_HIP(hipGetDeviceProperties(&prop, plan->device));
         const int64_t resident = int64_t(occupancy) * prop.multiProcessorCount;
-        plan->lds_grid = static_cast<int>(std::min<int64_t>(resident, plan->items.size()));
+        b.grid = static_cast<int>(std::min<int64_t>(resident, b.count));
+        if (b.f32cls >= 0)
+        {
+            auto resident_of = [&](const void *f, int block, int lds, int *grid) -> jsvd_status {
+                if (lds > 0)
+                    JSVD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+                int occ = 0;
+                JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, block, lds));
+                if (occ <= 0)
+                    return fail(JSVD_EDEVICE, "mixed-precision stage cannot be resident with its LDS size.");
+                *grid = static_cast<int>(std::min<int64_t>(int64_t(occ) * prop.multiProcessorCount, b.count));
+                return JSVD_OK;
+            };
+            JSVD_CHECK(resident_of(jsvd::f32_class_symbol(b.f32cls), jsvd::f32_class_block(b.f32cls), b.lds32,
+                                   &b.grid32));
+            JSVD_CHECK(resident_of(jsvd::orthonormalize_symbol(), jsvd::orthonormalize_block(), b.lds_ortho,
+                                   &b.grid_ortho));
+            JSVD_CHECK(resident_of(jsvd::gemm_aq_symbol(), jsvd::gemm_aq_block(), 0, &b.grid_gemm));
+        }
+        if (b.cls >= 0 && jsvd::class_uses_vlog(b.cls))
+        {
+            // One tangent per (sweep, round, pair slot): max_sweeps * E(E-1)/2
+            // doubles per workgroup.  Fewer workgroups if that exceeds 8 GiB.
+            const int64_t even = b.max_n + (b.max_n & 1);
+            const int64_t sweeps_logged = (b.f32cls >= 0 && cfg.refine_sweeps > 0) ? cfg.refine_sweeps : cfg.max_sweeps;
+            b.vlog_stride = sweeps_logged * even * (even - 1) / 2;
+            const int64_t cap = (int64_t(8) << 30) / 8;
+            b.grid = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(b.grid, cap / b.vlog_stride)));
+            DevBuf<double> *buf = new DevBuf<double>();
+            plan->vlogs.emplace_back(buf);
+            JSVD_CHECK(buf->alloc(size_t(b.grid) * b.vlog_stride));
+            b.vlog = buf->ptr;
+        }
+    }
+    if (!plan->items.empty())
+    {
         JSVD_CHECK(plan->d_items.alloc(plan->items.size()));
         JSVD_HIP(hipMemcpy(plan->d_items.ptr, plan->items.data(), sizeof(jsvd::WorkItem) * plan->items.size(),
                            hipMemcpyHostToDevice));
-        JSVD_CHECK(plan->d_counters.alloc(2));
-        JSVD_HIP(hipMemset(plan->d_counters.ptr, 0, sizeof(unsigned int) * 2));
+        JSVD_CHECK(plan->d_counters.alloc(kCountersPerBucket * plan->buckets.size()));
+        JSVD_HIP(hipMemset(plan->d_counters.ptr, 0, sizeof(unsigned int) * kCountersPerBucket * plan->buckets.size()));
+        if (any_mixed)
+            JSVD_CHECK(plan->q_scratch.alloc(plan->v_elems));
+        const size_t streams = std::min<size_t>(plan->buckets.size(), 4);
+        if (streams > 1)
+        {
+            JSVD_HIP(hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming));
+            for (size_t i = 1; i < streams; ++i)
+            {
+                hipStream_t st;
+                JSVD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+                plan->aux.push_back(st);
+                hipEvent_t e;
+                JSVD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                plan->join.push_back(e);
+            }
+        }
     }
     if (!plan->large.empty())
     {
@@ -387,11 +576,71 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
     if (dev != p->device)
         return fail(JSVD_EINVAL, "plan was created on another device.");
     hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
-    if (!p->items.empty())
+    if (!p->aux.empty())
     {
-        JSVD_HIP(jsvd::launch_lds_batch_f64(p->lds_block, p->lds_grid, p->lds_bytes, p->d_items.ptr,
-                                            static_cast<int>(p->items.size()), d_a, d_u, d_sigma, d_v, d_sweeps,
-                                            p->cfg.epsilon, p->cfg.max_sweeps, p->d_counters.ptr, stream));
+        JSVD_HIP(hipEventRecord(p->fork, stream));
+        for (hipStream_t st : p->aux)
+            JSVD_HIP(hipStreamWaitEvent(st, p->fork, 0));
+    }
+    for (size_t b = 0; b < p->buckets.size(); ++b)
+    {
+        const JsvdBucket &bk = p->buckets[b];
+        const size_t lane = b % (p->aux.size() + 1);
+        hipStream_t st = lane == 0 ? stream : p->aux[lane - 1];
+        const jsvd::WorkItem *items = p->d_items.ptr + bk.first;
+        unsigned int *ctr = p->d_counters.ptr + kCountersPerBucket * b;
+        const bool mixed = bk.f32cls >= 0;
+        if (mixed)
+        {
+            // FP32 sweeps -> Q = orth(V32) -> B = A Q (into the U buffer),
+            // then the FP64 refinement below starts from B and Q.
+            jsvd::MixedArgs margs{};
+            margs.items = items;
+            margs.count = static_cast<int>(bk.count);
+            margs.a = d_a;
+            margs.q = p->q_scratch.ptr;
+            margs.u = d_u;
+            margs.sweeps = d_sweeps;
+            margs.eps = p->cfg.epsilon;
+            margs.max_sweeps = p->cfg.max_sweeps;
+            margs.counters = ctr;
+            JSVD_HIP(jsvd::launch_mixed_sweeps_f32(bk.f32cls, bk.grid32, bk.lds32, margs, st));
+            JSVD_HIP(jsvd::launch_orthonormalize_f64(bk.grid_ortho, bk.lds_ortho, margs, st));
+            JSVD_HIP(jsvd::launch_gemm_aq_f64(bk.grid_gemm, margs, st));
+        }
+        if (bk.cls >= 0)
+        {
+            jsvd::ClassArgs args{};
+            args.items = items;
+            args.count = static_cast<int>(bk.count);
+            args.a = d_a;
+            args.u = d_u;
+            args.sigma = d_sigma;
+            args.v = d_v;
+            args.sweeps = d_sweeps;
+            args.eps = p->cfg.epsilon;
+            args.max_sweeps = p->cfg.max_sweeps;
+            args.vlog = bk.vlog;
+            args.vlog_stride = bk.vlog_stride;
+            args.counters = ctr + 6;
+            if (mixed)
+            {
+                args.b_cm = d_u;
+                args.q_init = p->q_scratch.ptr;
+                args.add_sweeps = 1;
+                args.max_sweeps = p->cfg.refine_sweeps > 0 ? p->cfg.refine_sweeps : p->cfg.max_sweeps;
+            }
+            JSVD_HIP(jsvd::launch_lds_class_f64(bk.cls, bk.grid, bk.lds_bytes, args, st));
+        }
+        else
+            JSVD_HIP(jsvd::launch_lds_batch_f64(bk.block, bk.grid, bk.lds_bytes, items, static_cast<int>(bk.count),
+                                                d_a, d_u, d_sigma, d_v, d_sweeps, p->cfg.epsilon, p->cfg.max_sweeps,
+                                                ctr, st));
+    }
+    for (size_t i = 0; i < p->aux.size(); ++i)
+    {
+        JSVD_HIP(hipEventRecord(p->join[i], p->aux[i]));
+        JSVD_HIP(hipStreamWaitEvent(stream, p->join[i], 0));
     }
     for (int64_t k : p->large)
         JSVD_CHECK(plan_large(*p, k, d_a, d_u, d_sigma, d_v, d_sweeps, stream));
@@ -408,9 +657,10 @@ jsvd_status jsvd_plan_get_info(const jsvd_plan *p, jsvd_plan_info *info)
     info->a_elements = p->a_elems;
     info->v_elements = p->v_elems;
     info->sigma_elements = p->s_elems;
-    info->workgroup_size = p->lds_block;
-    info->grid_size = p->lds_grid;
-    info->lds_bytes = p->lds_bytes;
+    info->workgroup_size = p->buckets.empty() ? 0 : p->buckets[0].block;
+    info->grid_size = p->buckets.empty() ? 0 : p->buckets[0].grid;
+    info->lds_bytes = p->buckets.empty() ? 0 : p->buckets[0].lds_bytes;
+    info->buckets = static_cast<int32_t>(p->buckets.size());
     info->precision = p->cfg.precision;
     return JSVD_OK;
 }
@@ -500,12 +750,120 @@ jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out)
     if (cfg)
         c = *cfg;
     JSVD_CHECK(validate_config(c));
+    // The benchmark mirrors the reference's (src/domain/layout_transpose_autotune.cu:78-126):
+    // deterministic sin/cos matrices, a capped sweep count, one warm-up and
+    // `layout_bench_reps` timed repetitions per candidate.  What it tunes here
+    // is the shape class (workgroup size and lanes per column pair) of each
+    // column-count bucket; the layout thresholds are reported unchanged
+    // because the solver always works on a column-major copy.
+    jsvd_config bench = c;
+    bench.max_sweeps = std::min(c.max_sweeps, std::max(1, c.layout_bench_sweeps));
+    bench.precision = JSVD_PRECISION_FP64;
     out->executed = 1;
     out->recommended_min_columns = c.layout_min_columns;
     out->recommended_min_elements = c.layout_min_elements;
     out->estimated_best_speedup = 1.0;
     out->sample_count = 0;
+
+    DeviceContext *ctx = nullptr;
+    JSVD_CHECK(device_context(&ctx));
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0, e1;
+    JSVD_HIP(hipEventCreate(&e0));
+    JSVD_HIP(hipEventCreate(&e1));
+    int64_t lo = 0;
+    for (int b = 0; b < kTuneBuckets; ++b)
+    {
+        const int64_t hi = kTuneUpper[b];
+        constexpr int64_t kCount = 512;
+        std::vector<int64_t> ms(kCount), ns(kCount);
+        for (int64_t k = 0; k < kCount; ++k)
+        {
+            ns[k] = lo + 1 + (k * 7919) % (hi - lo);
+            ms[k] = ns[k] + (k * 104729) % (ns[k] + 1);
+        }
+        const int64_t nmax = hi, mmax = 2 * hi;
+        std::vector<int> cands;
+        for (int cls = 0; cls < jsvd::class_count(); ++cls)
+            if (jsvd::class_fits(cls, mmax, nmax) && jsvd::class_fits(cls, lo + 1, lo + 1))
+                cands.push_back(cls);
+        lo = hi;
+        if (cands.size() < 2)
+            continue;
+        int64_t total = 0, vtotal = 0, stotal = 0;
+        for (int64_t k = 0; k < kCount; ++k)
+        {
+            total += ms[k] * ns[k];
+            vtotal += ns[k] * ns[k];
+            stotal += ns[k];
+        }
+        std::vector<double> host(total);
+        for (int64_t k = 0, off = 0; k < kCount; ++k)
+            for (int64_t i = 0; i < ms[k]; ++i)
+                for (int64_t j = 0; j < ns[k]; ++j)
+                    host[off++] = std::sin(0.013 * double((i + 1) * (j + 1))) +
+                                  std::cos(0.017 * double((i + 3) * (j + 5 + k)));
+        JSVD_CHECK(ctx->a.reserve(total));
+        JSVD_CHECK(ctx->u.reserve(total));
+        JSVD_CHECK(ctx->v.reserve(vtotal));
+        JSVD_CHECK(ctx->s.reserve(stotal));
+        JSVD_HIP(hipMemcpyAsync(ctx->a.ptr, host.data(), sizeof(double) * total, hipMemcpyHostToDevice, st));
+        double t_default = -1.0, t_best = 1e300;
+        int best = -1;
+        for (int cls : cands)
+        {
+            {
+                std::lock_guard<std::mutex> tl(g_tune_mu);
+                g_tuned[b] = cls;
+            }
+            jsvd_plan *plan = nullptr;
+            JSVD_CHECK(jsvd_plan_create(kCount, ms.data(), ns.data(), &bench, &plan));
+            std::unique_ptr<jsvd_plan, void (*)(jsvd_plan *)> guard(plan, jsvd_plan_destroy);
+            JSVD_CHECK(jsvd_plan_execute_f64(plan, ctx->a.ptr, ctx->u.ptr, ctx->s.ptr, ctx->v.ptr, nullptr, st));
+            float ms_total = 0.0f;
+            JSVD_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < c.layout_bench_reps; ++r)
+                JSVD_CHECK(jsvd_plan_execute_f64(plan, ctx->a.ptr, ctx->u.ptr, ctx->s.ptr, ctx->v.ptr, nullptr, st));
+            JSVD_HIP(hipEventRecord(e1, st));
+            JSVD_HIP(hipEventSynchronize(e1));
+            JSVD_HIP(hipEventElapsedTime(&ms_total, e0, e1));
+            const double t = double(ms_total) / c.layout_bench_reps;
+            out->sample_count += 1;
+            if (cls == jsvd::select_class_f64(mmax, nmax))
+                t_default = t;
+            if (t < t_best)
+            {
+                t_best = t;
+                best = cls;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> tl(g_tune_mu);
+            g_tuned[b] = best;
+        }
+        if (t_default > 0.0 && t_best > 0.0)
+            out->estimated_best_speedup = std::max(out->estimated_best_speedup, t_default / t_best);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return JSVD_OK;
+}
+
+int32_t jsvd_tune_class(int32_t bucket, int32_t cls)
+{
+    if (bucket < 0 || bucket >= kTuneBuckets)
+        return -2;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const int32_t prev = g_tuned[bucket];
+    if (cls >= -1 && cls < jsvd::class_count())
+        g_tuned[bucket] = cls;
+    return prev;
+}
+
+int32_t jsvd_class_count(void)
+{
+    return jsvd::class_count();
 }
 
 } // extern "C"

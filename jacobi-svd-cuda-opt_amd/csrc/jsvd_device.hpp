@@ -56,6 +56,43 @@ namespace jsvd::dev
         return x;
     }
 
+    // Cross-lane move of a double through DPP (two 32-bit halves).  CTRL is a
+    // GFX9 DPP control: 0xB1 = quad_perm(1,0,3,2), 0x4E = quad_perm(2,3,0,1),
+    // 0x141 = row_half_mirror (lane i <- 7-i in each group of 8),
+    // 0x140 = row_mirror (lane i <- 15-i in each row of 16).
+    template <int CTRL>
+    __device__ __forceinline__ double dpp_f64(double x)
+    {
+        const long long bits = __double_as_longlong(x);
+        const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(bits & 0xffffffffll), CTRL, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(bits >> 32), CTRL, 0xF, 0xF, true);
+        return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                    (static_cast<long long>(static_cast<unsigned int>(lo))));
+    }
+
+    // All-reduce over aligned groups of G lanes (compile-time G <= 64).  The
+    // first four levels are DPP moves (no LDS round trip): after the xor-1 and
+    // xor-2 levels every lane of a quad holds the quad sum, so the mirror
+    // patterns pair each lane with one in the other half of its 8- and
+    // 16-lane group.  Wider groups fall back to lane shuffles.
+    template <int G>
+    __device__ __forceinline__ double group_sum_dpp(double x)
+    {
+        if constexpr (G >= 2)
+            x += dpp_f64<0xB1>(x);
+        if constexpr (G >= 4)
+            x += dpp_f64<0x4E>(x);
+        if constexpr (G >= 8)
+            x += dpp_f64<0x141>(x);
+        if constexpr (G >= 16)
+            x += dpp_f64<0x140>(x);
+        if constexpr (G >= 32)
+            x += __shfl_xor(x, 16, 64);
+        if constexpr (G >= 64)
+            x += __shfl_xor(x, 32, 64);
+        return x;
+    }
+
     template <typename T>
     __device__ __forceinline__ T wave_sum(T x)
     {

@@ -38,11 +38,65 @@ namespace jsvd
                                     double eps, int max_sweeps, unsigned int *counters, hipStream_t stream);
     const void *lds_batch_f64_symbol(int block);
 
-    hipError_t launch_lds_batch_mixed(int block, int grid, int lds_bytes, const WorkItem *items, int count,
-                                      const double *a, double *u, double *sigma, double *v, int32_t *sweeps,
-                                      double eps, int max_sweeps, int refine_sweeps, unsigned int *counters,
-                                      hipStream_t stream);
-    const void *lds_batch_mixed_symbol(int block);
+    // Arguments of one launch of the shape-class work-list kernel.
+    struct ClassArgs
+    {
+        const WorkItem *items;
+        int count;
+        const double *a;      // packed row-major inputs (offset a_off)
+        const double *b_cm;   // optional packed column-major start matrices (offset a_off); replaces a
+        const double *q_init; // optional packed column-major initial V (offset v_off); identity if null
+        double *u;
+        double *sigma;
+        double *v;
+        int32_t *sweeps;
+        int add_sweeps;       // 1: add this solve's sweeps to the slot instead of storing them
+        double eps;
+        int max_sweeps;
+        double *vlog;         // rotation-log classes: per-workgroup tangent log
+        int64_t vlog_stride;  // doubles per workgroup log
+        unsigned int *counters;
+    };
+
+    // Shape classes of the LDS-resident kernel with compile-time lane groups.
+    // select_class_f64 returns -1 when no class fits (the generic kernel then
+    // takes the matrix).
+    int select_class_f64(int64_t m, int64_t n);
+    int class_count();
+    bool class_fits(int cls, int64_t m, int64_t n);
+    bool class_uses_vlog(int cls);
+    int64_t class_lds_bytes(int cls, int64_t m, int64_t n);
+    int class_block(int cls);
+    const void *lds_class_f64_symbol(int cls);
+    hipError_t launch_lds_class_f64(int cls, int grid, int lds_bytes, const ClassArgs &args, hipStream_t stream);
+
+    // Mixed-precision stages (FP32 sweeps, FP64 re-orthonormalisation of V,
+    // B = A Q on FP64 MFMA); the FP64 refinement is the class kernel.
+    struct MixedArgs
+    {
+        const WorkItem *items;
+        int count;
+        const double *a;          // packed row-major A (offset a_off)
+        double *q;                // packed column-major Q scratch (offset v_off)
+        double *u;                // U buffer, holds B column-major between stages
+        int32_t *sweeps;
+        double eps;
+        int max_sweeps;
+        unsigned int *counters;   // six: two per stage
+    };
+    int select_f32_class(int64_t m, int64_t n);
+    bool f32_class_fits(int cls, int64_t m, int64_t n);
+    int64_t f32_class_lds_bytes(int cls, int64_t m, int64_t n);
+    int f32_class_block(int cls);
+    const void *f32_class_symbol(int cls);
+    hipError_t launch_mixed_sweeps_f32(int cls, int grid, int lds_bytes, const MixedArgs &args, hipStream_t stream);
+    int64_t orthonormalize_lds_bytes(int64_t n);
+    const void *orthonormalize_symbol();
+    int orthonormalize_block();
+    hipError_t launch_orthonormalize_f64(int grid, int lds_bytes, const MixedArgs &args, hipStream_t stream);
+    const void *gemm_aq_symbol();
+    int gemm_aq_block();
+    hipError_t launch_gemm_aq_f64(int grid, const MixedArgs &args, hipStream_t stream);
 
     // Persistent plain-FP64 sweep kernel over one column-major matrix.
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,

@@ -1,0 +1,465 @@
+// jsvd_kernels_class.hip -- LDS-resident FP64 work-list kernels with
+// compile-time shape classes (the batched hot path).
+//
+// One workgroup solves one matrix at a time, claimed from a device work-list:
+// A (and V) are loaded into LDS once, every sweep runs with workgroup
+// barriers only, the convergence decision is taken on the device and U,
+// sigma, V are written once.
+//
+// Layout: columns are stored with a leading dimension padded with zero rows
+// to a multiple of 2G, so a lane moves two consecutive rows with one 16-byte
+// LDS access and the G lanes of a pair read 16*G contiguous bytes (no bank
+// conflicts).  Lane l of a group owns row pairs l, l+G, ... of both columns,
+// kept in registers between the dots and the update: every element crosses
+// LDS once in and once out per round.  Group sums use DPP.
+//
+// Two ways to carry V:
+//  * in LDS next to A (classes with vlog = false);
+//  * as a rotation log (vlog = true): for matrices where A fits in LDS but
+//    A and V together do not.  Each pair writes its tangent t (0 = no
+//    rotation) to a per-workgroup log in HBM; after U and sigma are written,
+//    V is rebuilt in LDS by replaying the log in the same order.  c and s are
+//    recomputed from t with the same expressions, so the replay applies
+//    bit-identical rotations.
+//
+// Arithmetic per pair follows the reference kernel
+// (src/domain/jacobi_rotation_kernels.cuh:276-353): dots, the
+// |pq| > eps*sqrt(pp*qq) test, Rutishauser's rotation, then
+// a_p <- c a_p - s a_q, a_q <- s a_p + c a_q on A and V, in the reference's
+// circle order.
+#include "jsvd_device.hpp"
+#include "jsvd_kernels.hpp"
+
+#include <algorithm>
+
+namespace jsvd
+{
+    namespace
+    {
+        struct ClassSpec
+        {
+            int block, group, rmax2, vmax2;
+            bool vlog;
+        };
+
+        // Shape classes.  The default for a shape is the first class that
+        // fits; the autotuner may choose another fitting class per column-count
+        // bucket.  rmax2 / vmax2 are register row pairs per lane for A and V.
+        constexpr ClassSpec kClasses[] = {
+            {64, 16, 1, 1, false},  // 0: n <= 8, m <= 32: one wave, 16 lanes per pair
+            {64, 8, 2, 1, false},   // 1: n <= 16, m <= 32
+            {64, 4, 8, 4, false},   // 2: n <= 32, m <= 64
+            {512, 16, 4, 2, false}, // 3: n <= 64, m <= 128: eight waves
+            {256, 8, 8, 4, false},  // 4: n <= 64, m <= 128: four waves
+            {512, 16, 8, 4, false}, // 5: n <= 128, m <= 256
+            {1024, 16, 4, 4, true}, // 6: n <= 128, m <= 128, V by rotation log
+            {512, 16, 8, 8, true},  // 7: n <= 256, m <= 256, V by rotation log
+        };
+        constexpr int kClassCount = sizeof(kClasses) / sizeof(kClasses[0]);
+
+        __host__ __device__ __forceinline__ int padded_rows(int rows, int G)
+        {
+            return ((rows + 2 * G - 1) / (2 * G)) * (2 * G);
+        }
+
+        // Rotation of one pair held by G lanes.  Returns 1 when the pair counts
+        // as rotated (reference flag semantics).  With VLOG the group's lane 0
+        // records t (0 when nothing is applied) at *tlog and V is not touched.
+        template <int G, int RMAX2, int VMAX2, bool VLOG>
+        __device__ __forceinline__ int pair_class_f64(double2 *__restrict__ ap, double2 *__restrict__ aq,
+                                                      double2 *__restrict__ vp, double2 *__restrict__ vq, int rows2,
+                                                      int vrows2, int lane, double eps, double *__restrict__ tlog)
+        {
+            double2 xp[RMAX2], xq[RMAX2];
+            double pp = 0.0, qq = 0.0, pq = 0.0;
+#pragma unroll
+            for (int j = 0; j < RMAX2; ++j)
+            {
+                if (j < rows2)
+                {
+                    const int i = lane + j * G;
+                    xp[j] = ap[i];
+                    xq[j] = aq[i];
+                    pp += xp[j].x * xp[j].x + xp[j].y * xp[j].y;
+                    qq += xq[j].x * xq[j].x + xq[j].y * xq[j].y;
+                    pq += xp[j].x * xq[j].x + xp[j].y * xq[j].y;
+                }
+            }
+            pp = dev::group_sum_dpp<G>(pp);
+            qq = dev::group_sum_dpp<G>(qq);
+            pq = dev::group_sum_dpp<G>(pq);
+            // Threshold test first (reference jacobi_rotation_kernels.cuh:310-312),
+            // then the V loads, whose latency overlaps the rotation parameters.
+            const double rhs = eps * sqrt(fmax(pp, 0.0) * fmax(qq, 0.0));
+            const double apq = fabs(pq);
+            if (!(apq > rhs && apq > 1.0e-300))
+            {
+                if (VLOG && lane == 0)
+                    *tlog = 0.0;
+                return 0;
+            }
+            double2 yp[VLOG ? 1 : VMAX2], yq[VLOG ? 1 : VMAX2];
+            if constexpr (!VLOG)
+            {
+#pragma unroll
+                for (int j = 0; j < VMAX2; ++j)
+                {
+                    if (j < vrows2)
+                    {
+                        const int i = lane + j * G;
+                        yp[j] = vp[i];
+                        yq[j] = vq[i];
+                    }
+                }
+            }
+            const double zeta = (qq - pp) / (2.0 * pq);
+            const double root = sqrt(1.0 + zeta * zeta);
+            const double t = (zeta >= 0.0) ? (1.0 / (zeta + root)) : (-1.0 / (-zeta + root));
+            const double c = 1.0 / sqrt(1.0 + t * t);
+            const double s = t * c;
+            if (VLOG && lane == 0)
+                *tlog = (s != 0.0) ? t : 0.0;
+            if (s != 0.0)
+            {
+#pragma unroll
+                for (int j = 0; j < RMAX2; ++j)
+                {
+                    if (j < rows2)
+                    {
+                        const int i = lane + j * G;
+                        double2 np, nq;
+                        np.x = c * xp[j].x - s * xq[j].x;
+                        np.y = c * xp[j].y - s * xq[j].y;
+                        nq.x = s * xp[j].x + c * xq[j].x;
+                        nq.y = s * xp[j].y + c * xq[j].y;
+                        ap[i] = np;
+                        aq[i] = nq;
+                    }
+                }
+                if constexpr (!VLOG)
+                {
+#pragma unroll
+                    for (int j = 0; j < VMAX2; ++j)
+                    {
+                        if (j < vrows2)
+                        {
+                            const int i = lane + j * G;
+                            double2 np, nq;
+                            np.x = c * yp[j].x - s * yq[j].x;
+                            np.y = c * yp[j].y - s * yq[j].y;
+                            nq.x = s * yp[j].x + c * yq[j].x;
+                            nq.y = s * yp[j].y + c * yq[j].y;
+                            vp[i] = np;
+                            vq[i] = nq;
+                        }
+                    }
+                }
+            }
+            return 1;
+        }
+
+        // Replays one logged rotation on the V columns of a pair.
+        template <int G, int VMAX2>
+        __device__ __forceinline__ void replay_pair_f64(double2 *__restrict__ vp, double2 *__restrict__ vq,
+                                                        int vrows2, int lane, double t)
+        {
+            if (t == 0.0)
+                return;
+            const double c = 1.0 / sqrt(1.0 + t * t);
+            const double s = t * c;
+            if (s == 0.0)
+                return;
+            double2 yp[VMAX2], yq[VMAX2];
+#pragma unroll
+            for (int j = 0; j < VMAX2; ++j)
+            {
+                if (j < vrows2)
+                {
+                    const int i = lane + j * G;
+                    yp[j] = vp[i];
+                    yq[j] = vq[i];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < VMAX2; ++j)
+            {
+                if (j < vrows2)
+                {
+                    const int i = lane + j * G;
+                    double2 np, nq;
+                    np.x = c * yp[j].x - s * yq[j].x;
+                    np.y = c * yp[j].y - s * yq[j].y;
+                    nq.x = s * yp[j].x + c * yq[j].x;
+                    nq.y = s * yp[j].y + c * yq[j].y;
+                    vp[i] = np;
+                    vq[i] = nq;
+                }
+            }
+        }
+
+        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG>
+        __device__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
+        {
+            constexpr int GROUPS = BS / G;
+            const int m = w.m, n = w.n, tid = threadIdx.x;
+            const int lda = padded_rows(m, G), ldv = padded_rows(n, G);
+            double *A = lds;                              // column-major, lda rows (zero padded)
+            double *V = VLOG ? lds : lds + lda * n;       // column-major, ldv rows (zero padded)
+            double *S = (VLOG ? lds + max(lda, ldv) * n : V + ldv * n); // sigma scratch
+            if (args.b_cm)
+            {
+                const double *bg = args.b_cm + w.a_off; // column-major m x n start matrix
+                for (int idx = tid; idx < lda * n; idx += BS)
+                {
+                    const int j = idx / lda, i = idx - j * lda;
+                    A[idx] = i < m ? bg[j * m + i] : 0.0;
+                }
+            }
+            else
+            {
+                const double *ag = args.a + w.a_off; // row-major m x n input
+                for (int idx = tid; idx < lda * n; idx += BS)
+                {
+                    const int j = idx / lda, i = idx - j * lda;
+                    A[idx] = i < m ? ag[i * n + j] : 0.0;
+                }
+            }
+            auto init_v = [&]() {
+                for (int idx = tid; idx < ldv * n; idx += BS)
+                {
+                    const int j = idx / ldv, i = idx - j * ldv;
+                    double x = (i == j) ? 1.0 : 0.0;
+                    if (args.q_init)
+                        x = i < n ? args.q_init[w.v_off + j * n + i] : 0.0;
+                    V[idx] = x;
+                }
+            };
+            if constexpr (!VLOG)
+                init_v();
+            __syncthreads();
+
+            const int even = n + (n & 1);
+            const int half = even >> 1;
+            const int rounds = n < 2 ? 0 : even - 1;
+            const int gid = tid / G, lane = tid & (G - 1);
+            const int rows2 = lda / (2 * G);
+            const int vrows2 = ldv / (2 * G);
+            double2 *A2 = reinterpret_cast<double2 *>(A);
+            double2 *V2 = reinterpret_cast<double2 *>(V);
+            const int lda2 = lda / 2, ldv2 = ldv / 2;
+            double *tlog = VLOG ? args.vlog + (int64_t)blockIdx.x * args.vlog_stride : nullptr;
+            double dummy_t;
+            int executed = 0;
+            for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
+            {
+                int any = 0;
+                for (int r = 0; r < rounds; ++r)
+                {
+                    for (int k = gid; k < half; k += GROUPS)
+                    {
+                        const int p = dev::column_at(k, r, n, even);
+                        const int q = dev::column_at(even - 1 - k, r, n, even);
+                        if (p < 0 || q < 0)
+                            continue;
+                        double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + k : &dummy_t;
+                        any |= pair_class_f64<G, RMAX2, VMAX2, VLOG>(A2 + p * lda2, A2 + q * lda2, V2 + p * ldv2,
+                                                                     V2 + q * ldv2, rows2, vrows2, lane, args.eps,
+                                                                     slot);
+                    }
+                    __syncthreads();
+                }
+                executed = sweep + 1;
+                if (!__syncthreads_or(any))
+                    break;
+            }
+
+            // U / Sigma epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
+            // sigma_j = ||a_j||; U_ij = a_ij / sigma_j if sigma_j > eps else 0.
+            for (int j = gid; j < n; j += GROUPS)
+            {
+                double acc = 0.0;
+                for (int i = lane; i < m; i += G)
+                {
+                    const double x = A[j * lda + i];
+                    acc += x * x;
+                }
+                acc = dev::group_sum_dpp<G>(acc);
+                if (lane == 0)
+                {
+                    const double sj = sqrt(fmax(acc, 0.0));
+                    S[j] = sj;
+                    args.sigma[w.s_off + j] = sj;
+                }
+            }
+            __syncthreads();
+            double *ug = args.u + w.a_off;
+            const int mn = m * n, nn = n * n;
+            for (int idx = tid; idx < mn; idx += BS)
+            {
+                const int i = idx / n, j = idx - i * n;
+                const double sj = S[j];
+                ug[idx] = (sj > args.eps) ? (A[j * lda + i] / sj) : 0.0;
+            }
+            if constexpr (VLOG)
+            {
+                // A is no longer needed: rebuild V in its place from the log.
+                __syncthreads();
+                init_v();
+                __syncthreads();
+                for (int sweep = 0; sweep < executed; ++sweep)
+                {
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        for (int k = gid; k < half; k += GROUPS)
+                        {
+                            const int p = dev::column_at(k, r, n, even);
+                            const int q = dev::column_at(even - 1 - k, r, n, even);
+                            if (p < 0 || q < 0)
+                                continue;
+                            // Agent-scope load: bypasses the CU's vector L1, which may
+                            // still hold this slot's entry from the previous matrix.
+                            const unsigned long long bits = __hip_atomic_load(
+                                reinterpret_cast<unsigned long long *>(tlog + ((int64_t)sweep * rounds + r) * half + k),
+                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            const double t = __longlong_as_double(static_cast<long long>(bits));
+                            replay_pair_f64<G, VMAX2>(V2 + p * ldv2, V2 + q * ldv2, vrows2, lane, t);
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+            double *vg = args.v + w.v_off;
+            for (int idx = tid; idx < nn; idx += BS)
+            {
+                const int i = idx / n, j = idx - i * n;
+                vg[idx] = V[j * ldv + i];
+            }
+            if (tid == 0 && args.sweeps)
+            {
+                if (args.add_sweeps)
+                    args.sweeps[w.index] += executed;
+                else
+                    args.sweeps[w.index] = executed;
+            }
+        }
+
+        // counters[0]: next work-list entry; counters[1]: workgroups finished.
+        // The last workgroup to finish resets both, so the next launch on the
+        // stream starts from a clean work-list without a memset.
+        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG>
+        __global__ __launch_bounds__(BS) void lds_class_f64(ClassArgs args)
+        {
+            extern __shared__ double lds[];
+            __shared__ int s_item;
+            for (;;)
+            {
+                if (threadIdx.x == 0)
+                    s_item = (int)atomicAdd(&args.counters[0], 1u);
+                __syncthreads();
+                const int item = s_item;
+                if (item >= args.count)
+                    break;
+                const WorkItem w = args.items[item];
+                solve_class_f64<BS, G, RMAX2, VMAX2, VLOG>(w, lds, args);
+                __syncthreads();
+            }
+            if (threadIdx.x == 0)
+            {
+                const unsigned int done = atomicAdd(&args.counters[1], 1u);
+                if (done == gridDim.x - 1)
+                {
+                    __hip_atomic_store(&args.counters[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&args.counters[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+
+        template <int I>
+        const void *class_symbol_t()
+        {
+            constexpr ClassSpec c = kClasses[I];
+            return reinterpret_cast<const void *>(&lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog>);
+        }
+
+        template <int I>
+        void class_launch_t(int grid, int lds_bytes, const ClassArgs &args, hipStream_t stream)
+        {
+            constexpr ClassSpec c = kClasses[I];
+            lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog><<<grid, c.block, lds_bytes, stream>>>(args);
+        }
+
+        using LaunchFn = void (*)(int, int, const ClassArgs &, hipStream_t);
+        const LaunchFn kClassLaunch[] = {class_launch_t<0>, class_launch_t<1>, class_launch_t<2>, class_launch_t<3>,
+                                         class_launch_t<4>, class_launch_t<5>, class_launch_t<6>, class_launch_t<7>};
+        const void *(*const kClassSymbol[])() = {class_symbol_t<0>, class_symbol_t<1>, class_symbol_t<2>,
+                                                 class_symbol_t<3>, class_symbol_t<4>, class_symbol_t<5>,
+                                                 class_symbol_t<6>, class_symbol_t<7>};
+        static_assert(sizeof(kClassLaunch) / sizeof(kClassLaunch[0]) == kClassCount);
+        static_assert(sizeof(kClassSymbol) / sizeof(kClassSymbol[0]) == kClassCount);
+    } // namespace
+
+    int class_count()
+    {
+        return kClassCount;
+    }
+
+    bool class_uses_vlog(int cls)
+    {
+        return cls >= 0 && cls < kClassCount && kClasses[cls].vlog;
+    }
+
+    int64_t class_lds_bytes(int cls, int64_t m, int64_t n)
+    {
+        const int g = kClasses[cls].group;
+        const int64_t lda = ((m + 2 * g - 1) / (2 * g)) * (2 * g);
+        const int64_t ldv = ((n + 2 * g - 1) / (2 * g)) * (2 * g);
+        if (kClasses[cls].vlog)
+            return (std::max(lda, ldv) * n + n) * 8;
+        return (lda * n + ldv * n + n) * 8;
+    }
+
+    bool class_fits(int cls, int64_t m, int64_t n)
+    {
+        if (cls < 0 || cls >= kClassCount)
+            return false;
+        const ClassSpec &c = kClasses[cls];
+        const int64_t half = (n + 1) / 2;
+        const int64_t groups = c.block / c.group;
+        // One wave: one pair per group per round.  Multi-wave classes may
+        // loop over pairs (up to two passes, four with the rotation log).
+        const int64_t passes = c.block == 64 ? 1 : (c.vlog ? 4 : 2);
+        if (half > passes * groups)
+            return false;
+        const int64_t lda = ((m + 2 * c.group - 1) / (2 * c.group)) * (2 * c.group);
+        const int64_t ldv = ((n + 2 * c.group - 1) / (2 * c.group)) * (2 * c.group);
+        if (lda / (2 * c.group) > c.rmax2 || ldv / (2 * c.group) > c.vmax2)
+            return false;
+        return class_lds_bytes(cls, m, n) <= kMaxLdsBytes;
+    }
+
+    int select_class_f64(int64_t m, int64_t n)
+    {
+        for (int i = 0; i < kClassCount; ++i)
+            if (class_fits(i, m, n))
+                return i;
+        return -1;
+    }
+
+    int class_block(int cls)
+    {
+        return kClasses[cls].block;
+    }
+
+    const void *lds_class_f64_symbol(int cls)
+    {
+        return (cls >= 0 && cls < kClassCount) ? kClassSymbol[cls]() : nullptr;
+    }
+
+    hipError_t launch_lds_class_f64(int cls, int grid, int lds_bytes, const ClassArgs &args, hipStream_t stream)
+    {
+        if (cls < 0 || cls >= kClassCount)
+            return hipErrorInvalidValue;
+        kClassLaunch[cls](grid, lds_bytes, args, stream);
+        return hipGetLastError();
+    }
+} // namespace jsvd

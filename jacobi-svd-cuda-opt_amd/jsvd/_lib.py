@@ -50,6 +50,8 @@ class JsvdPlanInfo(ctypes.Structure):
         ("grid_size", ctypes.c_int32),
         ("lds_bytes", ctypes.c_int32),
         ("precision", ctypes.c_int32),
+        ("buckets", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -96,6 +98,8 @@ SIGNATURES = {
     "jsvd_plan_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(JsvdPlanInfo)]),
     "jsvd_plan_destroy": (None, [_vp]),
     "jsvd_autotune": (ctypes.c_int, [ctypes.POINTER(JsvdConfig), ctypes.POINTER(JsvdAutotuneReport)]),
+    "jsvd_tune_class": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
+    "jsvd_class_count": (ctypes.c_int32, []),
 }
 
 _lib = None

@@ -1,0 +1,89 @@
+"""Mixed-precision path (FP32 sweeps, FP64 rotation parameters and
+convergence test, FP64 re-orthonormalisation, B = A Q on FP64 MFMA, FP64
+refinement sweeps) against the FP64 CPU oracle, through the C-ABI.
+
+Bar (the FP64 path's own bar against the oracle): normwise sigma error
+max_k |s_k - r_k| / max_k r_k <= 1e-12 on sigma compared sorted (the
+column order of a different pair sequence is not the reference's), and
+||U^T U - I||, ||V^T V - I||, ||A - U S V^T|| / ||A|| <= 1e-12 (n/64 + 1).
+The componentwise error is reported, not bounded: for kappa up to 1e8 two
+valid FP64 solves already differ by ~kappa u on the smallest sigma.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import jsvd
+import oracle
+from jsvd import gen, metrics
+
+pytestmark = pytest.mark.gpu
+
+
+def _note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _check(a, got, ref_sigma, eps, tag):
+    u, s, v, sw = got
+    n = a.shape[1]
+    tol = 1e-12 * (n / 64 + 1)
+    norm, comp = metrics.sigma_errors(s, ref_sigma)
+    assert norm <= 1e-12, f"{tag}: normwise sigma error {norm:.3e}"
+    assert metrics.orthogonality(u, s > eps) <= tol, f"{tag}: U orthogonality"
+    assert metrics.orthogonality(v) <= tol, f"{tag}: V orthogonality"
+    assert metrics.residual(a, u, s, v) <= tol, f"{tag}: residual"
+    return norm, comp
+
+
+@pytest.mark.parametrize("shape", [(40, 30), (64, 64), (100, 60), (128, 128), (128, 100)])
+def test_mixed_single_shapes(shape):
+    _note(f"mixed shape {shape}")
+    rng = np.random.default_rng(3 + shape[0] + shape[1])
+    sig = 10.0 ** (-5 * np.arange(shape[1]) / (shape[1] - 1))
+    a = gen.constructed(rng, shape[0], shape[1], sig)
+    ref = oracle.solve(a, 1e-15, 30)
+    got = jsvd.svd(a, epsilon=1e-15, max_sweeps=30, precision=jsvd.PRECISION_MIXED)
+    norm, comp = _check(a, got, ref[1], 1e-15, str(shape))
+    _note(f"  normwise {norm:.2e} componentwise {comp:.2e} sweeps {got[3]} (fp64 oracle {ref[3]})")
+
+
+def test_mixed_conditioned_batch_against_fp64_oracle():
+    # 128 x 128 matrices with kappa = 10^U(3, 8), the mixed-precision workload
+    # shape, on a device plan; every matrix against the FP64 oracle.
+    import torch
+
+    _note("mixed batch of 64 conditioned 128x128")
+    mats, sig = gen.conditioned_batch(64)
+    flat, ms, ns = gen.pack(mats)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=1e-15, max_sweeps=30, precision=jsvd.PRECISION_MIXED)
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(flat).to(dev)
+    u = torch.empty(plan.a_elems, dtype=torch.float64, device=dev)
+    s = torch.empty(plan.s_elems, dtype=torch.float64, device=dev)
+    v = torch.empty(plan.v_elems, dtype=torch.float64, device=dev)
+    sw = torch.zeros(len(ms), dtype=torch.int32, device=dev)
+    plan.execute(a, u, s, v, sw)
+    torch.cuda.synchronize()
+    u, s, v, sw = (t.cpu().numpy() for t in (u, s, v, sw))
+    refs = oracle.solve_batch(mats, 1e-15, 30)
+    worst = 0.0
+    for k in range(len(mats)):
+        m, n = int(ms[k]), int(ns[k])
+        g = (u[plan.a_off[k]:plan.a_off[k] + m * n].reshape(m, n), s[plan.s_off[k]:plan.s_off[k] + n],
+             v[plan.v_off[k]:plan.v_off[k] + n * n].reshape(n, n), int(sw[k]))
+        norm, _ = _check(mats[k], g, refs[k][1], 1e-15, f"mixed{k}")
+        worst = max(worst, norm)
+    _note(f"worst normwise sigma error vs FP64 oracle: {worst:.3e}")
+    plan.close()
+
+
+def test_mixed_uneven_batch():
+    _note("mixed uneven batch")
+    mats, sig = gen.uneven_batch(200, seed=99)
+    got = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, precision=jsvd.PRECISION_MIXED)
+    refs = oracle.solve_batch(mats, 1e-14, 30)
+    for k, (a, g, r) in enumerate(zip(mats, got, refs)):
+        _check(a, g, r[1], 1e-14, f"uneven{k}")

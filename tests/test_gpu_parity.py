@@ -67,8 +67,10 @@ def test_golden_fixtures():
 
 
 @pytest.mark.parametrize("shape", [(2, 2), (3, 3), (16, 8), (17, 9), (64, 33), (100, 1), (128, 64), (96, 64),
-                                   (250, 63), (40, 40)])
+                                   (250, 63), (40, 40), (96, 96), (128, 128), (160, 120), (129, 127)])
 def test_lds_path_against_oracle(shape):
+    # (128, 128), (160, 120) and (129, 127) take the rotation-log classes (A in
+    # LDS, V rebuilt from the logged tangents); (96, 96) the 512-thread class.
     _note(f"lds path shape {shape}")
     rng = np.random.default_rng(100 + shape[0] + 1000 * shape[1])
     a = rng.standard_normal(shape)
@@ -88,6 +90,28 @@ def test_persistent_path_against_oracle(shape, threads):
     got = jsvd.svd(a, epsilon=1e-13, max_sweeps=30, threads_per_block=threads)
     ref = oracle.solve(a, 1e-13, 30, threads=8)
     _assert_parity(a, got, ref, 1e-13, str(shape))
+
+
+def test_every_shape_class_against_oracle():
+    # The autotuner may assign any fitting class to a column-count bucket:
+    # force each class in turn on a batch whose shapes it fits.
+    lib = jsvd.load()
+    rng = np.random.default_rng(31)
+    shapes = [(12, 6), (16, 8), (30, 16), (40, 29), (64, 32), (90, 48), (128, 64), (128, 100), (128, 128),
+              (200, 120)]
+    mats = [rng.standard_normal(s) for s in shapes]
+    refs = oracle.solve_batch(mats, 1e-13, 30)
+    try:
+        for cls in range(lib.jsvd_class_count()):
+            for b in range(7):
+                lib.jsvd_tune_class(b, cls)
+            _note(f"forced class {cls}")
+            got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
+            for a, g, r in zip(mats, got, refs):
+                _assert_parity(a, g, r, 1e-13, f"class{cls}{a.shape}")
+    finally:
+        for b in range(7):
+            lib.jsvd_tune_class(b, -1)
 
 
 def test_gaussian_256_at_floor_tolerance():
