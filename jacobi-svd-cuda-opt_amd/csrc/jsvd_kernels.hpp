@@ -31,7 +31,9 @@ namespace jsvd
         return (m * n + n * n) * 4 + n * 8;
     }
 
-    constexpr int64_t kMaxLdsBytes = 160 * 1024 - 256;
+    // Dynamic LDS budget per workgroup: 160 KiB on gfx950 minus room for the
+    // kernels' static LDS and the 16-byte alignment of the dynamic block.
+    constexpr int64_t kMaxLdsBytes = 160 * 1024 - 1024;
 
     hipError_t launch_lds_batch_f64(int block, int grid, int lds_bytes, const WorkItem *items, int count,
                                     const double *a, double *u, double *sigma, double *v, int32_t *sweeps,

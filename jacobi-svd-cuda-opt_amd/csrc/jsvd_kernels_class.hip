@@ -349,7 +349,10 @@ namespace jsvd
         template <int BS, int G, int RMAX2, int VMAX2, bool VLOG>
         __global__ __launch_bounds__(BS) void lds_class_f64(ClassArgs args)
         {
-            extern __shared__ double lds[];
+            // Declared as double2 so the dynamic LDS block is 16-byte aligned
+            // (the row-pair accesses are 16-byte ds_read_b128 / ds_write_b128).
+            extern __shared__ double2 lds_pairs[];
+            double *lds = reinterpret_cast<double *>(lds_pairs);
             __shared__ int s_item;
             for (;;)
             {

@@ -160,7 +160,10 @@ namespace jsvd
         template <int BS, int G, int RMAX4, int VMAX4>
         __global__ __launch_bounds__(BS) void mixed_sweeps_f32(MixedArgs args)
         {
-            extern __shared__ float lds32[];
+            // float4 keeps the dynamic LDS block 16-byte aligned for the
+            // row-quad accesses.
+            extern __shared__ float4 lds_quads[];
+            float *lds32 = reinterpret_cast<float *>(lds_quads);
             __shared__ int s_item;
             constexpr int GROUPS = BS / G;
             const int tid = threadIdx.x;
