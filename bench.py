@@ -4,12 +4,18 @@
 Workload (one step = one pass of the hot path over one batch): 4096
 matrices, n ~ U{8..64}, m ~ U{n..2n}, A = Q1 diag(sigma) Q2^T with Haar
 factors and sigma log-spaced 1 .. 1e-6, eps = 1e-15, at most 30 sweeps,
-solved in one persistent work-list launch.  Inputs are resident in HBM
-before the timed region; outputs stay on the device.
+solved by one device work-list plan.  Inputs are resident in HBM before the
+timed region; outputs stay on the device.
 
-Prints one JSON line: matrices/s, the work-list kernel's FP64 throughput
-against the MI355X FP64 peak (HIP events on the launch stream), and the
-CPU oracle timed on a bounded sample of the same batch.
+Prints one JSON line: matrices/s, the work-list kernels' FP64 throughput and
+LDS traffic against the MI355X peaks (HIP events on the launch stream), the
+HBM traffic measured with rocprofv3 counters, the CPU oracle timed on a
+bounded sample, and the mixed-precision vs FP64 comparison on 1024 matrices
+of 128 x 128 with kappa = 10^U(3,8).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+    python bench.py --traffic-probe    # one 256 MiB copy + one plan execution,
+                                        # for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -24,8 +30,10 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "jacobi-svd-cuda-opt_amd"))
 
-FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix peaks are equal)
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix peaks are equal)
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_TBS = 78.6       # 128 B/clk/CU x 256 CUs x 2.4 GHz
+EPS, MAX_SWEEPS = 1e-15, 30
 
 
 def sweep_flops(m, n):
@@ -35,6 +43,14 @@ def sweep_flops(m, n):
     return n * (n - 1) / 2 * (12 * m + 6 * n)
 
 
+def sweep_lds_bytes(m, n):
+    """Bytes through LDS per plain sweep: every pair reads and writes both A
+    and both V columns once: 16 n(n-1)(m+n)."""
+    m = np.asarray(m, dtype=np.float64)
+    n = np.asarray(n, dtype=np.float64)
+    return 16 * n * (n - 1) * (m + n)
+
+
 def compulsory_bytes(m, n):
     """Read A, write U, sigma, V once: 8 (2mn + n^2 + n) bytes per matrix."""
     m = np.asarray(m, dtype=np.float64)
@@ -42,37 +58,113 @@ def compulsory_bytes(m, n):
     return 8 * (2 * m * n + n * n + n)
 
 
-def load_traffic(name):
+def load_json(name):
     path = os.path.join(REPO, "profiles", name)
-    if not os.path.exists(path):
-        return None
     try:
         with open(path) as f:
             return json.load(f)
-    except Exception:
+    except (OSError, ValueError):
         return None
 
 
-def cpu_baseline(mats, eps, max_sweeps, budget_s=15.0):
-    """CPU oracle (one matrix per OpenMP thread) on a bounded prefix sample."""
+def device_buffers(plan, flat, torch):
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(flat).to(dev)
+    u = torch.empty(plan.a_elems, dtype=torch.float64, device=dev)
+    s = torch.empty(plan.s_elems, dtype=torch.float64, device=dev)
+    v = torch.empty(plan.v_elems, dtype=torch.float64, device=dev)
+    sw = torch.zeros(len(plan.ms), dtype=torch.int32, device=dev)
+    return a, u, s, v, sw
+
+
+def time_plan(plan, bufs, steps, warmup, torch):
+    a, u, s, v, sw = bufs
+    for _ in range(warmup):
+        plan.execute(a, u, s, v, sw)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        ev[k][0].record(stream)
+        plan.execute(a, u, s, v, sw)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, float(np.mean([b.elapsed_time(e) for b, e in ev]))
+
+
+def unpack(plan, s, k):
+    n = int(plan.ns[k])
+    return s[plan.s_off[k]:plan.s_off[k] + n]
+
+
+def cpu_baseline(mats, budget_s=12.0):
+    """CPU oracle, one matrix per OpenMP thread, on a bounded sample of the batch
+    (whole batch repeated until about budget_s of CPU work)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    cores = os.cpu_count() or 1
-    cores = min(cores, 64)
-    sample = mats[:64]
+    cores = min(os.cpu_count() or 1, 64)
     t0 = time.perf_counter()
-    oracle.solve_batch(sample, eps, max_sweeps, threads=cores)
-    dt = time.perf_counter() - t0
-    per = dt / len(sample)
-    count = int(min(len(mats), max(64, budget_s / max(per, 1e-9))))
-    sample = mats[:count]
+    oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
+    once = time.perf_counter() - t0
+    reps = max(1, int(budget_s / max(once, 1e-6)))
     t0 = time.perf_counter()
-    oracle.solve_batch(sample, eps, max_sweeps, threads=cores)
+    for _ in range(reps):
+        oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
     dt = time.perf_counter() - t0
-    return {"value": len(sample) / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
-            "sample": f"first {len(sample)} matrices of the same 4096-matrix batch, {dt:.1f} s, "
-                      f"one matrix per OpenMP thread"}
+    return {"value": reps * len(mats) / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
+            "sample": f"the full {len(mats)}-matrix batch solved {reps} times ({dt:.1f} s) by the C oracle, "
+                      f"one matrix per OpenMP thread on {cores} threads"}
+
+
+def mixed_vs_fp64(torch, jsvd, gen, metrics, steps=3):
+    """1024 matrices 128 x 128, kappa = 10^U(3,8): FP64 plan vs mixed plan."""
+    mats, _ = gen.conditioned_batch(1024)
+    flat, ms, ns = gen.pack(mats)
+    out = {"workload": "1024 x 128x128, kappa=10^U(3,8), sigma_k = kappa^(-k/127)", "epsilon": EPS,
+           "max_sweeps": MAX_SWEEPS}
+    sig = {}
+    for name, prec in (("fp64", jsvd.PRECISION_FP64), ("mixed", jsvd.PRECISION_MIXED)):
+        plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS, precision=prec)
+        bufs = device_buffers(plan, flat, torch)
+        wall, kernel_ms = time_plan(plan, bufs, steps, 1, torch)
+        s_h = bufs[2].cpu().numpy()
+        sweeps = bufs[4].cpu().numpy()
+        sig[name] = [unpack(plan, s_h, k).copy() for k in range(len(mats))]
+        out[name] = {"matrices_per_s": len(mats) * steps / wall, "ms_per_batch": wall / steps * 1e3,
+                     "mean_sweeps": float(sweeps.mean())}
+        plan.close()
+    norm = [metrics.sigma_errors(sm, sf)[0] for sm, sf in zip(sig["mixed"], sig["fp64"])]
+    comp = [metrics.sigma_errors(sm, sf)[1] for sm, sf in zip(sig["mixed"], sig["fp64"])]
+    out["mixed_vs_fp64_sigma_err_normwise_max"] = float(np.max(norm))
+    out["mixed_vs_fp64_sigma_err_componentwise_max"] = float(np.max(comp))
+    out["speedup_mixed_over_fp64"] = out["mixed"]["matrices_per_s"] / out["fp64"]["matrices_per_s"]
+    return out
+
+
+def traffic_probe():
+    """One 256 MiB device copy (the counter calibration) and one execution of
+    the benchmark plan, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs."""
+    import torch
+
+    import jsvd
+    from jsvd import gen
+
+    mats, _ = gen.uneven_batch(4096)
+    flat, ms, ns = gen.pack(mats)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    bufs = device_buffers(plan, flat, torch)
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    plan.execute(*bufs)
+    torch.cuda.synchronize()
+    print("traffic probe done", flush=True)
 
 
 def main():
@@ -82,60 +174,42 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed vs FP64 comparison")
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "batches would shard by matrix across GPUs with no exchange, which is not built here.",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe()
+        return
 
     import torch
 
     import jsvd
     from jsvd import gen, metrics
 
-    eps, max_sweeps = 1e-15, 30
     mats, sig = gen.uneven_batch(args.count)
     flat, ms, ns = gen.pack(mats)
-    plan = jsvd.BatchPlan(ms, ns, epsilon=eps, max_sweeps=max_sweeps)
-    dev = torch.device("cuda:0")
-    a = torch.from_numpy(flat).to(dev)
-    u = torch.empty(plan.a_elems, dtype=torch.float64, device=dev)
-    s = torch.empty(plan.s_elems, dtype=torch.float64, device=dev)
-    v = torch.empty(plan.v_elems, dtype=torch.float64, device=dev)
-    sw = torch.zeros(len(ms), dtype=torch.int32, device=dev)
-
-    for _ in range(args.warmup):
-        plan.execute(a, u, s, v, sw)
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ev[k][0].record(stream)
-        plan.execute(a, u, s, v, sw)
-        ev[k][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms = float(np.mean([b.elapsed_time(e) for b, e in ev]))
+    plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    bufs = device_buffers(plan, flat, torch)
+    wall, kernel_ms = time_plan(plan, bufs, args.steps, args.warmup, torch)
 
     # Sanity of the last step's results (outside the timed region).
-    sweeps = sw.cpu().numpy()
-    s_h = s.cpu().numpy()
-    worst = 0.0
-    for k in range(0, len(mats), 61):
-        n = int(ns[k])
-        worst = max(worst, metrics.sigma_errors(s_h[plan.s_off[k]:plan.s_off[k] + n], sig[k])[0])
+    sweeps = bufs[4].cpu().numpy()
+    s_h = bufs[2].cpu().numpy()
+    worst = max(metrics.sigma_errors(unpack(plan, s_h, k), sig[k])[0] for k in range(0, len(mats), 61))
     if not worst < 1e-12:
         print(f"bench.py: sigma error {worst:.3e} exceeds 1e-12", file=sys.stderr)
         sys.exit(1)
 
     flops = float(np.sum(sweep_flops(ms, ns) * sweeps))
-    ms_per_step = wall / args.steps * 1e3
+    lds_bytes = float(np.sum(sweep_lds_bytes(ms, ns) * sweeps))
     achieved_tflops = flops / (kernel_ms * 1e-3) / 1e12
-    traffic = load_traffic("lds_batch_f64_uneven4096_traffic.json")
+    traffic = load_json("lds_class_f64_uneven4096_traffic.json")
+    info = plan.info
     result = {
         "metric": "matrices/s (batched FP64 one-sided Jacobi, 4096 uneven matrices)",
         "value": len(ms) * args.steps / wall,
@@ -143,26 +217,32 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "batched_uneven_fp64", "matrices": int(len(ms)), "n": "U{8..64}",
-                   "m": "U{n..2n}", "sigma": "logspace(0,-6,n)", "epsilon": eps, "max_sweeps": max_sweeps,
-                   "mean_sweeps": float(sweeps.mean()), "max_sigma_err_normwise": worst},
+                   "m": "U{n..2n}", "sigma": "logspace(0,-6,n)", "epsilon": EPS, "max_sweeps": MAX_SWEEPS,
+                   "mean_sweeps": float(sweeps.mean()), "max_sigma_err_normwise": worst,
+                   "buckets": info["buckets"]},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "lds_batch_f64", "kernel_ms": kernel_ms,
+                     "kernel": "lds_class_f64 (all buckets of one plan execution)", "kernel_ms": kernel_ms,
                      "flops_per_launch": flops,
+                     "lds": {"achieved_TBs": lds_bytes / (kernel_ms * 1e-3) / 1e12, "peak_TBs": LDS_PEAK_TBS,
+                             "frac": lds_bytes / (kernel_ms * 1e-3) / 1e12 / LDS_PEAK_TBS,
+                             "bytes_per_launch": lds_bytes},
                      "compulsory_bytes_per_launch": float(np.sum(compulsory_bytes(ms, ns)))},
     }
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(mats, eps, max_sweeps)
-    print(json.dumps(result), flush=True)
     plan.close()
+    if not args.no_mixed:
+        result["mixed_vs_fp64"] = mixed_vs_fp64(torch, jsvd, gen, metrics)
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(mats)
+    print(json.dumps(result), flush=True)
 
 
 if __name__ == "__main__":

@@ -24,6 +24,26 @@ namespace jsvd::dev
         return column < n ? column : -1;
     }
 
+    // Incremental form of column_at for the two slots of pair position k:
+    // at round 0 they hold columns k and E-1-k; each round every slot except
+    // slot 0 takes the next lower column, wrapping 1 -> E-1.  A value >= n is
+    // the dummy column of an odd n.  Replaces two integer modulos per round.
+    struct CirclePair
+    {
+        int p, q;
+        __device__ __forceinline__ void start(int k, int even)
+        {
+            p = k;
+            q = even - 1 - k;
+        }
+        __device__ __forceinline__ void advance(int even)
+        {
+            if (p >= 1)
+                p = (p == 1) ? even - 1 : p - 1;
+            q = (q == 1) ? even - 1 : q - 1;
+        }
+    };
+
     // Rutishauser rotation for the 2x2 Gram [[pp, pq], [pq, qq]] with the
     // reference's threshold |pq| > eps*sqrt(pp*qq) and |pq| > 1e-300
     // (src/domain/jacobi_rotation_kernels.cuh:305-326).  Returns true when the

@@ -40,20 +40,22 @@ namespace jsvd
         {
             int block, group, rmax2, vmax2;
             bool vlog;
+            int passes; // pair positions per lane group and round
         };
 
         // Shape classes.  The default for a shape is the first class that
         // fits; the autotuner may choose another fitting class per column-count
         // bucket.  rmax2 / vmax2 are register row pairs per lane for A and V.
         constexpr ClassSpec kClasses[] = {
-            {64, 16, 1, 1, false},  // 0: n <= 8, m <= 32: one wave, 16 lanes per pair
-            {64, 8, 2, 1, false},   // 1: n <= 16, m <= 32
-            {64, 4, 8, 4, false},   // 2: n <= 32, m <= 64
-            {512, 16, 4, 2, false}, // 3: n <= 64, m <= 128: eight waves
-            {256, 8, 8, 4, false},  // 4: n <= 64, m <= 128: four waves
-            {512, 16, 8, 4, false}, // 5: n <= 128, m <= 256
-            {1024, 16, 4, 4, true}, // 6: n <= 128, m <= 128, V by rotation log
-            {512, 16, 8, 8, true},  // 7: n <= 256, m <= 256, V by rotation log
+            {64, 16, 1, 1, false, 1},  // 0: n <= 8, m <= 32: one wave, 16 lanes per pair
+            {64, 8, 2, 1, false, 1},   // 1: n <= 16, m <= 32
+            {64, 4, 8, 4, false, 1},   // 2: n <= 32, m <= 64
+            {512, 16, 4, 2, false, 1}, // 3: n <= 64, m <= 128: eight waves
+            {256, 8, 8, 4, false, 1},  // 4: n <= 64, m <= 128: four waves
+            {512, 16, 8, 4, false, 2}, // 5: n <= 128, m <= 256
+            {1024, 16, 4, 4, true, 1}, // 6: n <= 128, m <= 128, V by rotation log
+            {512, 16, 8, 8, true, 4},  // 7: n <= 256, m <= 256, V by rotation log
+            {512, 16, 4, 2, true, 1},  // 8: n <= 64, m <= 128, V by rotation log (two workgroups per CU)
         };
         constexpr int kClassCount = sizeof(kClasses) / sizeof(kClasses[0]);
 
@@ -197,7 +199,7 @@ namespace jsvd
             }
         }
 
-        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG>
+        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES>
         __device__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
         {
             constexpr int GROUPS = BS / G;
@@ -249,22 +251,32 @@ namespace jsvd
             const int lda2 = lda / 2, ldv2 = ldv / 2;
             double *tlog = VLOG ? args.vlog + (int64_t)blockIdx.x * args.vlog_stride : nullptr;
             double dummy_t;
+            dev::CirclePair circ[PASSES]; // pair positions of this group, one per pass
             int executed = 0;
             for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
             {
                 int any = 0;
+#pragma unroll
+                for (int i = 0; i < PASSES; ++i)
+                    circ[i].start(gid + i * GROUPS, even);
                 for (int r = 0; r < rounds; ++r)
                 {
-                    for (int k = gid; k < half; k += GROUPS)
+#pragma unroll
+                    for (int i = 0; i < PASSES; ++i)
                     {
-                        const int p = dev::column_at(k, r, n, even);
-                        const int q = dev::column_at(even - 1 - k, r, n, even);
-                        if (p < 0 || q < 0)
-                            continue;
-                        double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + k : &dummy_t;
-                        any |= pair_class_f64<G, RMAX2, VMAX2, VLOG>(A2 + p * lda2, A2 + q * lda2, V2 + p * ldv2,
-                                                                     V2 + q * ldv2, rows2, vrows2, lane, args.eps,
-                                                                     slot);
+                        const int k = gid + i * GROUPS;
+                        if (k < half)
+                        {
+                            const int p = circ[i].p, q = circ[i].q;
+                            if (p < n && q < n)
+                            {
+                                double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + k : &dummy_t;
+                                any |= pair_class_f64<G, RMAX2, VMAX2, VLOG>(A2 + p * lda2, A2 + q * lda2,
+                                                                             V2 + p * ldv2, V2 + q * ldv2, rows2,
+                                                                             vrows2, lane, args.eps, slot);
+                            }
+                            circ[i].advance(even);
+                        }
                     }
                     __syncthreads();
                 }
@@ -306,25 +318,56 @@ namespace jsvd
                 __syncthreads();
                 init_v();
                 __syncthreads();
+                // The tangents of one sweep are preloaded across the group's
+                // lanes (lane l holds rounds l, l+G, ...), so the log's HBM
+                // latency is paid once per sweep, then broadcast per round.
+                constexpr int RT = (2 * PASSES * GROUPS - 1 + G - 1) / G; // rounds per lane
+                const int group_base = threadIdx.x & ~(G - 1);
                 for (int sweep = 0; sweep < executed; ++sweep)
                 {
-                    for (int r = 0; r < rounds; ++r)
+                    double tp[PASSES][RT];
+#pragma unroll
+                    for (int i = 0; i < PASSES; ++i)
                     {
-                        for (int k = gid; k < half; k += GROUPS)
+                        circ[i].start(gid + i * GROUPS, even);
+                        const int k = gid + i * GROUPS;
+#pragma unroll
+                        for (int j = 0; j < RT; ++j)
                         {
-                            const int p = dev::column_at(k, r, n, even);
-                            const int q = dev::column_at(even - 1 - k, r, n, even);
-                            if (p < 0 || q < 0)
-                                continue;
-                            // Agent-scope load: bypasses the CU's vector L1, which may
-                            // still hold this slot's entry from the previous matrix.
-                            const unsigned long long bits = __hip_atomic_load(
-                                reinterpret_cast<unsigned long long *>(tlog + ((int64_t)sweep * rounds + r) * half + k),
-                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            const double t = __longlong_as_double(static_cast<long long>(bits));
-                            replay_pair_f64<G, VMAX2>(V2 + p * ldv2, V2 + q * ldv2, vrows2, lane, t);
+                            const int r = lane + j * G;
+                            // Agent-scope loads bypass the CU's vector L1, which may
+                            // still hold this slot's entries from the previous matrix.
+                            tp[i][j] = (k < half && r < rounds)
+                                           ? __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+                                                 reinterpret_cast<unsigned long long *>(
+                                                     tlog + ((int64_t)sweep * rounds + r) * half + k),
+                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                                           : 0.0;
                         }
-                        __syncthreads();
+                    }
+#pragma unroll
+                    for (int j = 0; j < RT; ++j)
+                    {
+                        for (int rr = 0; rr < G; ++rr)
+                        {
+                            const int r = j * G + rr;
+                            if (r >= rounds)
+                                break;
+#pragma unroll
+                            for (int i = 0; i < PASSES; ++i)
+                            {
+                                const int k = gid + i * GROUPS;
+                                const double t = __shfl(tp[i][j], group_base + rr, 64);
+                                if (k < half)
+                                {
+                                    const int p = circ[i].p, q = circ[i].q;
+                                    if (p < n && q < n)
+                                        replay_pair_f64<G, VMAX2>(V2 + p * ldv2, V2 + q * ldv2, vrows2, lane, t);
+                                    circ[i].advance(even);
+                                }
+                            }
+                            __syncthreads();
+                        }
                     }
                 }
             }
@@ -346,7 +389,7 @@ namespace jsvd
         // counters[0]: next work-list entry; counters[1]: workgroups finished.
         // The last workgroup to finish resets both, so the next launch on the
         // stream starts from a clean work-list without a memset.
-        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG>
+        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES>
         __global__ __launch_bounds__(BS) void lds_class_f64(ClassArgs args)
         {
             // Declared as double2 so the dynamic LDS block is 16-byte aligned
@@ -363,7 +406,7 @@ namespace jsvd
                 if (item >= args.count)
                     break;
                 const WorkItem w = args.items[item];
-                solve_class_f64<BS, G, RMAX2, VMAX2, VLOG>(w, lds, args);
+                solve_class_f64<BS, G, RMAX2, VMAX2, VLOG, PASSES>(w, lds, args);
                 __syncthreads();
             }
             if (threadIdx.x == 0)
@@ -381,22 +424,25 @@ namespace jsvd
         const void *class_symbol_t()
         {
             constexpr ClassSpec c = kClasses[I];
-            return reinterpret_cast<const void *>(&lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog>);
+            return reinterpret_cast<const void *>(
+                &lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog, c.passes>);
         }
 
         template <int I>
         void class_launch_t(int grid, int lds_bytes, const ClassArgs &args, hipStream_t stream)
         {
             constexpr ClassSpec c = kClasses[I];
-            lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog><<<grid, c.block, lds_bytes, stream>>>(args);
+            lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog, c.passes>
+                <<<grid, c.block, lds_bytes, stream>>>(args);
         }
 
         using LaunchFn = void (*)(int, int, const ClassArgs &, hipStream_t);
-        const LaunchFn kClassLaunch[] = {class_launch_t<0>, class_launch_t<1>, class_launch_t<2>, class_launch_t<3>,
-                                         class_launch_t<4>, class_launch_t<5>, class_launch_t<6>, class_launch_t<7>};
+        const LaunchFn kClassLaunch[] = {class_launch_t<0>, class_launch_t<1>, class_launch_t<2>,
+                                         class_launch_t<3>, class_launch_t<4>, class_launch_t<5>,
+                                         class_launch_t<6>, class_launch_t<7>, class_launch_t<8>};
         const void *(*const kClassSymbol[])() = {class_symbol_t<0>, class_symbol_t<1>, class_symbol_t<2>,
                                                  class_symbol_t<3>, class_symbol_t<4>, class_symbol_t<5>,
-                                                 class_symbol_t<6>, class_symbol_t<7>};
+                                                 class_symbol_t<6>, class_symbol_t<7>, class_symbol_t<8>};
         static_assert(sizeof(kClassLaunch) / sizeof(kClassLaunch[0]) == kClassCount);
         static_assert(sizeof(kClassSymbol) / sizeof(kClassSymbol[0]) == kClassCount);
     } // namespace
@@ -430,7 +476,7 @@ namespace jsvd
         const int64_t groups = c.block / c.group;
         // One wave: one pair per group per round.  Multi-wave classes may
         // loop over pairs (up to two passes, four with the rotation log).
-        const int64_t passes = c.block == 64 ? 1 : (c.vlog ? 4 : 2);
+        const int64_t passes = c.passes;
         if (half > passes * groups)
             return false;
         const int64_t lda = ((m + 2 * c.group - 1) / (2 * c.group)) * (2 * c.group);
