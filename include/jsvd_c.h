@@ -130,6 +130,22 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *plan, const double *d_a, double *d_
 jsvd_status jsvd_plan_get_info(const jsvd_plan *plan, jsvd_plan_info *info);
 void jsvd_plan_destroy(jsvd_plan *plan);
 
+/* SVD adjoint (no reference counterpart: the reference has no derivative).
+ * For each matrix of the plan, from the solve's U, sigma, V (any column
+ * order, as returned) and upstream gradients gU, gsigma, gV (each may be
+ * NULL = zero), writes
+ *   gA = U [ (F o (U^T gU - gU^T U)) S + S (F o (V^T gV - gV^T V)) + diag(gsigma) ] V^T
+ *        + (I - U U^T) gU S^-1 V^T,   F_ij = 1/(s_j^2 - s_i^2) (i != j),
+ * packed like A.  Requires distinct nonzero singular values and
+ * (2n^2 + mn) elements of LDS per matrix (n <= 64 at m = 2n in FP64).
+ * Asynchronous on `stream`; allocates nothing. */
+jsvd_status jsvd_plan_backward_f64(jsvd_plan *plan, const double *d_u, const double *d_sigma, const double *d_v,
+                                   const double *d_gu, const double *d_gsigma, const double *d_gv, double *d_ga,
+                                   void *stream);
+jsvd_status jsvd_plan_backward_f32(jsvd_plan *plan, const float *d_u, const float *d_sigma, const float *d_v,
+                                   const float *d_gu, const float *d_gsigma, const float *d_gv, float *d_ga,
+                                   void *stream);
+
 /* Layout-threshold autotune.  Replaces jacobi::svd::auto_tune_layout_transpose_threshold
  * (include/jacobi/svd/domain/jacobi_svd.cuh:36-37, src/domain/layout_transpose_autotune.cu:78-126). */
 typedef struct jsvd_autotune_report

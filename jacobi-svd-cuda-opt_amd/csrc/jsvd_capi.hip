@@ -225,6 +225,10 @@ struct jsvd_plan
     DevBuf<unsigned int> d_counters; // kCountersPerBucket per bucket
     std::vector<std::unique_ptr<DevBuf<double>>> vlogs;
     DevBuf<double> q_scratch;        // mixed precision: packed column-major Q
+    // Every matrix in batch order (the adjoint kernel's work-list) and the
+    // largest LDS need of the adjoint in FP64 / FP32.
+    DevBuf<jsvd::WorkItem> d_all;
+    int64_t bw_lds_f64 = 0, bw_lds_f32 = 0;
     // Buckets beyond the first run on plan-owned streams forked from the
     // caller's stream and joined back (event record / wait only).
     std::vector<hipStream_t> aux;
@@ -546,6 +550,30 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             }
         }
     }
+    if (count > 0)
+    {
+        std::vector<jsvd::WorkItem> all(count);
+        for (int64_t k = 0; k < count; ++k)
+        {
+            all[k].m = static_cast<int32_t>(m[k]);
+            all[k].n = static_cast<int32_t>(n[k]);
+            all[k].index = static_cast<int32_t>(k);
+            all[k].a_off = plan->a_off[k];
+            all[k].v_off = plan->v_off[k];
+            all[k].s_off = plan->s_off[k];
+            plan->bw_lds_f64 = std::max(plan->bw_lds_f64, jsvd::backward_lds_bytes(m[k], n[k], 8));
+            plan->bw_lds_f32 = std::max(plan->bw_lds_f32, jsvd::backward_lds_bytes(m[k], n[k], 4));
+        }
+        JSVD_CHECK(plan->d_all.alloc(count));
+        JSVD_HIP(hipMemcpy(plan->d_all.ptr, all.data(), sizeof(jsvd::WorkItem) * count, hipMemcpyHostToDevice));
+        for (int elem : {8, 4})
+        {
+            const int64_t need = elem == 8 ? plan->bw_lds_f64 : plan->bw_lds_f32;
+            if (need <= jsvd::kMaxLdsBytes)
+                JSVD_HIP(hipFuncSetAttribute(jsvd::backward_symbol(elem), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             static_cast<int>(need)));
+        }
+    }
     if (!plan->large.empty())
     {
         plan->big_block = persistent_kernel_block(cfg.threads_per_block);
@@ -668,6 +696,56 @@ jsvd_status jsvd_plan_get_info(const jsvd_plan *p, jsvd_plan_info *info)
 void jsvd_plan_destroy(jsvd_plan *p)
 {
     delete p;
+}
+
+} // extern "C"
+
+namespace
+{
+    template <typename T>
+    jsvd_status plan_backward(jsvd_plan *p, const T *u, const T *s, const T *v, const T *gu, const T *gs, const T *gv,
+                              T *ga, void *stream_ptr)
+    {
+        g_last_error.clear();
+        if (!p)
+            return fail(JSVD_EINVAL, "plan is null.");
+        if (p->count == 0)
+            return JSVD_OK;
+        if (!u || !s || !v || !ga)
+            return fail(JSVD_EINVAL, "device buffer pointer is null.");
+        const int64_t need = sizeof(T) == 8 ? p->bw_lds_f64 : p->bw_lds_f32;
+        if (need > jsvd::kMaxLdsBytes)
+            return fail(JSVD_EINVAL, "SVD backward needs (2n^2 + mn) elements of LDS per matrix; a matrix of this "
+                                     "plan exceeds the 159 KiB budget.");
+        int dev = 0;
+        JSVD_HIP(hipGetDevice(&dev));
+        if (dev != p->device)
+            return fail(JSVD_EINVAL, "plan was created on another device.");
+        hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+        if constexpr (sizeof(T) == 8)
+            JSVD_HIP(jsvd::launch_backward_f64(p->d_all.ptr, static_cast<int>(p->count), static_cast<int>(need), u, s,
+                                               v, gu, gs, gv, ga, stream));
+        else
+            JSVD_HIP(jsvd::launch_backward_f32(p->d_all.ptr, static_cast<int>(p->count), static_cast<int>(need), u, s,
+                                               v, gu, gs, gv, ga, stream));
+        return JSVD_OK;
+    }
+} // namespace
+
+extern "C" {
+
+jsvd_status jsvd_plan_backward_f64(jsvd_plan *plan, const double *d_u, const double *d_sigma, const double *d_v,
+                                   const double *d_gu, const double *d_gsigma, const double *d_gv, double *d_ga,
+                                   void *stream)
+{
+    return plan_backward<double>(plan, d_u, d_sigma, d_v, d_gu, d_gsigma, d_gv, d_ga, stream);
+}
+
+jsvd_status jsvd_plan_backward_f32(jsvd_plan *plan, const float *d_u, const float *d_sigma, const float *d_v,
+                                   const float *d_gu, const float *d_gsigma, const float *d_gv, float *d_ga,
+                                   void *stream)
+{
+    return plan_backward<float>(plan, d_u, d_sigma, d_v, d_gu, d_gsigma, d_gv, d_ga, stream);
 }
 
 jsvd_status jsvd_solve_batched_f64(int64_t count, const double *const *a_rm, const uint64_t *m, const uint64_t *n,

@@ -100,6 +100,17 @@ namespace jsvd
     int gemm_aq_block();
     hipError_t launch_gemm_aq_f64(int grid, const MixedArgs &args, hipStream_t stream);
 
+    // SVD adjoint: gA from (U, s, V) and upstream (gU, gs, gV); any upstream
+    // pointer may be null (zero).  One workgroup per work-list entry.
+    int64_t backward_lds_bytes(int64_t m, int64_t n, int elem);
+    const void *backward_symbol(int elem);
+    hipError_t launch_backward_f64(const WorkItem *items, int count, int lds_bytes, const double *u, const double *s,
+                                   const double *v, const double *gu, const double *gs, const double *gv, double *ga,
+                                   hipStream_t stream);
+    hipError_t launch_backward_f32(const WorkItem *items, int count, int lds_bytes, const float *u, const float *s,
+                                   const float *v, const float *gu, const float *gs, const float *gv, float *ga,
+                                   hipStream_t stream);
+
     // Persistent plain-FP64 sweep kernel over one column-major matrix.
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,

@@ -113,6 +113,22 @@ class BatchPlan:
                                                 v.data_ptr(), sweeps.data_ptr() if sweeps is not None else None,
                                                 st.cuda_stream))
 
+    def backward(self, u, s, v, gu, gs, gv, ga, stream=None):
+        """SVD adjoint on device tensors of the packed layout (float64 or float32,
+        all the same dtype); gu / gs / gv may be None (zero gradient)."""
+        import torch
+
+        dt = u.dtype
+        if dt not in (torch.float64, torch.float32):
+            raise ValueError("backward supports float64 and float32")
+        for t in (u, s, v, gu, gs, gv, ga):
+            if t is not None and not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
+                raise ValueError("backward needs contiguous CUDA tensors of one dtype")
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        st = stream if stream is not None else torch.cuda.current_stream()
+        fn = load().jsvd_plan_backward_f64 if dt == torch.float64 else load().jsvd_plan_backward_f32
+        _lib.check(fn(self._handle, ptr(u), ptr(s), ptr(v), ptr(gu), ptr(gs), ptr(gv), ptr(ga), st.cuda_stream))
+
     def close(self):
         if self._handle:
             load().jsvd_plan_destroy(self._handle)

@@ -97,6 +97,8 @@ SIGNATURES = {
     "jsvd_plan_execute_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jsvd_plan_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(JsvdPlanInfo)]),
     "jsvd_plan_destroy": (None, [_vp]),
+    "jsvd_plan_backward_f64": (ctypes.c_int, [_vp] * 9),
+    "jsvd_plan_backward_f32": (ctypes.c_int, [_vp] * 9),
     "jsvd_autotune": (ctypes.c_int, [ctypes.POINTER(JsvdConfig), ctypes.POINTER(JsvdAutotuneReport)]),
     "jsvd_tune_class": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
     "jsvd_class_count": (ctypes.c_int32, []),
