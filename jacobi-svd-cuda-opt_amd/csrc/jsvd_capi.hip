@@ -240,6 +240,7 @@ struct jsvd_plan
     DevBuf<int32_t> ws_flags;
     DevBuf<unsigned int> ws_bar;
     int big_block = 256;
+    hipEvent_t big_in = nullptr, big_out = nullptr;
     int64_t a_elems = 0, v_elems = 0, s_elems = 0;
 
     ~jsvd_plan()
@@ -250,11 +251,31 @@ struct jsvd_plan
             (void)hipEventDestroy(e);
         if (fork)
             (void)hipEventDestroy(fork);
+        if (big_in)
+            (void)hipEventDestroy(big_in);
+        if (big_out)
+            (void)hipEventDestroy(big_out);
     }
 };
 
 namespace
 {
+    // One stream per device carries every grid-barrier launch, so at most one
+    // such kernel is on the GPU at a time (see launch_persistent_sweeps_f64).
+    std::mutex g_barrier_mu;
+    std::vector<hipStream_t> g_barrier_streams;
+
+    jsvd_status barrier_stream(int device, hipStream_t *out)
+    {
+        std::lock_guard<std::mutex> lock(g_barrier_mu);
+        if (static_cast<int>(g_barrier_streams.size()) <= device)
+            g_barrier_streams.resize(device + 1, nullptr);
+        if (!g_barrier_streams[device])
+            JSVD_HIP(hipStreamCreateWithFlags(&g_barrier_streams[device], hipStreamNonBlocking));
+        *out = g_barrier_streams[device];
+        return JSVD_OK;
+    }
+
     jsvd_status plan_large(jsvd_plan &p, int64_t k, const double *d_a, double *d_u, double *d_sigma, double *d_v,
                            int32_t *d_sweeps, hipStream_t stream)
     {
@@ -583,6 +604,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         JSVD_CHECK(plan->ws_flags.alloc(cfg.max_sweeps));
         JSVD_CHECK(plan->ws_bar.alloc(2));
         JSVD_HIP(hipMemset(plan->ws_bar.ptr, 0, sizeof(unsigned int) * 2));
+        JSVD_HIP(hipEventCreateWithFlags(&plan->big_in, hipEventDisableTiming));
+        JSVD_HIP(hipEventCreateWithFlags(&plan->big_out, hipEventDisableTiming));
     }
     JSVD_HIP(hipDeviceSynchronize());
     *out = plan.release();
@@ -670,8 +693,19 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
         JSVD_HIP(hipEventRecord(p->join[i], p->aux[i]));
         JSVD_HIP(hipStreamWaitEvent(stream, p->join[i], 0));
     }
-    for (int64_t k : p->large)
-        JSVD_CHECK(plan_large(*p, k, d_a, d_u, d_sigma, d_v, d_sweeps, stream));
+    if (!p->large.empty())
+    {
+        // Large matrices run on the device's barrier stream, ordered after
+        // the caller's prior work and before its later work.
+        hipStream_t bs = nullptr;
+        JSVD_CHECK(barrier_stream(p->device, &bs));
+        JSVD_HIP(hipEventRecord(p->big_in, stream));
+        JSVD_HIP(hipStreamWaitEvent(bs, p->big_in, 0));
+        for (int64_t k : p->large)
+            JSVD_CHECK(plan_large(*p, k, d_a, d_u, d_sigma, d_v, d_sweeps, bs));
+        JSVD_HIP(hipEventRecord(p->big_out, bs));
+        JSVD_HIP(hipStreamWaitEvent(stream, p->big_out, 0));
+    }
     return JSVD_OK;
 }
 

@@ -499,15 +499,32 @@ namespace jsvd
         }
     }
 
+    // The grid barrier needs every workgroup resident at once.  The caller
+    // sizes `grid` to the occupancy-guaranteed resident count and issues all
+    // barrier kernels of a device on one stream, so two of them never share
+    // the GPU; any other kernel that delays residency finishes on its own.
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
                                             int32_t *sweeps_out, hipStream_t stream)
     {
-        const void *fn = persistent_sweeps_f64_symbol(block);
-        if (!fn)
-            return hipErrorInvalidValue;
-        void *args[] = {&a_cm, &v_cm, &m, &n, &eps, &max_sweeps, &flags, &barrier, &sweeps_out};
-        return hipLaunchCooperativeKernel(fn, dim3(grid), dim3(block), args, 0, stream);
+        switch (block)
+        {
+        case 256:
+            persistent_sweeps_f64<256, kPersistentRmax><<<grid, 256, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps,
+                                                                                flags, barrier, sweeps_out);
+            break;
+        case 512:
+            persistent_sweeps_f64<512, kPersistentRmax><<<grid, 512, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps,
+                                                                                flags, barrier, sweeps_out);
+            break;
+        case 1024:
+            persistent_sweeps_f64<1024, kPersistentRmax><<<grid, 1024, 0, stream>>>(a_cm, v_cm, m, n, eps,
+                                                                                  max_sweeps, flags, barrier,
+                                                                                  sweeps_out);
+            break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
     }
 
     hipError_t launch_transpose_f64(const double *in, double *out, int64_t rows, int64_t cols,

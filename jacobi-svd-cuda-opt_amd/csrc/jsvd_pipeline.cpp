@@ -1,0 +1,412 @@
+// jsvd_pipeline.cpp -- the batched pipeline (reference src/application/pipeline.cu:25-139,
+// src/application/output_stage.cuh:20-276, src/application/kernel_stage.cuh:30-58).
+//
+// The reference hands every matrix to a host thread pool and makes one
+// synchronous solver call per matrix.  Here matrices are gathered into
+// device batches: a batch is packed into pinned staging, copied to HBM,
+// solved by one work-list plan, copied back and written out.  Batches rotate
+// over `max_queued_results` slots, each with its own HIP stream, so reading
+// and packing the next batch overlaps the device work of the previous ones;
+// the oldest slot is always drained first, which keeps the output in input
+// order (U, Sigma as 1 x n, V for every testcase, as the reference writes).
+// A slot whose next batch has the same shapes as its last one replays a
+// captured hipGraph of copy-in, solve and copy-out instead of re-issuing the
+// launches.
+#include "jacobi/svd/application/pipeline.cuh"
+#include "jacobi/svd/io/io.cuh"
+#include "jsvd_c.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace jacobi::svd::pipeline
+{
+    namespace
+    {
+        constexpr std::size_t kBatchMatrices = 4096;
+        constexpr std::size_t kBatchElements = std::size_t(32) << 20; // 256 MiB of FP64 input
+
+        void hip_check(hipError_t e, const char *what)
+        {
+            if (e == hipSuccess)
+                return;
+            throw CudaError((std::string("HIP call failed: ") + what + ", code=" + std::to_string(int(e)) +
+                             ", message=" + hipGetErrorString(e))
+                                .c_str());
+        }
+
+        void rethrow(jsvd_status status)
+        {
+            if (status == JSVD_OK)
+                return;
+            const std::string msg = jsvd_last_error();
+            if (status == JSVD_EINVAL)
+                throw std::invalid_argument(msg);
+            throw CudaError(msg.c_str());
+        }
+
+        MatrixFileFormat resolve(MatrixFileFormat requested, const std::filesystem::path &path)
+        {
+            if (requested != MatrixFileFormat::auto_detect)
+                return requested;
+            std::string ext = path.extension().string();
+            std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return std::tolower(c); });
+            if (ext == ".mat")
+                return MatrixFileFormat::mat;
+            if (ext == ".txt")
+                return MatrixFileFormat::txt;
+            throw std::invalid_argument("Cannot auto-detect file format for path: " + path.string());
+        }
+
+        jsvd_config to_c(const JacobiSvdConfig &c)
+        {
+            jsvd_config out;
+            jsvd_config_default(&out);
+            out.epsilon = c.epsilon;
+            out.max_sweeps = c.max_sweeps;
+            out.threads_per_block = c.threads_per_block;
+            out.layout_mode = c.layout_transpose_mode == LayoutTransposeMode::force_enable    ? JSVD_LAYOUT_ON
+                              : c.layout_transpose_mode == LayoutTransposeMode::force_disable ? JSVD_LAYOUT_OFF
+                                                                                               : JSVD_LAYOUT_AUTO;
+            out.layout_min_columns = c.layout_transpose_min_columns;
+            out.layout_min_elements = c.layout_transpose_min_elements;
+            out.layout_auto_tune = c.layout_transpose_auto_tune ? 1 : 0;
+            out.layout_bench_reps = c.layout_transpose_benchmark_repetitions;
+            out.layout_bench_sweeps = c.layout_transpose_benchmark_sweeps;
+            out.precision = c.precision == JacobiPrecision::mixed ? JSVD_PRECISION_MIXED : JSVD_PRECISION_FP64;
+            out.block_width = c.block_width;
+            out.refine_sweeps = c.refine_sweeps;
+            return out;
+        }
+
+        template <typename T>
+        struct Pinned
+        {
+            T *ptr = nullptr;
+            std::size_t count = 0;
+            ~Pinned()
+            {
+                if (ptr)
+                    (void)hipHostFree(ptr);
+            }
+            bool reserve(std::size_t n)
+            {
+                if (n <= count)
+                    return false;
+                if (ptr)
+                    hip_check(hipHostFree(ptr), "hipHostFree");
+                ptr = nullptr;
+                hip_check(hipHostMalloc(reinterpret_cast<void **>(&ptr), std::max<std::size_t>(n, 1) * sizeof(T), 0),
+                          "hipHostMalloc");
+                count = n;
+                return true;
+            }
+        };
+
+        template <typename T>
+        struct Device
+        {
+            T *ptr = nullptr;
+            std::size_t count = 0;
+            ~Device()
+            {
+                if (ptr)
+                    (void)hipFree(ptr);
+            }
+            bool reserve(std::size_t n)
+            {
+                if (n <= count)
+                    return false;
+                if (ptr)
+                    hip_check(hipFree(ptr), "hipFree");
+                ptr = nullptr;
+                hip_check(hipMalloc(reinterpret_cast<void **>(&ptr), std::max<std::size_t>(n, 1) * sizeof(T)),
+                          "hipMalloc");
+                count = n;
+                return true;
+            }
+        };
+
+        // One in-flight device batch.
+        struct Slot
+        {
+            hipStream_t stream = nullptr;
+            hipEvent_t done = nullptr;
+            jsvd_plan *plan = nullptr;
+            hipGraphExec_t graph = nullptr;
+            std::vector<int64_t> m, n;
+            std::size_t a_elems = 0, v_elems = 0, s_elems = 0;
+            bool capturable = false;
+            Pinned<double> h_in, h_u, h_s, h_v;
+            Pinned<int32_t> h_sw;
+            Device<double> d_a, d_u, d_s, d_v;
+            Device<int32_t> d_sw;
+            bool busy = false;
+
+            Slot()
+            {
+                hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+                hip_check(hipEventCreateWithFlags(&done, hipEventDisableTiming), "hipEventCreateWithFlags");
+            }
+            ~Slot()
+            {
+                if (stream)
+                    (void)hipStreamSynchronize(stream);
+                drop_plan();
+                if (done)
+                    (void)hipEventDestroy(done);
+                if (stream)
+                    (void)hipStreamDestroy(stream);
+            }
+            void drop_plan()
+            {
+                if (graph)
+                    (void)hipGraphExecDestroy(graph);
+                graph = nullptr;
+                if (plan)
+                    jsvd_plan_destroy(plan);
+                plan = nullptr;
+            }
+
+            void issue()
+            {
+                hip_check(hipMemcpyAsync(d_a.ptr, h_in.ptr, a_elems * sizeof(double), hipMemcpyHostToDevice, stream),
+                          "hipMemcpyAsync(H2D A)");
+                rethrow(jsvd_plan_execute_f64(plan, d_a.ptr, d_u.ptr, d_s.ptr, d_v.ptr, d_sw.ptr, stream));
+                hip_check(hipMemcpyAsync(h_u.ptr, d_u.ptr, a_elems * sizeof(double), hipMemcpyDeviceToHost, stream),
+                          "hipMemcpyAsync(D2H U)");
+                hip_check(hipMemcpyAsync(h_s.ptr, d_s.ptr, s_elems * sizeof(double), hipMemcpyDeviceToHost, stream),
+                          "hipMemcpyAsync(D2H sigma)");
+                hip_check(hipMemcpyAsync(h_v.ptr, d_v.ptr, v_elems * sizeof(double), hipMemcpyDeviceToHost, stream),
+                          "hipMemcpyAsync(D2H V)");
+                hip_check(hipMemcpyAsync(h_sw.ptr, d_sw.ptr, m.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                         stream),
+                          "hipMemcpyAsync(D2H sweeps)");
+            }
+
+            void submit(const std::vector<io::Matrix> &batch, std::size_t first_index, const jsvd_config &cfg)
+            {
+                std::vector<int64_t> mm(batch.size()), nn(batch.size());
+                for (std::size_t k = 0; k < batch.size(); ++k)
+                {
+                    const io::Matrix &x = batch[k];
+                    if (x.rows == 0 || x.columns == 0)
+                        throw std::invalid_argument("Testcase[" + std::to_string(first_index + k) +
+                                                    "] has zero dimension.");
+                    if (x.values.size() != x.rows * x.columns)
+                        throw std::invalid_argument("Testcase[" + std::to_string(first_index + k) +
+                                                    "] payload size does not match rows*columns.");
+                    mm[k] = static_cast<int64_t>(x.rows);
+                    nn[k] = static_cast<int64_t>(x.columns);
+                }
+                if (!plan || mm != m || nn != n)
+                {
+                    drop_plan();
+                    m = std::move(mm);
+                    n = std::move(nn);
+                    rethrow(jsvd_plan_create(static_cast<int64_t>(m.size()), m.data(), n.data(), &cfg, &plan));
+                    jsvd_plan_info info{};
+                    rethrow(jsvd_plan_get_info(plan, &info));
+                    a_elems = static_cast<std::size_t>(info.a_elements);
+                    v_elems = static_cast<std::size_t>(info.v_elements);
+                    s_elems = static_cast<std::size_t>(info.sigma_elements);
+                    // Cooperative launches of the large-matrix path stay out of graphs.
+                    capturable = info.large_matrices == 0;
+                }
+                bool moved = false;
+                moved |= h_in.reserve(a_elems);
+                moved |= h_u.reserve(a_elems);
+                moved |= h_s.reserve(s_elems);
+                moved |= h_v.reserve(v_elems);
+                moved |= h_sw.reserve(m.size());
+                moved |= d_a.reserve(a_elems);
+                moved |= d_u.reserve(a_elems);
+                moved |= d_s.reserve(s_elems);
+                moved |= d_v.reserve(v_elems);
+                moved |= d_sw.reserve(m.size());
+                if (moved && graph)
+                {
+                    (void)hipGraphExecDestroy(graph);
+                    graph = nullptr;
+                }
+                std::size_t off = 0;
+                for (const io::Matrix &x : batch)
+                {
+                    std::memcpy(h_in.ptr + off, x.values.data(), x.values.size() * sizeof(double));
+                    off += x.values.size();
+                }
+                if (capturable)
+                {
+                    if (!graph)
+                    {
+                        hipGraph_t g = nullptr;
+                        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+                        try
+                        {
+                            issue();
+                        }
+                        catch (...)
+                        {
+                            (void)hipStreamEndCapture(stream, &g);
+                            if (g)
+                                (void)hipGraphDestroy(g);
+                            throw;
+                        }
+                        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+                        const hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                        (void)hipGraphDestroy(g);
+                        hip_check(e, "hipGraphInstantiate");
+                    }
+                    hip_check(hipGraphLaunch(graph, stream), "hipGraphLaunch");
+                }
+                else
+                    issue();
+                hip_check(hipEventRecord(done, stream), "hipEventRecord");
+                busy = true;
+            }
+
+            std::size_t drain(const std::function<void(const io::Matrix &)> &write)
+            {
+                hip_check(hipEventSynchronize(done), "hipEventSynchronize");
+                std::size_t sweeps = 0, a = 0, v = 0, s = 0;
+                for (std::size_t k = 0; k < m.size(); ++k)
+                {
+                    const std::size_t rows = static_cast<std::size_t>(m[k]), cols = static_cast<std::size_t>(n[k]);
+                    io::Matrix out;
+                    out.rows = rows;
+                    out.columns = cols;
+                    out.values.assign(h_u.ptr + a, h_u.ptr + a + rows * cols);
+                    write(out);
+                    out.rows = 1;
+                    out.values.assign(h_s.ptr + s, h_s.ptr + s + cols);
+                    write(out);
+                    out.rows = cols;
+                    out.values.assign(h_v.ptr + v, h_v.ptr + v + cols * cols);
+                    write(out);
+                    a += rows * cols;
+                    s += cols;
+                    v += cols * cols;
+                    sweeps += static_cast<std::size_t>(h_sw.ptr[k]);
+                }
+                busy = false;
+                return sweeps;
+            }
+        };
+    } // namespace
+
+    JacobiSvdPipeline::JacobiSvdPipeline(PipelineConfig config) : config_(std::move(config)) {}
+
+    PipelineReport JacobiSvdPipeline::run() const
+    {
+        if (config_.input_path.empty())
+            throw std::invalid_argument("Pipeline input path is empty.");
+        if (config_.output_path.empty())
+            throw std::invalid_argument("Pipeline output path is empty.");
+        const MatrixFileFormat in_fmt = resolve(config_.input_format, config_.input_path);
+        const MatrixFileFormat out_fmt = resolve(config_.output_format, config_.output_path);
+
+        const JacobiSvdConfig &kc = config_.kernel_config;
+        PipelineReport report;
+        report.layout_transpose_mode = kc.layout_transpose_mode;
+        report.layout_transpose_min_columns = kc.layout_transpose_min_columns;
+        report.layout_transpose_min_elements = kc.layout_transpose_min_elements;
+        jsvd_config cfg = to_c(kc);
+        if (kc.layout_transpose_auto_tune)
+        {
+            // Reference pipeline.cu:39-48: tune before the run, report the result.
+            jsvd_autotune_report tuned{};
+            rethrow(jsvd_autotune(&cfg, &tuned));
+            report.layout_transpose_auto_tuned = tuned.executed != 0;
+            report.layout_transpose_min_columns = tuned.recommended_min_columns;
+            report.layout_transpose_min_elements = tuned.recommended_min_elements;
+            report.layout_transpose_estimated_best_speedup = tuned.estimated_best_speedup;
+            cfg.layout_min_columns = tuned.recommended_min_columns;
+            cfg.layout_min_elements = tuned.recommended_min_elements;
+        }
+
+        std::function<bool(io::Matrix &)> read;
+        std::unique_ptr<io::MatReader> mat_in;
+        std::unique_ptr<io::TxtReader> txt_in;
+        if (in_fmt == MatrixFileFormat::mat)
+        {
+            mat_in = std::make_unique<io::MatReader>(config_.input_path);
+            read = [&](io::Matrix &x) { return io::MatFilePolicy::read_next(*mat_in, x); };
+        }
+        else
+        {
+            txt_in = std::make_unique<io::TxtReader>(config_.input_path);
+            read = [&](io::Matrix &x) { return io::TxtFilePolicy::read_next(*txt_in, x); };
+        }
+        std::function<void(const io::Matrix &)> write;
+        std::function<void()> flush;
+        std::unique_ptr<io::MatWriter> mat_out;
+        std::unique_ptr<io::TxtWriter> txt_out;
+        if (out_fmt == MatrixFileFormat::mat)
+        {
+            mat_out = std::make_unique<io::MatWriter>(config_.output_path);
+            write = [&](const io::Matrix &x) { io::MatFilePolicy::write_next(*mat_out, x); };
+            flush = [&]() { io::MatFilePolicy::flush(*mat_out); };
+        }
+        else
+        {
+            txt_out = std::make_unique<io::TxtWriter>(config_.output_path);
+            write = [&](const io::Matrix &x) { io::TxtFilePolicy::write_next(*txt_out, x); };
+            flush = [&]() { io::TxtFilePolicy::flush(*txt_out); };
+        }
+
+        const std::size_t window = std::clamp<std::size_t>(config_.max_queued_results, 1, 8);
+        std::vector<std::unique_ptr<Slot>> slots;
+        std::size_t next_slot = 0, submitted = 0;
+        std::vector<io::Matrix> batch;
+        std::size_t batch_elems = 0;
+        auto submit = [&]() {
+            if (batch.empty())
+                return;
+            if (slots.size() < window)
+                slots.push_back(std::make_unique<Slot>());
+            Slot &slot = *slots[next_slot];
+            if (slot.busy)
+                report.total_sweeps += slot.drain(write);
+            slot.submit(batch, submitted, cfg);
+            submitted += batch.size();
+            next_slot = (next_slot + 1) % window;
+            batch.clear();
+            batch_elems = 0;
+        };
+        io::Matrix x;
+        while (read(x))
+        {
+            batch_elems += x.values.size();
+            batch.push_back(std::move(x));
+            x = io::Matrix{};
+            if (batch.size() >= kBatchMatrices || batch_elems >= kBatchElements)
+                submit();
+        }
+        submit();
+        // Drain the remaining slots oldest first.
+        for (std::size_t i = 0; i < slots.size(); ++i)
+        {
+            Slot &slot = *slots[(next_slot + i) % slots.size()];
+            if (slot.busy)
+                report.total_sweeps += slot.drain(write);
+        }
+        flush();
+        report.testcase_count = submitted;
+        report.emitted_matrix_count = 3 * submitted;
+        return report;
+    }
+
+    PipelineReport run_pipeline(const PipelineConfig &config)
+    {
+        return JacobiSvdPipeline(config).run();
+    }
+} // namespace jacobi::svd::pipeline

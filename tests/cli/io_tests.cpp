@@ -1,0 +1,125 @@
+// C++ matrix-stream codec tests (host only).  Round trips of .mat and .txt,
+// the reference's error cases (truncated header / payload, ragged text rows,
+// bad tokens), and a dump of an externally written .mat file (argv[1]) that
+// the Python test compares with what it wrote.
+#include "jacobi/svd/io/io.cuh"
+
+#include <cmath>
+#include <cstdio>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <stdexcept>
+
+namespace fs = std::filesystem;
+using jacobi::svd::io::Matrix;
+
+namespace
+{
+    int failures = 0;
+    void check(bool ok, const char *what)
+    {
+        if (!ok)
+        {
+            std::cerr << "FAILED: " << what << '\n';
+            ++failures;
+        }
+    }
+
+    template <typename F>
+    bool throws(F f, const std::string &needle)
+    {
+        try
+        {
+            f();
+        }
+        catch (const std::exception &e)
+        {
+            return std::string(e.what()).find(needle) != std::string::npos;
+        }
+        return false;
+    }
+
+    std::vector<Matrix> sample()
+    {
+        std::vector<Matrix> ms;
+        ms.push_back({2, 3, {1.0, -2.5, 3.25, 1e-300, -0.0, 7.0}});
+        ms.push_back({1, 1, {std::acos(-1.0)}});
+        Matrix big{40, 17, {}};
+        for (std::size_t i = 0; i < 40 * 17; ++i)
+            big.values.push_back(std::sin(double(i)) * std::pow(10.0, double(int(i % 13) - 6)));
+        ms.push_back(big);
+        return ms;
+    }
+
+    bool same(const std::vector<Matrix> &a, const std::vector<Matrix> &b)
+    {
+        if (a.size() != b.size())
+            return false;
+        for (std::size_t k = 0; k < a.size(); ++k)
+            if (a[k].rows != b[k].rows || a[k].columns != b[k].columns || a[k].values != b[k].values)
+                return false;
+        return true;
+    }
+} // namespace
+
+int main(int argc, char **argv)
+{
+    if (argc == 2)
+    {
+        // Dump mode: rows cols then values, one matrix per line, %.17g.
+        for (const Matrix &m : jacobi::svd::io::read_mat_file(argv[1]))
+        {
+            std::printf("%zu %zu", m.rows, m.columns);
+            for (double x : m.values)
+                std::printf(" %.17g", x);
+            std::printf("\n");
+        }
+        return 0;
+    }
+    const fs::path dir = fs::temp_directory_path();
+    const std::vector<Matrix> ms = sample();
+    const fs::path mat = dir / "jsvd-io-test.mat", txt = dir / "jsvd-io-test.txt";
+    jacobi::svd::io::write_mat_file(mat, ms);
+    check(same(jacobi::svd::io::read_mat_file(mat), ms), ".mat round trip is exact");
+    check(fs::file_size(mat) == 3 * 16 + 8 * (6 + 1 + 40 * 17), ".mat size = headers + payloads");
+    jacobi::svd::io::write_txt_file(txt, ms);
+    check(same(jacobi::svd::io::read_txt_file(txt), ms), ".txt round trip is exact (max_digits10)");
+    {
+        jacobi::svd::io::MatrixInputStream<jacobi::svd::io::MatFilePolicy> in(mat);
+        Matrix m;
+        int count = 0;
+        while (in.read_one(m))
+            ++count;
+        check(count == 3 && in.eof(), "MatrixInputStream reads every record");
+    }
+    // Truncations.
+    fs::resize_file(mat, 10);
+    check(throws([&] { (void)jacobi::svd::io::read_mat_file(mat); }, "Truncated *.mat header."), "truncated header");
+    jacobi::svd::io::write_mat_file(mat, ms);
+    fs::resize_file(mat, 16 + 8 * 3);
+    check(throws([&] { (void)jacobi::svd::io::read_mat_file(mat); }, "Truncated *.mat payload."), "truncated payload");
+    // Text errors.
+    std::ofstream(txt) << "1 2\n3\n";
+    check(throws([&] { (void)jacobi::svd::io::read_txt_file(txt); }, "Inconsistent column count"), "ragged rows");
+    std::ofstream(txt) << "1 x\n";
+    check(throws([&] { (void)jacobi::svd::io::read_txt_file(txt); }, "Invalid numeric token"), "bad token");
+    std::ofstream(txt) << "\n\n1 2\n3 4\n\n\n5\n";
+    {
+        const auto r = jacobi::svd::io::read_txt_file(txt);
+        check(r.size() == 2 && r[0].rows == 2 && r[1].columns == 1, "blank lines separate blocks");
+    }
+    check(throws([&] { jacobi::svd::io::write_mat_file(mat, std::vector<Matrix>{{2, 2, {1.0}}}); },
+                 "payload size"),
+          "layout check on write");
+    fs::remove(mat);
+    fs::remove(txt);
+    if (failures)
+    {
+        std::cerr << failures << " IO check(s) failed\n";
+        return 1;
+    }
+    std::cout << "io_tests passed\n";
+    return 0;
+}
