@@ -68,7 +68,9 @@ typedef struct jsvd_config
     int32_t layout_bench_sweeps; /* default 8 */
     /* --- no reference counterpart --- */
     int32_t precision;           /* JSVD_PRECISION_*; default FP64 */
-    int32_t block_width;         /* 0 = plain pairs; 16 or 32 = blocked Jacobi for large matrices */
+    int32_t block_width;         /* 0 (default) = column pairs everywhere; 16 or 32 = matrices too large
+                                    for LDS with n >= 2*block_width use blocked Jacobi (Gram blocks and
+                                    block updates on FP64 MFMA; same convergence test and sweep count) */
     int32_t refine_sweeps;       /* mixed: FP64 sweep cap after the FP32 phase; 0 (default) = max_sweeps */
 } jsvd_config;
 
@@ -104,7 +106,7 @@ typedef struct jsvd_plan_info
 {
     int64_t count;             /* matrices in the plan */
     int64_t lds_matrices;      /* solved by the LDS-resident work-list kernel */
-    int64_t large_matrices;    /* solved by the persistent grid-barrier kernel */
+    int64_t large_matrices;    /* too large for LDS: grid-barrier kernel or blocked Jacobi */
     int64_t a_elements;        /* packed A / U elements */
     int64_t v_elements;        /* packed V elements */
     int64_t sigma_elements;    /* packed sigma elements */
@@ -113,7 +115,7 @@ typedef struct jsvd_plan_info
     int32_t lds_bytes;         /* dynamic LDS per workgroup of that bucket */
     int32_t precision;
     int32_t buckets;           /* work-list launches (shape class x LDS occupancy) */
-    int32_t reserved;
+    int32_t blocked_matrices;  /* large matrices on the blocked path */
 } jsvd_plan_info;
 
 /* Builds the device work-list for count matrices of shapes m[k] x n[k].
@@ -123,7 +125,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
 
 /* Runs the plan on device buffers (packed layout above) on `stream`
  * (a hipStream_t, or NULL for the null stream).  Asynchronous.  d_sweeps may
- * be NULL.  For JSVD_PRECISION_MIXED the inputs and outputs are still FP64. */
+ * be NULL.  For JSVD_PRECISION_MIXED the inputs and outputs are still FP64.
+ * A plan with blocked matrices (info.blocked_matrices > 0) waits for the
+ * device once per sweep of each of them to read its convergence flag. */
 jsvd_status jsvd_plan_execute_f64(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma, double *d_v,
                                   int32_t *d_sweeps, void *stream);
 

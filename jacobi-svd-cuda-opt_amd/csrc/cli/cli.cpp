@@ -56,6 +56,17 @@ namespace jacobi::svd::cli
             throw CliArgumentError("Invalid value for --" + std::string(name) + ": expected one of {fp64, mixed}.");
         }
 
+        int parse_block_width(std::string_view raw, std::string_view name)
+        {
+            if (raw == "0")
+                return 0;
+            if (raw == "16")
+                return 16;
+            if (raw == "32")
+                return 32;
+            throw CliArgumentError("Invalid value for --" + std::string(name) + ": expected one of {0, 16, 32}.");
+        }
+
         std::optional<pipeline::MatrixFileFormat> format_from_extension(const std::filesystem::path &path)
         {
             const std::string ext = lower(path.extension().string());
@@ -157,6 +168,8 @@ namespace jacobi::svd::cli
               {OptionId::help, "help", 'h', false, "", "Show this help message."},
               {OptionId::version, "version", 'v', false, "", "Show version information."},
               {OptionId::precision, "precision", '\0', true, "P", "Arithmetic: fp64|mixed (default fp64)."},
+              {OptionId::block_width, "block-width", '\0', true, "B",
+               "Blocked Jacobi for large matrices: 0 (off, default), 16 or 32."},
           }
     {
     }
@@ -311,6 +324,7 @@ namespace jacobi::svd::cli
         case OptionId::help: r.action = ParseAction::help; return;
         case OptionId::version: r.action = ParseAction::version; return;
         case OptionId::precision: opt.precision = parse_precision(*v, o.long_name); return;
+        case OptionId::block_width: opt.block_width = parse_block_width(*v, o.long_name); return;
         }
     }
 
@@ -403,6 +417,7 @@ namespace jacobi::svd::cli
             .layout_transpose_benchmark_repetitions = o.layout_transpose_benchmark_repetitions,
             .layout_transpose_benchmark_sweeps = o.layout_transpose_benchmark_sweeps,
             .precision = o.precision,
+            .block_width = o.block_width,
         };
         return c;
     }
@@ -448,7 +463,8 @@ namespace jacobi::svd::cli
                   << "layout-bench-swp: " << o.layout_transpose_benchmark_sweeps << '\n'
                   << "queue-capacity  : " << o.queue_capacity << '\n'
                   << "force-overwrite : " << (o.force_overwrite ? "true" : "false") << '\n'
-                  << "precision       : " << (o.precision == JacobiPrecision::mixed ? "mixed" : "fp64") << '\n';
+                  << "precision       : " << (o.precision == JacobiPrecision::mixed ? "mixed" : "fp64") << '\n'
+                  << "block-width     : " << o.block_width << '\n';
     }
 
     void print_text_report(const pipeline::PipelineReport &r, double ms)

@@ -51,8 +51,10 @@ namespace jacobi::svd::cli
         bool print_config = false;
         bool json_report = false;
         bool quiet = false;
-        // No reference counterpart: arithmetic of the solve (--precision fp64|mixed).
+        // No reference counterpart: arithmetic of the solve (--precision fp64|mixed)
+        // and the column block width of large matrices (--block-width 0|16|32).
         JacobiPrecision precision = JacobiPrecision::fp64;
+        int block_width = 0;
     };
 
     struct ParseResult final
@@ -66,7 +68,7 @@ namespace jacobi::svd::cli
         input, output, input_format, output_format, format, epsilon, max_sweeps, threads_per_block,
         layout_transpose_mode, layout_transpose_min_columns, layout_transpose_min_elements,
         layout_transpose_auto_tune, layout_transpose_benchmark_repetitions, layout_transpose_benchmark_sweeps,
-        queue_capacity, force, dry_run, print_config, json_report, quiet, help, version, precision
+        queue_capacity, force, dry_run, print_config, json_report, quiet, help, version, precision, block_width
     };
 
     struct OptionDefinition final

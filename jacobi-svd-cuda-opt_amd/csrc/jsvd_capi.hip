@@ -110,6 +110,41 @@ namespace
         return 512;
     }
 
+    // Blocked path: matrices too large for LDS with at least two column
+    // blocks when the caller asked for block_width 16 or 32.
+    bool uses_blocked(const jsvd_config &cfg, int64_t n)
+    {
+        return cfg.block_width != 0 && n >= 2 * int64_t(cfg.block_width);
+    }
+
+    // Column blocking and row chunking of one matrix on the blocked path:
+    // zero columns pad n to nv = nb * block with nb even (no idle block in
+    // any round), zero rows pad m to lda (a multiple of 64); row chunks
+    // (multiples of 64 rows) are sized for about 512 workgroups per Gram
+    // launch.
+    struct BlockedShape
+    {
+        int block = 0, nb = 0, nv = 0, pairs = 0, chunks = 0, vchunks = 0;
+        int64_t lda = 0, chunk_rows = 0, vchunk_rows = 0;
+    };
+
+    BlockedShape blocked_shape(int block, int64_t m, int64_t n)
+    {
+        BlockedShape s;
+        s.block = block;
+        s.nb = static_cast<int>((n + 2 * block - 1) / (2 * block)) * 2;
+        s.nv = s.nb * block;
+        s.pairs = s.nb / 2;
+        s.lda = (m + 63) / 64 * 64;
+        const int64_t panels = s.lda / 64;
+        const int64_t chunks = std::clamp<int64_t>((512 + s.pairs - 1) / s.pairs, 1, panels);
+        s.chunk_rows = ((panels + chunks - 1) / chunks) * 64;
+        s.chunks = static_cast<int>((s.lda + s.chunk_rows - 1) / s.chunk_rows);
+        s.vchunk_rows = s.chunk_rows;
+        s.vchunks = static_cast<int>((s.nv + s.vchunk_rows - 1) / s.vchunk_rows);
+        return s;
+    }
+
     int persistent_kernel_block(int raw)
     {
         const int w = normalize_workgroup(raw);
@@ -241,6 +276,13 @@ struct jsvd_plan
     DevBuf<unsigned int> ws_bar;
     int big_block = 256;
     hipEvent_t big_in = nullptr, big_out = nullptr;
+    // Blocked Jacobi (cfg.block_width 16 or 32): per-round partial Grams,
+    // rotations Q and flags, sized for the largest blocked matrix; the sweep
+    // flag is read back through pinned host memory once per sweep.
+    int64_t blocked = 0;
+    DevBuf<double> bk_gram, bk_q;
+    DevBuf<int32_t> bk_flags;
+    int32_t *h_flag = nullptr;
     int64_t a_elems = 0, v_elems = 0, s_elems = 0;
 
     ~jsvd_plan()
@@ -255,6 +297,8 @@ struct jsvd_plan
             (void)hipEventDestroy(big_in);
         if (big_out)
             (void)hipEventDestroy(big_out);
+        if (h_flag)
+            (void)hipHostFree(h_flag);
     }
 };
 
@@ -276,9 +320,76 @@ namespace
         return JSVD_OK;
     }
 
+    // Writes sigma, U (row-major m x n, columns divided by sigma) and V
+    // (row-major n x n) from the column-major workspaces, whose leading
+    // dimensions are lda >= m and ldv >= n.
+    jsvd_status finish_large(jsvd_plan &p, int64_t k, int64_t m, int n, int64_t lda, int64_t ldv, double *d_u,
+                             double *d_sigma, double *d_v, hipStream_t stream)
+    {
+        JSVD_HIP(jsvd::launch_column_norms_f64(p.ws_a.ptr, lda, p.ws_s.ptr, m, n, stream));
+        JSVD_HIP(hipMemcpyAsync(d_sigma + p.s_off[k], p.ws_s.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                stream));
+        // U (m x n row-major) = transpose of the n x m row-major view of A_cm,
+        // each A column divided by its sigma.
+        JSVD_HIP(jsvd::launch_transpose_ld_f64(p.ws_a.ptr, lda, d_u + p.a_off[k], n, n, m, p.ws_s.ptr,
+                                               p.cfg.epsilon, stream));
+        JSVD_HIP(jsvd::launch_transpose_ld_f64(p.ws_v.ptr, ldv, d_v + p.v_off[k], n, n, n, nullptr, 0.0, stream));
+        return JSVD_OK;
+    }
+
+    // Blocked Jacobi: sweeps of block-pair rounds until a sweep applies no
+    // rotation (one 4-byte read-back per sweep, as the reference's sweep
+    // loop does, src/domain/jacobi_svd.cu:156-185).
+    jsvd_status plan_blocked(jsvd_plan &p, int64_t k, const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                             int32_t *d_sweeps, hipStream_t stream)
+    {
+        const int64_t m = p.m[k];
+        const int n = static_cast<int>(p.n[k]);
+        const BlockedShape sh = blocked_shape(p.cfg.block_width, m, n);
+        // Zero padding rows and columns, then A (row-major) into the
+        // column-major workspace with leading dimension lda.
+        JSVD_HIP(hipMemsetAsync(p.ws_a.ptr, 0, sizeof(double) * sh.lda * sh.nv, stream));
+        JSVD_HIP(jsvd::launch_transpose_ld_f64(d_a + p.a_off[k], n, p.ws_a.ptr, sh.lda, m, n, nullptr, 0.0, stream));
+        JSVD_HIP(jsvd::launch_identity_f64(p.ws_v.ptr, sh.nv, stream));
+        jsvd::BlockedArgs args{};
+        args.block = sh.block;
+        args.nb = sh.nb;
+        args.lda = sh.lda;
+        args.nv = sh.nv;
+        args.a = p.ws_a.ptr;
+        args.v = p.ws_v.ptr;
+        args.chunk_rows = sh.chunk_rows;
+        args.chunks = sh.chunks;
+        args.vchunk_rows = sh.vchunk_rows;
+        args.vchunks = sh.vchunks;
+        args.eps = p.cfg.epsilon;
+        args.gram = p.bk_gram.ptr;
+        args.q = p.bk_q.ptr;
+        args.pair_rotated = p.bk_flags.ptr;
+        args.sweep_flag = p.bk_flags.ptr + sh.pairs;
+        const int rounds = sh.nb - 1;
+        int executed = 0;
+        for (int sweep = 0; sweep < p.cfg.max_sweeps; ++sweep)
+        {
+            JSVD_HIP(jsvd::launch_fill_i32(args.sweep_flag, 0, 1, stream));
+            for (int r = 0; r < rounds; ++r)
+                JSVD_HIP(jsvd::launch_blocked_round(args, r, stream));
+            JSVD_HIP(hipMemcpyAsync(p.h_flag, args.sweep_flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+            JSVD_HIP(hipStreamSynchronize(stream));
+            executed = sweep + 1;
+            if (*p.h_flag == 0)
+                break;
+        }
+        if (d_sweeps)
+            JSVD_HIP(jsvd::launch_fill_i32(d_sweeps + k, executed, 1, stream));
+        return finish_large(p, k, m, n, sh.lda, sh.nv, d_u, d_sigma, d_v, stream);
+    }
+
     jsvd_status plan_large(jsvd_plan &p, int64_t k, const double *d_a, double *d_u, double *d_sigma, double *d_v,
                            int32_t *d_sweeps, hipStream_t stream)
     {
+        if (uses_blocked(p.cfg, p.n[k]))
+            return plan_blocked(p, k, d_a, d_u, d_sigma, d_v, d_sweeps, stream);
         const int64_t m = p.m[k];
         const int n = static_cast<int>(p.n[k]);
         const double *a = d_a + p.a_off[k];
@@ -298,14 +409,7 @@ namespace
         JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n,
                                                     p.cfg.epsilon, p.cfg.max_sweeps, p.ws_flags.ptr,
                                                     p.ws_bar.ptr, sweeps_slot, stream));
-        JSVD_HIP(jsvd::launch_column_norms_f64(p.ws_a.ptr, p.ws_s.ptr, m, n, stream));
-        JSVD_HIP(hipMemcpyAsync(d_sigma + p.s_off[k], p.ws_s.ptr, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                stream));
-        // U (m x n row-major) = transpose of the n x m row-major view of A_cm,
-        // each A column divided by its sigma.
-        JSVD_HIP(jsvd::launch_transpose_f64(p.ws_a.ptr, d_u + p.a_off[k], n, m, p.ws_s.ptr, p.cfg.epsilon, stream));
-        JSVD_HIP(jsvd::launch_transpose_f64(p.ws_v.ptr, d_v + p.v_off[k], n, n, nullptr, 0.0, stream));
-        return JSVD_OK;
+        return finish_large(p, k, m, n, m, n, d_u, d_sigma, d_v, stream);
     }
 
     struct DeviceContext
@@ -398,6 +502,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     plan->v_off.resize(count);
     plan->s_off.resize(count);
     int64_t ao = 0, vo = 0, so = 0, big_mn = 0, big_nn = 0, big_n = 0;
+    int64_t big_gram = 0, big_q = 0, big_pairs = 0;
     hipDeviceProp_t prop;
     JSVD_HIP(hipGetDeviceProperties(&prop, plan->device));
     const int generic_block = lds_kernel_block(cfg.threads_per_block);
@@ -453,8 +558,20 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         else
         {
             plan->large.push_back(k);
-            big_mn = std::max(big_mn, m[k] * n[k]);
-            big_nn = std::max(big_nn, n[k] * n[k]);
+            int64_t nv = n[k], rows = m[k];
+            if (uses_blocked(cfg, n[k]))
+            {
+                const BlockedShape sh = blocked_shape(cfg.block_width, m[k], n[k]);
+                const int64_t w2 = 4 * int64_t(sh.block) * sh.block;
+                nv = sh.nv;
+                rows = sh.lda;
+                big_gram = std::max(big_gram, int64_t(sh.pairs) * sh.chunks * w2);
+                big_q = std::max(big_q, int64_t(sh.pairs) * w2);
+                big_pairs = std::max(big_pairs, int64_t(sh.pairs));
+                ++plan->blocked;
+            }
+            big_mn = std::max(big_mn, rows * nv);
+            big_nn = std::max(big_nn, nv * nv);
             big_n = std::max(big_n, n[k]);
         }
     }
@@ -606,6 +723,13 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         JSVD_HIP(hipMemset(plan->ws_bar.ptr, 0, sizeof(unsigned int) * 2));
         JSVD_HIP(hipEventCreateWithFlags(&plan->big_in, hipEventDisableTiming));
         JSVD_HIP(hipEventCreateWithFlags(&plan->big_out, hipEventDisableTiming));
+        if (plan->blocked > 0)
+        {
+            JSVD_CHECK(plan->bk_gram.alloc(big_gram));
+            JSVD_CHECK(plan->bk_q.alloc(big_q));
+            JSVD_CHECK(plan->bk_flags.alloc(big_pairs + 1));
+            JSVD_HIP(hipHostMalloc(reinterpret_cast<void **>(&plan->h_flag), sizeof(int32_t), hipHostMallocDefault));
+        }
     }
     JSVD_HIP(hipDeviceSynchronize());
     *out = plan.release();
@@ -724,6 +848,7 @@ jsvd_status jsvd_plan_get_info(const jsvd_plan *p, jsvd_plan_info *info)
     info->lds_bytes = p->buckets.empty() ? 0 : p->buckets[0].lds_bytes;
     info->buckets = static_cast<int32_t>(p->buckets.size());
     info->precision = p->cfg.precision;
+    info->blocked_matrices = static_cast<int32_t>(p->blocked);
     return JSVD_OK;
 }
 

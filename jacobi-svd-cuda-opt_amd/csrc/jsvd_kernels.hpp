@@ -117,14 +117,47 @@ namespace jsvd
                                             int32_t *sweeps_out, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
 
+    // Blocked Jacobi over one column-major matrix whose n columns are padded
+    // with zero columns to nv = nb * block and whose m rows are padded with
+    // zero rows to lda (a multiple of 64).  Per round: partial Grams of each
+    // block pair over `chunks` row chunks of A, the Gram solve, and the
+    // panel update of A (chunks) and V (vchunks).  Round 0 of a sweep rotates
+    // every column pair of each block pair; later rounds rotate only the
+    // pairs across the two blocks.
+    struct BlockedArgs
+    {
+        int block;          // column block width: 16 or 32
+        int nb;             // number of column blocks
+        int64_t lda;        // leading dimension (padded rows) of A, multiple of 64
+        int nv;             // padded column count = rows and columns of V
+        double *a;          // lda x nv column-major
+        double *v;          // nv x nv column-major
+        int64_t chunk_rows; // multiple of 64
+        int chunks;
+        int64_t vchunk_rows; // multiple of 64
+        int vchunks;
+        double eps;
+        double *gram;          // pairs * chunks * (2 block)^2
+        double *q;             // pairs * (2 block)^2
+        int32_t *pair_rotated; // pairs
+        int32_t *sweep_flag;   // one int, raised when any rotation was applied
+    };
+    hipError_t launch_blocked_round(const BlockedArgs &args, int round, hipStream_t stream);
+
     // out (cols x rows, row-major) = transpose of in (rows x cols, row-major);
     // when row_scale is given, input row r is divided by row_scale[r] (and
     // written as 0 when row_scale[r] <= eps).
     hipError_t launch_transpose_f64(const double *in, double *out, int64_t rows, int64_t cols,
                                     const double *row_scale, double eps, hipStream_t stream);
+    // Same with explicit leading dimensions: in[r * ld_in + c] ->
+    // out[c * ld_out + r].
+    hipError_t launch_transpose_ld_f64(const double *in, int64_t ld_in, double *out, int64_t ld_out, int64_t rows,
+                                       int64_t cols, const double *row_scale, double eps, hipStream_t stream);
 
-    // sigma[j] = ||column j|| of a column-major m x n matrix.
-    hipError_t launch_column_norms_f64(const double *a_cm, double *sigma, int64_t m, int n, hipStream_t stream);
+    // sigma[j] = ||column j|| of a column-major m x n matrix whose columns
+    // are lda elements apart.
+    hipError_t launch_column_norms_f64(const double *a_cm, int64_t lda, double *sigma, int64_t m, int n,
+                                       hipStream_t stream);
 
     // V <- I (n x n), flags[0..count) <- 0, sweeps slot <- 0.
     hipError_t launch_identity_f64(double *v, int n, hipStream_t stream);

@@ -372,9 +372,10 @@ namespace jsvd
         // Tiled transpose through a padded LDS tile (32 x 33 doubles: the
         // extra column puts the 32 elements of a tile column in distinct bank
         // pairs).  Block 32 x 8.
-        __global__ __launch_bounds__(256) void transpose_f64(const double *__restrict__ in, double *__restrict__ out,
-                                                             int64_t rows, int64_t cols,
-                                                             const double *__restrict__ row_scale, double eps)
+        __global__ __launch_bounds__(256) void transpose_f64(const double *__restrict__ in, int64_t ld_in,
+                                                             double *__restrict__ out, int64_t ld_out, int64_t rows,
+                                                             int64_t cols, const double *__restrict__ row_scale,
+                                                             double eps)
         {
             __shared__ double tile[32][33];
             const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
@@ -385,7 +386,7 @@ namespace jsvd
                 const int64_t r = r0 + ty + k, c = c0 + tx;
                 if (r < rows && c < cols)
                 {
-                    double x = in[r * cols + c];
+                    double x = in[r * ld_in + c];
                     if (row_scale)
                     {
                         const double sc = row_scale[r];
@@ -400,17 +401,17 @@ namespace jsvd
             {
                 const int64_t c = c0 + ty + k, r = r0 + tx;
                 if (c < cols && r < rows)
-                    out[c * rows + r] = tile[tx][ty + k];
+                    out[c * ld_out + r] = tile[tx][ty + k];
             }
         }
 
         template <int BS>
         __global__ __launch_bounds__(BS) void column_norms_f64(const double *__restrict__ a, double *__restrict__ sigma,
-                                                               int64_t m)
+                                                               int64_t m, int64_t lda)
         {
             constexpr int WAVES = BS / 64;
             __shared__ double red[WAVES];
-            const double *col = a + (int64_t)blockIdx.x * m;
+            const double *col = a + (int64_t)blockIdx.x * lda;
             double acc = 0.0;
             for (int64_t i = threadIdx.x; i < m; i += BS)
             {
@@ -527,21 +528,28 @@ namespace jsvd
         return hipGetLastError();
     }
 
-    hipError_t launch_transpose_f64(const double *in, double *out, int64_t rows, int64_t cols,
-                                    const double *row_scale, double eps, hipStream_t stream)
+    hipError_t launch_transpose_ld_f64(const double *in, int64_t ld_in, double *out, int64_t ld_out, int64_t rows,
+                                       int64_t cols, const double *row_scale, double eps, hipStream_t stream)
     {
         if (rows == 0 || cols == 0)
             return hipSuccess;
         dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
-        transpose_f64<<<grid, dim3(32, 8), 0, stream>>>(in, out, rows, cols, row_scale, eps);
+        transpose_f64<<<grid, dim3(32, 8), 0, stream>>>(in, ld_in, out, ld_out, rows, cols, row_scale, eps);
         return hipGetLastError();
     }
 
-    hipError_t launch_column_norms_f64(const double *a_cm, double *sigma, int64_t m, int n, hipStream_t stream)
+    hipError_t launch_transpose_f64(const double *in, double *out, int64_t rows, int64_t cols,
+                                    const double *row_scale, double eps, hipStream_t stream)
+    {
+        return launch_transpose_ld_f64(in, cols, out, rows, rows, cols, row_scale, eps, stream);
+    }
+
+    hipError_t launch_column_norms_f64(const double *a_cm, int64_t lda, double *sigma, int64_t m, int n,
+                                       hipStream_t stream)
     {
         if (n == 0)
             return hipSuccess;
-        column_norms_f64<256><<<n, 256, 0, stream>>>(a_cm, sigma, m);
+        column_norms_f64<256><<<n, 256, 0, stream>>>(a_cm, sigma, m, lda);
         return hipGetLastError();
     }
 

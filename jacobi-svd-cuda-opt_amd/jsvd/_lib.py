@@ -51,7 +51,7 @@ class JsvdPlanInfo(ctypes.Structure):
         ("lds_bytes", ctypes.c_int32),
         ("precision", ctypes.c_int32),
         ("buckets", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("blocked_matrices", ctypes.c_int32),
     ]
 
 

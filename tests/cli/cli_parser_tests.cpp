@@ -100,6 +100,7 @@ int main()
     check(rejects({"jsvd", "--format", "csv", "a.mat"}, "expected one of {auto, mat, txt}"), "bad format");
     check(rejects({"jsvd", "--layout-transpose-mode", "maybe", "a.mat"}, "{auto, on, off}"), "bad layout mode");
     check(rejects({"jsvd", "--precision", "fp16", "a.mat"}, "{fp64, mixed}"), "bad precision");
+    check(rejects({"jsvd", "--block-width", "8", "a.mat"}, "{0, 16, 32}"), "bad block width");
     check(rejects({"jsvd", "--dry-run", "a.bin"}, "Cannot infer input format"), "unknown extension");
     check(rejects({"jsvd", "--dry-run", "missing-input-file.mat"}, "Input file does not exist"), "missing file");
     check(rejects({"jsvd", "a.mat", "b.mat", "c.mat"}, "Unexpected positional argument: c.mat"), "extra positional");
@@ -109,12 +110,13 @@ int main()
         const fs::path out = touch("jsvd-cli-exists.txt", "x");
         check(rejects({"jsvd", "--dry-run", in.string(), out.string()}, "Use --force"), "existing output");
         const ParseResult r = parse({"jsvd", "--dry-run", "--force", "--output=" + out.string(), "--input=" + in.string(),
-                                     "--precision", "mixed", "-c8", "-t", "512"});
+                                     "--precision", "mixed", "-c8", "-t", "512", "--block-width=32"});
         check(r.options.force_overwrite && r.options.queue_capacity == 8 && r.options.threads_per_block == 512,
               "inline values and flags");
         // Known extensions leave both formats on auto; the pipeline resolves them.
         check(r.options.output_format == Fmt::auto_detect && r.options.input_format == Fmt::auto_detect, "formats");
         check(r.options.precision == jacobi::svd::JacobiPrecision::mixed, "precision mixed");
+        check(r.options.block_width == 32, "block width");
         const ParseResult r2 = parse({"jsvd", "--dry-run", in.string(), "--output", (fs::temp_directory_path() / "jsvd-noext").string()});
         check(r2.options.output_path.extension() == ".mat" && r2.options.output_format == Fmt::mat,
               "extension appended from input format");

@@ -76,13 +76,14 @@ def test_cli_help_version_and_errors(tmp_path):
 def test_cli_dry_run_normalisation(tmp_path):
     src = tmp_path / "cases.txt"
     jio.write_txt(src, [np.eye(3, 2)])
-    r = _run([CLI, "--dry-run", str(src), "-e", "1e-12", "--precision", "mixed"])
+    r = _run([CLI, "--dry-run", str(src), "-e", "1e-12", "--precision", "mixed", "--block-width", "16"])
     assert r.returncode == 0, r.stderr
     out = dict(line.split(":", 1) for line in r.stdout.strip().split("\n")[1:])
     assert out["output          "].strip() == str(tmp_path / "cases.svd.txt")
     assert out["output-format   "].strip() == "txt"
     assert out["epsilon         "].strip() == "1e-12"
     assert out["precision       "].strip() == "mixed"
+    assert out["block-width     "].strip() == "16"
     (tmp_path / "cases.svd.txt").write_text("x")
     r = _run([CLI, "--dry-run", str(src)])
     assert r.returncode == 2 and "Use --force" in r.stderr
