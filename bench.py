@@ -10,10 +10,12 @@ timed region; outputs stay on the device.
 Prints one JSON line: matrices/s, the work-list kernels' FP64 throughput and
 LDS traffic against the MI355X peaks (HIP events on the launch stream), the
 HBM traffic measured with rocprofv3 counters, the CPU oracle timed on a
-bounded sample, and the mixed-precision vs FP64 comparison on 1024 matrices
-of 128 x 128 with kappa = 10^U(3,8).
+bounded sample, the mixed-precision vs FP64 comparison on 1024 matrices
+of 128 x 128 with kappa = 10^U(3,8), the blocked solve of one 8192 x 2048
+matrix (sweeps, HBM GB/s and FP64 MFMA rate per sweep) and the plain solve
+of one 256 x 256 matrix beside the oracle.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed] [--no-blocked] [--no-single]
     python bench.py --traffic-probe    # one 256 MiB copy + one plan execution,
                                         # for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 """
@@ -145,6 +147,71 @@ def mixed_vs_fp64(torch, jsvd, gen, metrics, steps=3):
     return out
 
 
+def blocked_bytes(m, n, b):
+    """HBM bytes of one blocked sweep: each of the Nb(Nb-1)/2 block-pair visits
+    reads the 2b panel columns of A for the Gram (16 b m) and reads and writes
+    the panels of A and V in the update (32 b (m + n)): 8 b Nb (Nb-1) (3m + 2n)."""
+    nb = -(-n // (2 * b)) * 2
+    return 8.0 * b * nb * (nb - 1) * (3 * m + 2 * n)
+
+
+def blocked_flops(m, n, b):
+    """FP64 flops of one blocked sweep: per visit the upper triangle of the
+    2b x 2b Gram (W (W+1) m) and the two panel products (2 W^2 (m + n))."""
+    nb = -(-n // (2 * b)) * 2
+    w = 2 * b
+    return nb * (nb - 1) / 2 * (w * (w + 1) * m + 2.0 * w * w * (m + n))
+
+
+def blocked_tall(torch, jsvd, gen, metrics):
+    """Single 8192 x 2048 matrix, sigma_k = 10^(-10k/2047), Haar factors, blocked
+    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps)."""
+    a, sig = gen.tall_geometric()
+    m, n = a.shape
+    out = {"workload": "8192x2048, sigma_k=10^(-10k/2047), Haar factors", "epsilon": EPS, "max_sweeps": 80}
+    for bw in (32, 16):
+        plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=80, block_width=bw)
+        bufs = device_buffers(plan, a.ravel(), torch)
+        wall, kernel_ms = time_plan(plan, bufs, 1, 1, torch)
+        A, U, S, V, sw = bufs
+        sweeps = int(sw.item())
+        Am, Um, Vm = A.view(m, n), U.view(m, n), V.view(n, n)
+        res = (torch.linalg.norm(Am - (Um * S) @ Vm.T) / torch.linalg.norm(Am)).item()
+        norm_err, comp_err = metrics.sigma_errors(S.cpu().numpy(), sig)
+        per_sweep_s = kernel_ms * 1e-3 / sweeps
+        byt, fl = blocked_bytes(m, n, bw), blocked_flops(m, n, bw)
+        out[f"block_width_{bw}"] = {
+            "ms": kernel_ms, "sweeps": sweeps, "ms_per_sweep": per_sweep_s * 1e3,
+            "bytes_per_sweep": byt, "hbm_GBs_per_sweep": byt / per_sweep_s / 1e9,
+            "hbm_frac": byt / per_sweep_s / 1e9 / HBM_PEAK_GBS,
+            "flops_per_sweep": fl, "fp64_TFs": fl / per_sweep_s / 1e12,
+            "mfma_frac": fl / per_sweep_s / 1e12 / FP64_PEAK_TFLOPS,
+            "residual": res, "sigma_err_normwise": norm_err, "sigma_err_componentwise": comp_err}
+        plan.close()
+    return out
+
+
+def plain_single(torch, jsvd, gen, cpu):
+    """The reference's CPU-runnable case: one 256 x 256 N(0,1) matrix, plain
+    column-pair Jacobi (grid-barrier kernel), with the oracle beside it."""
+    a = gen.gaussian_square(256)
+    plan = jsvd.BatchPlan([256], [256], epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    bufs = device_buffers(plan, a.ravel(), torch)
+    wall, kernel_ms = time_plan(plan, bufs, 3, 1, torch)
+    out = {"workload": "256x256 N(0,1), eps 1e-15, 30 sweeps", "gpu_ms": kernel_ms,
+           "sweeps": int(bufs[4].item())}
+    plan.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+
+        t0 = time.perf_counter()
+        ref = oracle.solve(a, EPS, MAX_SWEEPS, threads=1)
+        out["oracle_ms_1_thread"] = (time.perf_counter() - t0) * 1e3
+        out["oracle_sweeps"] = int(ref[3])
+    return out
+
+
 def traffic_probe():
     """One 256 MiB device copy (the counter calibration) and one execution of
     the benchmark plan, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs."""
@@ -175,6 +242,8 @@ def main():
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed vs FP64 comparison")
+    ap.add_argument("--no-blocked", action="store_true", help="skip the 8192 x 2048 blocked solve")
+    ap.add_argument("--no-single", action="store_true", help="skip the single 256 x 256 plain solve")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -240,6 +309,10 @@ def main():
     plan.close()
     if not args.no_mixed:
         result["mixed_vs_fp64"] = mixed_vs_fp64(torch, jsvd, gen, metrics)
+    if not args.no_blocked:
+        result["blocked_8192x2048"] = blocked_tall(torch, jsvd, gen, metrics)
+    if not args.no_single:
+        result["plain_256x256"] = plain_single(torch, jsvd, gen, not args.no_cpu)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(mats)
     print(json.dumps(result), flush=True)

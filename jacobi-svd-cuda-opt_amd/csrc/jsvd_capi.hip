@@ -119,13 +119,12 @@ namespace
 
     // Column blocking and row chunking of one matrix on the blocked path:
     // zero columns pad n to nv = nb * block with nb even (no idle block in
-    // any round), zero rows pad m to lda (a multiple of 64); row chunks
-    // (multiples of 64 rows) are sized for about 512 workgroups per Gram
-    // launch.
+    // any round), zero rows pad m to lda (a multiple of 64); Gram row chunks
+    // (multiples of 64 rows) are sized for about 512 workgroups per launch.
     struct BlockedShape
     {
-        int block = 0, nb = 0, nv = 0, pairs = 0, chunks = 0, vchunks = 0;
-        int64_t lda = 0, chunk_rows = 0, vchunk_rows = 0;
+        int block = 0, nb = 0, nv = 0, pairs = 0, chunks = 0;
+        int64_t lda = 0, chunk_rows = 0;
     };
 
     BlockedShape blocked_shape(int block, int64_t m, int64_t n)
@@ -140,8 +139,6 @@ namespace
         const int64_t chunks = std::clamp<int64_t>((512 + s.pairs - 1) / s.pairs, 1, panels);
         s.chunk_rows = ((panels + chunks - 1) / chunks) * 64;
         s.chunks = static_cast<int>((s.lda + s.chunk_rows - 1) / s.chunk_rows);
-        s.vchunk_rows = s.chunk_rows;
-        s.vchunks = static_cast<int>((s.nv + s.vchunk_rows - 1) / s.vchunk_rows);
         return s;
     }
 
@@ -360,8 +357,6 @@ namespace
         args.v = p.ws_v.ptr;
         args.chunk_rows = sh.chunk_rows;
         args.chunks = sh.chunks;
-        args.vchunk_rows = sh.vchunk_rows;
-        args.vchunks = sh.vchunks;
         args.eps = p.cfg.epsilon;
         args.gram = p.bk_gram.ptr;
         args.q = p.bk_q.ptr;

@@ -142,10 +142,12 @@ namespace jsvd::dev
             }
             else
             {
-                while (__hip_atomic_load(&bar[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen)
+                // Poll with relaxed loads (an acquire load would invalidate
+                // this XCD's L2 on every iteration); one acquire fence below.
+                while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
                     __builtin_amdgcn_s_sleep(1);
             }
-            __threadfence();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
     }

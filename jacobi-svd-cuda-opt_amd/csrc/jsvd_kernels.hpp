@@ -121,7 +121,7 @@ namespace jsvd
     // with zero columns to nv = nb * block and whose m rows are padded with
     // zero rows to lda (a multiple of 64).  Per round: partial Grams of each
     // block pair over `chunks` row chunks of A, the Gram solve, and the
-    // panel update of A (chunks) and V (vchunks).  Round 0 of a sweep rotates
+    // panel update of A and V.  Round 0 of a sweep rotates
     // every column pair of each block pair; later rounds rotate only the
     // pairs across the two blocks.
     struct BlockedArgs
@@ -132,10 +132,8 @@ namespace jsvd
         int nv;             // padded column count = rows and columns of V
         double *a;          // lda x nv column-major
         double *v;          // nv x nv column-major
-        int64_t chunk_rows; // multiple of 64
+        int64_t chunk_rows; // rows per Gram chunk, multiple of 64
         int chunks;
-        int64_t vchunk_rows; // multiple of 64
-        int vchunks;
         double eps;
         double *gram;          // pairs * chunks * (2 block)^2
         double *q;             // pairs * (2 block)^2

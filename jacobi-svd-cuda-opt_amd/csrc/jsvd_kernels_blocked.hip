@@ -163,8 +163,8 @@ namespace jsvd
             }
         }
 
-        template <int B>
-        __global__ __launch_bounds__(256) void gram_solve(const double *__restrict__ gram, int chunks, int nb, int round,
+        template <int B, int NT>
+        __global__ __launch_bounds__(NT) void gram_solve(const double *__restrict__ gram, int chunks, int nb, int round,
                                                          double eps, double *__restrict__ qout,
                                                          int32_t *__restrict__ pair_rotated,
                                                          int32_t *__restrict__ sweep_flag)
@@ -172,8 +172,8 @@ namespace jsvd
             constexpr int W = 2 * B;
             constexpr int LDG = W + 1;
             constexpr int NBLK = B * (B + 1) / 2; // 2 x 2 blocks (i <= j) of the inner pairs
-            constexpr int BPT = (NBLK + 255) / 256;
-            constexpr int QPT = (B * W + 255) / 256;
+            constexpr int BPT = (NBLK + NT - 1) / NT;
+            constexpr int QPT = (B * W + NT - 1) / NT;
             __shared__ double G[W * LDG];
             __shared__ double Q[W * LDG];
             __shared__ double rc[B], rs[B];
@@ -184,7 +184,7 @@ namespace jsvd
             if (I < 0 || J < 0)
                 return;
             const double *gp = gram + int64_t(pair) * chunks * (W * W);
-            for (int idx = tid; idx < W * W; idx += 256)
+            for (int idx = tid; idx < W * W; idx += NT)
             {
                 const int i = idx / W, j = idx - i * W;
                 if (i <= j)
@@ -203,7 +203,7 @@ namespace jsvd
 #pragma unroll
             for (int t = 0; t < BPT; ++t)
             {
-                int rem = tid + 256 * t, i = 0;
+                int rem = tid + NT * t, i = 0;
                 if (rem < NBLK)
                 {
                     while (rem >= B - i)
@@ -292,7 +292,7 @@ namespace jsvd
 #pragma unroll
                 for (int t = 0; t < QPT; ++t)
                 {
-                    const int e = tid + 256 * t;
+                    const int e = tid + NT * t;
                     if (e >= B * W)
                         continue;
                     const int k = e / W, row = e - k * W;
@@ -309,7 +309,7 @@ namespace jsvd
             if (rotated)
             {
                 double *q = qout + int64_t(pair) * (W * W);
-                for (int idx = tid; idx < W * W; idx += 256)
+                for (int idx = tid; idx < W * W; idx += NT)
                 {
                     const int i = idx / W, j = idx - i * W;
                     q[idx] = Q[i * LDG + j];
@@ -323,80 +323,117 @@ namespace jsvd
             }
         }
 
-        // [X_I X_J] <- [X_I X_J] Q over one row chunk: the first `chunks`
-        // workgroups of a pair take A (lda rows), the next `vchunks` take V.
+        // [X_I X_J] <- [X_I X_J] Q.  Work units are 128-row pieces of A (ua
+        // per pair) and V (uv per pair) in pair order; workgroup w takes the
+        // contiguous units [w U / grid, (w + 1) U / grid), so a grid of
+        // resident workgroups finishes together and Q^T is reloaded only
+        // when the pair changes.
+        constexpr int kUnitRows = 128;
+
         template <int B>
         __global__ __launch_bounds__(256) void block_update(double *__restrict__ A, double *__restrict__ V, int64_t lda,
-                                                           int nv, int nb, int round, int64_t chunk_rows, int chunks,
-                                                           int64_t vchunk_rows, int vchunks,
+                                                           int nv, int nb, int round, int ua, int uv, int64_t units,
                                                            const double *__restrict__ qin,
                                                            const int32_t *__restrict__ pair_rotated)
         {
             constexpr int W = 2 * B;
             constexpr int T16 = W / 16;
             constexpr int KS = W / 4; // MFMA k-steps
-            const int per = chunks + vchunks;
-            const int pair = blockIdx.x / per, part = blockIdx.x - pair * per;
-            int I, J;
-            pair_blocks(pair, round, nb, I, J);
-            if (I < 0 || J < 0 || pair_rotated[pair] == 0)
-                return;
-            const bool is_v = part >= chunks;
-            double *X = is_v ? V : A;
-            const int64_t ld = is_v ? int64_t(nv) : lda;
-            const int64_t r0 = is_v ? int64_t(part - chunks) * vchunk_rows : int64_t(part) * chunk_rows;
-            const int64_t len = is_v ? vchunk_rows : chunk_rows;
-            const int64_t r1 = r0 + len < ld ? r0 + len : ld;
             const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, i = l & 15, g = l >> 4;
-            // Operand A of result tile tq at k-step ks: element (i, g) of Q^T,
-            // i.e. Q[4 ks + g][16 tq + i].
-            const double *qg = qin + int64_t(pair) * (W * W);
+            const int per = ua + uv;
+            const int64_t u0 = int64_t(blockIdx.x) * units / gridDim.x;
+            const int64_t u1 = int64_t(blockIdx.x + 1) * units / gridDim.x;
+            int cur = -1, I = -1, J = -1;
+            bool active = false;
             double qa[T16][KS];
-#pragma unroll
-            for (int tq = 0; tq < T16; ++tq)
-#pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
-                    qa[tq][ks] = qg[(4 * ks + g) * W + tq * 16 + i];
-            // Operand B at k-step ks: P[base + i][4 ks + g]; panel column
-            // 4 ks + g lies in block I exactly when ks < B / 4.  Addresses are
-            // a per-lane offset plus a wave-uniform one.
-            const int64_t colI = int64_t(I) * B, colJ = int64_t(J) * B - B;
-            const double *Xl = X + int64_t(g) * ld + i;
-            double b[KS];
-            int64_t base = r0 + 16 * wave;
-            if (base < r1)
+            for (int64_t u = u0; u < u1; ++u)
             {
-#pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
-                    b[ks] = Xl[((ks < B / 4 ? colI : colJ) + 4 * ks) * ld + base];
-            }
-            for (; base < r1; base += 64)
-            {
-                const bool next = base + 64 < r1;
-                f64x4 acc[T16];
-#pragma unroll
-                for (int tq = 0; tq < T16; ++tq)
-                    acc[tq] = f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int ks = 0; ks < KS; ++ks)
+                const int pair = static_cast<int>(u / per), part = static_cast<int>(u - int64_t(pair) * per);
+                if (pair != cur)
                 {
+                    cur = pair;
+                    pair_blocks(pair, round, nb, I, J);
+                    active = I >= 0 && J >= 0 && pair_rotated[pair] != 0;
+                    if (active)
+                    {
+                        // Operand A of result tile tq at k-step ks: element
+                        // (i, g) of Q^T, i.e. Q[4 ks + g][16 tq + i].
+                        const double *qg = qin + int64_t(pair) * (W * W);
+#pragma unroll
+                        for (int tq = 0; tq < T16; ++tq)
+#pragma unroll
+                            for (int ks = 0; ks < KS; ++ks)
+                                qa[tq][ks] = qg[(4 * ks + g) * W + tq * 16 + i];
+                    }
+                }
+                if (!active)
+                    continue;
+                const bool is_v = part >= ua;
+                double *X = is_v ? V : A;
+                const int64_t ld = is_v ? int64_t(nv) : lda;
+                const int64_t r0 = int64_t(is_v ? part - ua : part) * kUnitRows;
+                const int64_t r1 = r0 + kUnitRows < ld ? r0 + kUnitRows : ld;
+                // Operand B at k-step ks: P[base + i][4 ks + g]; panel column
+                // 4 ks + g lies in block I exactly when ks < B / 4.  Addresses
+                // are a per-lane offset plus a wave-uniform one.
+                const int64_t colI = int64_t(I) * B, colJ = int64_t(J) * B - B;
+                const double *Xl = X + int64_t(g) * ld + i;
+                double b[KS];
+                int64_t base = r0 + 16 * wave;
+                if (base < r1)
+                {
+#pragma unroll
+                    for (int ks = 0; ks < KS; ++ks)
+                        b[ks] = Xl[((ks < B / 4 ? colI : colJ) + 4 * ks) * ld + base];
+                }
+                for (; base < r1; base += 64)
+                {
+                    const bool next = base + 64 < r1;
+                    f64x4 acc[T16];
 #pragma unroll
                     for (int tq = 0; tq < T16; ++tq)
-                        acc[tq] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[tq][ks], b[ks], acc[tq], 0, 0, 0);
-                    // Operand consumed: fetch the same slot of the next slice.
-                    if (next)
-                        b[ks] = Xl[((ks < B / 4 ? colI : colJ) + 4 * ks) * ld + base + 64];
-                }
-                // Item r of lane l: result column 16 tq + g + 4r, row base + i.
+                        acc[tq] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int tq = 0; tq < T16; ++tq)
-                {
-                    double *dst = X + (tile_column<B>(tq, I, J) + g) * ld + base + i;
+                    for (int ks = 0; ks < KS; ++ks)
+                    {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        dst[4 * r * ld] = acc[tq][r];
+                        for (int tq = 0; tq < T16; ++tq)
+                            acc[tq] = __builtin_amdgcn_mfma_f64_16x16x4f64(qa[tq][ks], b[ks], acc[tq], 0, 0, 0);
+                        // Operand consumed: fetch the same slot of the next slice.
+                        if (next)
+                            b[ks] = Xl[((ks < B / 4 ? colI : colJ) + 4 * ks) * ld + base + 64];
+                    }
+                    // Item r of lane l: result column 16 tq + g + 4r, row base + i.
+#pragma unroll
+                    for (int tq = 0; tq < T16; ++tq)
+                    {
+                        double *dst = X + (tile_column<B>(tq, I, J) + g) * ld + base + i;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            dst[4 * r * ld] = acc[tq][r];
+                    }
                 }
             }
+        }
+
+        // Workgroups of `fn` (256 threads, static LDS only) resident on the
+        // current device at once, cached per device.
+        int resident_workgroups(const void *fn, int slot)
+        {
+            static int cache[16][4] = {};
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16)
+                return 256;
+            if (cache[dev][slot] == 0)
+            {
+                int per_cu = 0, cus = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess || per_cu < 1)
+                    per_cu = 1;
+                if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+                    cus = 256;
+                cache[dev][slot] = per_cu * cus;
+            }
+            return cache[dev][slot];
         }
 
         template <int B>
@@ -405,11 +442,16 @@ namespace jsvd
             const int pairs = (a.nb + (a.nb & 1)) / 2;
             gram_partial<B><<<pairs * a.chunks, 256, 0, stream>>>(a.a, a.lda, a.nb, round, a.chunk_rows, a.chunks,
                                                                 a.gram);
-            gram_solve<B><<<pairs, 256, 0, stream>>>(a.gram, a.chunks, a.nb, round, a.eps, a.q, a.pair_rotated,
+            constexpr int NT = B == 32 ? 1024 : 512;
+            gram_solve<B, NT><<<pairs, NT, 0, stream>>>(a.gram, a.chunks, a.nb, round, a.eps, a.q, a.pair_rotated,
                                                     a.sweep_flag);
-            block_update<B><<<pairs * (a.chunks + a.vchunks), 256, 0, stream>>>(
-                a.a, a.v, a.lda, a.nv, a.nb, round, a.chunk_rows, a.chunks, a.vchunk_rows, a.vchunks, a.q,
-                a.pair_rotated);
+            const int ua = static_cast<int>((a.lda + kUnitRows - 1) / kUnitRows);
+            const int uv = (a.nv + kUnitRows - 1) / kUnitRows;
+            const int64_t units = int64_t(pairs) * (ua + uv);
+            const int slots = resident_workgroups(reinterpret_cast<const void *>(&block_update<B>), B == 32 ? 0 : 1);
+            const int grid = static_cast<int>(units < slots ? units : slots);
+            block_update<B><<<grid, 256, 0, stream>>>(a.a, a.v, a.lda, a.nv, a.nb, round, ua, uv, units, a.q,
+                                                     a.pair_rotated);
             return hipGetLastError();
         }
     } // namespace
