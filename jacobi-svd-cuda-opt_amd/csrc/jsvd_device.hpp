@@ -56,7 +56,10 @@ namespace jsvd::dev
         {
             const double zeta = (qq - pp) / (2.0 * pq);
             const double root = sqrt(1.0 + zeta * zeta);
-            const double t = (zeta >= 0.0) ? (1.0 / (zeta + root)) : (-1.0 / (-zeta + root));
+            // One division: 1 / (|zeta| + root) equals both branches of the
+            // sign-split form bit for bit (also for zeta = -0, which takes +).
+            const double tmag = 1.0 / (fabs(zeta) + root);
+            const double t = (zeta >= 0.0) ? tmag : -tmag;
             c = 1.0 / sqrt(1.0 + t * t);
             s = t * c;
             return true;
@@ -113,6 +116,55 @@ namespace jsvd::dev
         return x;
     }
 
+    // Three independent group all-reduces with their DPP levels interleaved
+    // (each level's moves read registers the previous level's adds wrote, so
+    // three chains fill the wait states one chain would idle through).  Same
+    // results as three group_sum_dpp calls.
+    template <int G>
+    __device__ __forceinline__ void group_sum3_dpp(double &a, double &b, double &c)
+    {
+        if constexpr (G >= 2)
+        {
+            const double ta = dpp_f64<0xB1>(a), tb = dpp_f64<0xB1>(b), tc = dpp_f64<0xB1>(c);
+            a += ta;
+            b += tb;
+            c += tc;
+        }
+        if constexpr (G >= 4)
+        {
+            const double ta = dpp_f64<0x4E>(a), tb = dpp_f64<0x4E>(b), tc = dpp_f64<0x4E>(c);
+            a += ta;
+            b += tb;
+            c += tc;
+        }
+        if constexpr (G >= 8)
+        {
+            const double ta = dpp_f64<0x141>(a), tb = dpp_f64<0x141>(b), tc = dpp_f64<0x141>(c);
+            a += ta;
+            b += tb;
+            c += tc;
+        }
+        if constexpr (G >= 16)
+        {
+            const double ta = dpp_f64<0x140>(a), tb = dpp_f64<0x140>(b), tc = dpp_f64<0x140>(c);
+            a += ta;
+            b += tb;
+            c += tc;
+        }
+        if constexpr (G >= 32)
+        {
+            a += __shfl_xor(a, 16, 64);
+            b += __shfl_xor(b, 16, 64);
+            c += __shfl_xor(c, 16, 64);
+        }
+        if constexpr (G >= 64)
+        {
+            a += __shfl_xor(a, 32, 64);
+            b += __shfl_xor(b, 32, 64);
+            c += __shfl_xor(c, 32, 64);
+        }
+    }
+
     template <typename T>
     __device__ __forceinline__ T wave_sum(T x)
     {
@@ -122,8 +174,9 @@ namespace jsvd::dev
         return x;
     }
 
-    // Sense-free generation barrier across all workgroups of a cooperative
-    // launch.  bar[0] counts arrivals, bar[1] is the generation.  The
+    // Sense-free generation barrier across all workgroups of a launch whose
+    // grid is co-resident (grid <= resident workgroups, one such launch on
+    // the GPU at a time).  bar[0] counts arrivals, bar[1] is the generation.  The
     // agent-scope fences write back and invalidate the per-XCD L2 so that
     // columns written in one round are seen by every XCD in the next.
     __device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned int nblocks)

@@ -109,7 +109,10 @@ namespace jsvd
             }
             const double zeta = (dqq - dpp) / (2.0 * dpq);
             const double root = sqrt(1.0 + zeta * zeta);
-            const double t = (zeta >= 0.0) ? (1.0 / (zeta + root)) : (-1.0 / (-zeta + root));
+            // One division: 1 / (|zeta| + root) equals both branches of the
+            // sign-split form bit for bit (also for zeta = -0, which takes +).
+            const double tmag = 1.0 / (fabs(zeta) + root);
+            const double t = (zeta >= 0.0) ? tmag : -tmag;
             const double cd = 1.0 / sqrt(1.0 + t * t);
             const double sd = t * cd;
             const float c = static_cast<float>(cd), s = static_cast<float>(sd);
