@@ -32,6 +32,8 @@
 #include "jsvd_device.hpp"
 #include "jsvd_kernels.hpp"
 
+#include <atomic>
+
 namespace jsvd
 {
     namespace
@@ -417,23 +419,26 @@ namespace jsvd
         }
 
         // Workgroups of `fn` (256 threads, static LDS only) resident on the
-        // current device at once, cached per device.
+        // current device at once, cached per device (threads that race on the
+        // first query compute and store the same value).
         int resident_workgroups(const void *fn, int slot)
         {
-            static int cache[16][4] = {};
+            static std::atomic<int> cache[16][2];
             int dev = 0;
             if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16)
                 return 256;
-            if (cache[dev][slot] == 0)
+            int v = cache[dev][slot].load(std::memory_order_relaxed);
+            if (v == 0)
             {
                 int per_cu = 0, cus = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess || per_cu < 1)
                     per_cu = 1;
                 if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
                     cus = 256;
-                cache[dev][slot] = per_cu * cus;
+                v = per_cu * cus;
+                cache[dev][slot].store(v, std::memory_order_relaxed);
             }
-            return cache[dev][slot];
+            return v;
         }
 
         template <int B>

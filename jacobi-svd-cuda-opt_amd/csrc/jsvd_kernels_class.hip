@@ -314,36 +314,42 @@ namespace jsvd
                 __syncthreads();
                 init_v();
                 __syncthreads();
-                // The tangents of one sweep are preloaded across the group's
-                // lanes (lane l holds rounds l, l+G, ...), so the log's HBM
-                // latency is paid once per sweep, then broadcast per round.
-                constexpr int RT = (2 * PASSES * GROUPS - 1 + G - 1) / G; // rounds per lane
+                // The tangents are read G rounds at a time, spread across the
+                // group's lanes (lane l holds round j*G + l of chunk j) and
+                // broadcast per round; the next chunk is fetched while the
+                // current one is replayed.
+                constexpr int RT = (2 * PASSES * GROUPS - 1 + G - 1) / G; // chunks per sweep (upper bound)
                 const int group_base = threadIdx.x & ~(G - 1);
+                const int chunks = (rounds + G - 1) / G;
+                auto fetch = [&](int sweep, int j, int i) -> double {
+                    const int k = gid + i * GROUPS;
+                    const int r = lane + j * G;
+                    // Agent-scope loads bypass the CU's vector L1, which may
+                    // still hold this slot's entries from the previous matrix.
+                    return (k < half && r < rounds)
+                               ? __longlong_as_double(static_cast<long long>(__hip_atomic_load(
+                                     reinterpret_cast<unsigned long long *>(tlog + ((int64_t)sweep * rounds + r) * half + k),
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
+                               : 0.0;
+                };
+                static_assert(RT >= 1, "at least one chunk");
                 for (int sweep = 0; sweep < executed; ++sweep)
                 {
-                    double tp[PASSES][RT];
+                    double cur[PASSES], nxt[PASSES];
 #pragma unroll
                     for (int i = 0; i < PASSES; ++i)
                     {
                         circ[i].start(gid + i * GROUPS, even);
-                        const int k = gid + i * GROUPS;
-#pragma unroll
-                        for (int j = 0; j < RT; ++j)
-                        {
-                            const int r = lane + j * G;
-                            // Agent-scope loads bypass the CU's vector L1, which may
-                            // still hold this slot's entries from the previous matrix.
-                            tp[i][j] = (k < half && r < rounds)
-                                           ? __longlong_as_double(static_cast<long long>(__hip_atomic_load(
-                                                 reinterpret_cast<unsigned long long *>(
-                                                     tlog + ((int64_t)sweep * rounds + r) * half + k),
-                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)))
-                                           : 0.0;
-                        }
+                        cur[i] = fetch(sweep, 0, i);
                     }
-#pragma unroll
-                    for (int j = 0; j < RT; ++j)
+                    for (int j = 0; j < chunks; ++j)
                     {
+                        if (j + 1 < chunks)
+                        {
+#pragma unroll
+                            for (int i = 0; i < PASSES; ++i)
+                                nxt[i] = fetch(sweep, j + 1, i);
+                        }
                         for (int rr = 0; rr < G; ++rr)
                         {
                             const int r = j * G + rr;
@@ -353,7 +359,7 @@ namespace jsvd
                             for (int i = 0; i < PASSES; ++i)
                             {
                                 const int k = gid + i * GROUPS;
-                                const double t = __shfl(tp[i][j], group_base + rr, 64);
+                                const double t = __shfl(cur[i], group_base + rr, 64);
                                 if (k < half)
                                 {
                                     const int p = circ[i].p, q = circ[i].q;
@@ -364,6 +370,9 @@ namespace jsvd
                             }
                             __syncthreads();
                         }
+#pragma unroll
+                        for (int i = 0; i < PASSES; ++i)
+                            cur[i] = nxt[i];
                     }
                 }
             }
