@@ -12,10 +12,12 @@ LDS traffic against the MI355X peaks (HIP events on the launch stream), the
 HBM traffic measured with rocprofv3 counters, the CPU oracle timed on a
 bounded sample, the mixed-precision vs FP64 comparison on 1024 matrices
 of 128 x 128 with kappa = 10^U(3,8), the blocked solve of one 8192 x 2048
-matrix (sweeps, HBM GB/s and FP64 MFMA rate per sweep) and the plain solve
-of one 256 x 256 matrix beside the oracle.
+matrix (sweeps, HBM GB/s and FP64 MFMA rate per sweep, the oracle's sweep
+beside it), the plain solve of one 256 x 256 matrix beside the oracle, and
+the adjoint kernel on 512 matrices of 96 x 64 (FP64 and FP32).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed] [--no-blocked] [--no-single]
+                    [--no-backward]
     python bench.py --traffic-probe    # one 256 MiB copy + one plan execution,
                                         # for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 """
@@ -122,8 +124,9 @@ def cpu_baseline(mats, budget_s=12.0):
                       f"one matrix per OpenMP thread on {cores} threads"}
 
 
-def mixed_vs_fp64(torch, jsvd, gen, metrics, steps=3):
-    """1024 matrices 128 x 128, kappa = 10^U(3,8): FP64 plan vs mixed plan."""
+def mixed_vs_fp64(torch, jsvd, gen, metrics, cpu, steps=3):
+    """1024 matrices 128 x 128, kappa = 10^U(3,8): FP64 plan vs mixed plan (and,
+    with cpu, the oracle on a sample of the batch)."""
     mats, _ = gen.conditioned_batch(1024)
     flat, ms, ns = gen.pack(mats)
     out = {"workload": "1024 x 128x128, kappa=10^U(3,8), sigma_k = kappa^(-k/127)", "epsilon": EPS,
@@ -139,6 +142,17 @@ def mixed_vs_fp64(torch, jsvd, gen, metrics, steps=3):
         out[name] = {"matrices_per_s": len(mats) * steps / wall, "ms_per_batch": wall / steps * 1e3,
                      "mean_sweeps": float(sweeps.mean())}
         plan.close()
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+
+        cores = min(os.cpu_count() or 1, 64)
+        sample = mats[:4 * cores]
+        t0 = time.perf_counter()
+        oracle.solve_batch(sample, EPS, MAX_SWEEPS, threads=cores)
+        dt = time.perf_counter() - t0
+        out["oracle_fp64"] = {"matrices_per_s": len(sample) / dt, "cores": cores,
+                              "sample": f"the first {len(sample)} matrices, one per OpenMP thread"}
     norm = [metrics.sigma_errors(sm, sf)[0] for sm, sf in zip(sig["mixed"], sig["fp64"])]
     comp = [metrics.sigma_errors(sm, sf)[1] for sm, sf in zip(sig["mixed"], sig["fp64"])]
     out["mixed_vs_fp64_sigma_err_normwise_max"] = float(np.max(norm))
@@ -163,12 +177,22 @@ def blocked_flops(m, n, b):
     return nb * (nb - 1) / 2 * (w * (w + 1) * m + 2.0 * w * w * (m + n))
 
 
-def blocked_tall(torch, jsvd, gen, metrics):
+def blocked_tall(torch, jsvd, gen, metrics, cpu):
     """Single 8192 x 2048 matrix, sigma_k = 10^(-10k/2047), Haar factors, blocked
-    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps)."""
+    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps).  With cpu,
+    the oracle's plain sweep is timed beside it (one sweep, all host cores)."""
     a, sig = gen.tall_geometric()
     m, n = a.shape
     out = {"workload": "8192x2048, sigma_k=10^(-10k/2047), Haar factors", "epsilon": EPS, "max_sweeps": 80}
+    if cpu:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle
+
+        cores = min(os.cpu_count() or 1, 64)
+        t0 = time.perf_counter()
+        oracle.solve(a, EPS, 1, threads=cores)
+        out["oracle_plain_s_per_sweep"] = time.perf_counter() - t0
+        out["oracle_cores"] = cores
     for bw in (32, 16):
         plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=80, block_width=bw)
         bufs = device_buffers(plan, a.ravel(), torch)
@@ -212,6 +236,47 @@ def plain_single(torch, jsvd, gen, cpu):
     return out
 
 
+def backward_gapped(torch, jsvd, gen, steps=10):
+    """512 matrices 96 x 64, sigma log-spaced 1 .. 1e-2: forward solve, then the
+    adjoint kernel on seeded upstream gradients, FP64 and FP32.  The adjoint
+    reads U, gU (m n), V, gV (n^2), sigma, gsigma and writes gA (m n): its HBM
+    bytes are 8 (3mn + 2n^2 + 2n) per matrix in FP64 (half in FP32); its flops
+    are 6 m n^2 + 2 n^3 (U^T gU, V^T gV, U N, W V^T)."""
+    mats, _ = gen.gapped_batch(512)
+    flat, ms, ns = gen.pack(mats)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    a, u, s, v, sw = device_buffers(plan, flat, torch)
+    plan.execute(a, u, s, v, sw)
+    torch.cuda.synchronize()
+    g = torch.Generator(device="cpu").manual_seed(5)
+    gu = torch.randn(plan.a_elems, generator=g, dtype=torch.float64).to(u.device)
+    gs = torch.randn(plan.s_elems, generator=g, dtype=torch.float64).to(u.device)
+    gv = torch.randn(plan.v_elems, generator=g, dtype=torch.float64).to(u.device)
+    m, n = 96, 64
+    out = {"workload": "512 x 96x64, sigma_k = 10^(-2k/63), gU, gsigma, gV ~ N(0,1)"}
+    for name, dt in (("fp64", torch.float64), ("fp32", torch.float32)):
+        tens = [t.to(dt) for t in (u, s, v, gu, gs, gv)]
+        ga = torch.empty_like(tens[0])
+        for _ in range(2):
+            plan.backward(*tens, ga)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            plan.backward(*tens, ga)
+        e1.record()
+        torch.cuda.synchronize()
+        ms_per = e0.elapsed_time(e1) / steps
+        elem = 8 if dt == torch.float64 else 4
+        byt = 512 * elem * (3 * m * n + 2 * n * n + 2 * n)
+        fl = 512 * (6.0 * m * n * n + 2.0 * n ** 3)
+        out[name] = {"ms": ms_per, "matrices_per_s": 512 / (ms_per * 1e-3),
+                     "hbm_GBs": byt / (ms_per * 1e-3) / 1e9, "hbm_frac": byt / (ms_per * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "TFs": fl / (ms_per * 1e-3) / 1e12}
+    plan.close()
+    return out
+
+
 def traffic_probe():
     """One 256 MiB device copy (the counter calibration) and one execution of
     the benchmark plan, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs."""
@@ -244,6 +309,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed vs FP64 comparison")
     ap.add_argument("--no-blocked", action="store_true", help="skip the 8192 x 2048 blocked solve")
     ap.add_argument("--no-single", action="store_true", help="skip the single 256 x 256 plain solve")
+    ap.add_argument("--no-backward", action="store_true", help="skip the 512 x 96x64 adjoint")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -308,9 +374,11 @@ def main():
     }
     plan.close()
     if not args.no_mixed:
-        result["mixed_vs_fp64"] = mixed_vs_fp64(torch, jsvd, gen, metrics)
+        result["mixed_vs_fp64"] = mixed_vs_fp64(torch, jsvd, gen, metrics, not args.no_cpu)
     if not args.no_blocked:
-        result["blocked_8192x2048"] = blocked_tall(torch, jsvd, gen, metrics)
+        result["blocked_8192x2048"] = blocked_tall(torch, jsvd, gen, metrics, not args.no_cpu)
+    if not args.no_backward:
+        result["backward_512x96x64"] = backward_gapped(torch, jsvd, gen)
     if not args.no_single:
         result["plain_256x256"] = plain_single(torch, jsvd, gen, not args.no_cpu)
     if not args.no_cpu:
