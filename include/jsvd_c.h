@@ -141,7 +141,8 @@ void jsvd_plan_destroy(jsvd_plan *plan);
  *   gA = U [ (F o (U^T gU - gU^T U)) S + S (F o (V^T gV - gV^T V)) + diag(gsigma) ] V^T
  *        + (I - U U^T) gU S^-1 V^T,   F_ij = 1/(s_j^2 - s_i^2) (i != j),
  * packed like A.  Requires distinct nonzero singular values and
- * (2n^2 + mn) elements of LDS per matrix (n <= 64 at m = 2n in FP64).
+ * (n'^2 + m'(n'+1)) doubles of LDS per matrix, m' and n' being m and n rounded
+ * up to 16 (n <= 80 at m = 2n), in FP64 and FP32 alike.
  * Asynchronous on `stream`; allocates nothing. */
 jsvd_status jsvd_plan_backward_f64(jsvd_plan *plan, const double *d_u, const double *d_sigma, const double *d_v,
                                    const double *d_gu, const double *d_gsigma, const double *d_gv, double *d_ga,

@@ -869,8 +869,8 @@ namespace
             return fail(JSVD_EINVAL, "device buffer pointer is null.");
         const int64_t need = sizeof(T) == 8 ? p->bw_lds_f64 : p->bw_lds_f32;
         if (need > jsvd::kMaxLdsBytes)
-            return fail(JSVD_EINVAL, "SVD backward needs (2n^2 + mn) elements of LDS per matrix; a matrix of this "
-                                     "plan exceeds the 159 KiB budget.");
+            return fail(JSVD_EINVAL, "SVD backward needs (n'^2 + m'(n'+1)) doubles of LDS per matrix (m', n' = m, n "
+                                     "rounded up to 16); a matrix of this plan exceeds the 159 KiB budget.");
         int dev = 0;
         JSVD_HIP(hipGetDevice(&dev));
         if (dev != p->device)

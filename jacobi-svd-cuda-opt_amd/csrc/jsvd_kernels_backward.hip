@@ -8,18 +8,30 @@
 // which is rewritten as gA = (U N + gU diag(s)^-1) V^T with
 //   N = M - J diag(s)^-1
 // so that only four small products are needed.  One workgroup per matrix
-// keeps J, K (then N) and W = U N + gU diag(s)^-1 in LDS; U, gU, V, gV are
-// read from HBM (they stay in L2 across the products).  The result is
-// invariant to a common permutation of (U, s, V) columns, so the unsorted
-// column order of the solver needs no sorting.  Distinct nonzero singular
-// values are required (F and 1/s are unbounded otherwise), as for any SVD
-// adjoint.
+// forms them on FP64 MFMA (v_mfma_f64_16x16x4f64) in 16 x 16 tiles, the four
+// waves taking tiles round-robin; operands are read straight from HBM/L2
+// (U, gU, V, gV) or from LDS (N, W), with zero padding of m and n up to
+// multiples of 16.  LDS holds N (np x np) and W (mp x (np + 1), which holds K
+// while N is formed): (np^2 + mp (np + 1)) doubles, np/mp = n/m rounded up to
+// 16.  FP32 inputs are widened to FP64 on load and the result rounded once.
+// The result is invariant to a common permutation of (U, s, V) columns, so
+// the unsorted column order of the solver needs no sorting.  Distinct
+// nonzero singular values are required (F and 1/s are unbounded otherwise),
+// as for any SVD adjoint.
 #include "jsvd_kernels.hpp"
 
 namespace jsvd
 {
     namespace
     {
+        typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+        template <typename T>
+        __device__ __forceinline__ double load_or_zero(const T *p, int64_t idx, bool ok)
+        {
+            return ok ? static_cast<double>(p[idx]) : 0.0;
+        }
+
         template <typename T, int BS>
         __global__ __launch_bounds__(BS) void svd_backward(const WorkItem *__restrict__ items, int count,
                                                            const T *__restrict__ U, const T *__restrict__ S,
@@ -27,82 +39,141 @@ namespace jsvd
                                                            const T *__restrict__ gS, const T *__restrict__ gV,
                                                            T *__restrict__ gA)
         {
+            constexpr int WAVES = BS / 64;
             extern __shared__ double2 lds_raw[];
-            T *lds = reinterpret_cast<T *>(lds_raw);
+            double *lds = reinterpret_cast<double *>(lds_raw);
             const int item = blockIdx.x;
             if (item >= count)
                 return;
             const WorkItem w = items[item];
-            const int m = w.m, n = w.n, tid = threadIdx.x;
+            const int m = w.m, n = w.n;
+            const int np = (n + 15) & ~15, mp = (m + 15) & ~15;
+            const int tn = np / 16, tm = mp / 16;
+            const int ldw = np + 1; // odd stride: the A-operand reads of W spread over the banks
+            const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, i = l & 15, g = l >> 4;
             const T *u = U + w.a_off, *gu = gU ? gU + w.a_off : nullptr;
             const T *v = V + w.v_off, *gv = gV ? gV + w.v_off : nullptr;
             const T *s = S + w.s_off, *gs = gS ? gS + w.s_off : nullptr;
-            T *J = lds;         // n x n row-major, becomes N
-            T *K = lds + n * n; // n x n row-major
-            T *W = K + n * n;   // m x n row-major
+            double *Nm = lds;           // np x np row-major: J, then N
+            double *Wm = lds + np * np; // mp x ldw row-major: K (first np rows), then W
+            // MFMA operand element (l % 16, l / 16) of A and (l / 16, l % 16) of
+            // B; result item r of lane l is tile row l/16 + 4r, column l % 16.
+
             // J = U^T gU and K = V^T gV.
-            for (int idx = tid; idx < n * n; idx += BS)
+            for (int t = wave; t < 2 * tn * tn; t += WAVES)
             {
-                const int i = idx / n, j = idx - (idx / n) * n;
-                T jacc = 0, kacc = 0;
-                if (gu)
-                    for (int k = 0; k < m; ++k)
-                        jacc += u[k * n + i] * gu[k * n + j];
-                if (gv)
-                    for (int k = 0; k < n; ++k)
-                        kacc += v[k * n + i] * gv[k * n + j];
-                J[idx] = jacc;
-                K[idx] = kacc;
-            }
-            __syncthreads();
-            // N is staged in W's space (m*n >= n*n) because entry (i,j) reads
-            // J_ji and K_ji, then copied over J.
-            for (int idx = tid; idx < n * n; idx += BS)
-            {
-                const int i = idx / n, j = idx - (idx / n) * n;
-                const T si = s[i], sj = s[j];
-                T val;
-                if (i == j)
-                    val = (gs ? gs[i] : T(0)) - J[idx] / si;
-                else
+                const bool is_k = t >= tn * tn;
+                const int tt = is_k ? t - tn * tn : t;
+                const int ti = tt / tn, tj = tt - (tt / tn) * tn;
+                const T *a = is_k ? v : u, *b = is_k ? gv : gu;
+                const int rows = is_k ? n : m;
+                f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+                if (b)
                 {
-                    const T f = T(1) / (sj * sj - si * si);
-                    val = ((J[idx] - J[j * n + i]) * sj + si * (K[idx] - K[j * n + i])) * f - J[idx] / sj;
+                    const int ca = ti * 16 + i, cb = tj * 16 + i;
+#pragma unroll 4
+                    for (int k0 = 0; k0 < rows; k0 += 4)
+                    {
+                        const int k = k0 + g;
+                        const double av = load_or_zero(a, int64_t(k) * n + ca, k < rows && ca < n);
+                        const double bv = load_or_zero(b, int64_t(k) * n + cb, k < rows && cb < n);
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                    }
                 }
-                W[idx] = val;
+                double *dst = is_k ? Wm : Nm;
+                const int ld = is_k ? ldw : np;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    dst[(ti * 16 + g + 4 * r) * ld + tj * 16 + i] = acc[r];
             }
             __syncthreads();
-            for (int idx = tid; idx < n * n; idx += BS)
-                J[idx] = W[idx];
+            // N in place of J: the thread of (r, c), r <= c, also writes (c, r);
+            // padding rows and columns become zero.
+            for (int idx = threadIdx.x; idx < np * np; idx += BS)
+            {
+                const int r = idx / np, c = idx - (idx / np) * np;
+                if (r > c)
+                    continue;
+                double nrc = 0.0, ncr = 0.0;
+                if (c < n)
+                {
+                    const double sr = static_cast<double>(s[r]);
+                    if (r == c)
+                        nrc = (gs ? static_cast<double>(gs[r]) : 0.0) - Nm[r * np + r] / sr;
+                    else
+                    {
+                        const double sc = static_cast<double>(s[c]);
+                        const double f = 1.0 / (sc * sc - sr * sr); // F_rc; F_cr = -F_rc
+                        const double jrc = Nm[r * np + c], jcr = Nm[c * np + r];
+                        const double krc = Wm[r * ldw + c], kcr = Wm[c * ldw + r];
+                        nrc = ((jrc - jcr) * sc + sr * (krc - kcr)) * f - jrc / sc;
+                        ncr = ((jcr - jrc) * sr + sc * (kcr - krc)) * (-f) - jcr / sr;
+                    }
+                }
+                Nm[r * np + c] = nrc;
+                if (r != c)
+                    Nm[c * np + r] = ncr;
+            }
             __syncthreads();
             // W = U N + gU diag(s)^-1.
-            for (int idx = tid; idx < m * n; idx += BS)
+            for (int t = wave; t < tm * tn; t += WAVES)
             {
-                const int i = idx / n, j = idx - (idx / n) * n;
-                T acc = gu ? gu[idx] / s[j] : T(0);
-                for (int k = 0; k < n; ++k)
-                    acc += u[i * n + k] * J[k * n + j];
-                W[idx] = acc;
+                const int ti = t / tn, tj = t - (t / tn) * tn;
+                f64x4 acc;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                {
+                    const int row = ti * 16 + g + 4 * r, col = tj * 16 + i;
+                    acc[r] = (gu && row < m && col < n)
+                                 ? static_cast<double>(gu[int64_t(row) * n + col]) / static_cast<double>(s[col])
+                                 : 0.0;
+                }
+                const int row = ti * 16 + i;
+#pragma unroll 4
+                for (int k0 = 0; k0 < np; k0 += 4)
+                {
+                    const int k = k0 + g;
+                    const double av = load_or_zero(u, int64_t(row) * n + k, row < m && k < n);
+                    const double bv = Nm[k * np + tj * 16 + i];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Wm[(ti * 16 + g + 4 * r) * ldw + tj * 16 + i] = acc[r];
             }
             __syncthreads();
             // gA = W V^T.
             T *ga = gA + w.a_off;
-            for (int idx = tid; idx < m * n; idx += BS)
+            for (int t = wave; t < tm * tn; t += WAVES)
             {
-                const int i = idx / n, j = idx - (idx / n) * n;
-                T acc = 0;
-                for (int k = 0; k < n; ++k)
-                    acc += W[i * n + k] * v[j * n + k];
-                ga[idx] = acc;
+                const int ti = t / tn, tj = t - (t / tn) * tn;
+                f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+                const int c = tj * 16 + i;
+#pragma unroll 4
+                for (int k0 = 0; k0 < np; k0 += 4)
+                {
+                    const int k = k0 + g;
+                    const double av = Wm[(ti * 16 + i) * ldw + k];
+                    const double bv = load_or_zero(v, int64_t(c) * n + k, c < n && k < n);
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                {
+                    const int row = ti * 16 + g + 4 * r;
+                    if (row < m && c < n)
+                        ga[int64_t(row) * n + c] = static_cast<T>(acc[r]);
+                }
             }
         }
     } // namespace
 
     constexpr int kBackwardBlock = 256;
 
-    int64_t backward_lds_bytes(int64_t m, int64_t n, int elem)
+    int64_t backward_lds_bytes(int64_t m, int64_t n, int /*elem*/)
     {
-        return (2 * n * n + m * n) * elem;
+        const int64_t np = (n + 15) / 16 * 16, mp = (m + 15) / 16 * 16;
+        return (np * np + mp * (np + 1)) * 8;
     }
 
     const void *backward_symbol(int elem)
