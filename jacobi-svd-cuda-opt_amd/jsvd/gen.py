@@ -86,3 +86,72 @@ def pack(mats):
     ns = np.array([x.shape[1] for x in mats], dtype=np.int64)
     flat = np.concatenate([np.ascontiguousarray(x, dtype=np.float64).ravel() for x in mats])
     return flat, ms, ns
+
+
+# ---- stress mix (robustness cases) ---------------------------------------
+# The same scenario kinds as the reference generator
+# (psrc/generator/common.py:340-451): dense (uniform, normal, log-uniform, each
+# with a small smooth sin term), diagonal-dominant, low-rank plus noise,
+# ill-conditioned factor product plus noise, sparse and deterministic zero
+# columns.  The random streams are numpy PCG64, not the reference's
+# random.Random, so the values differ; the structure of every kind is the
+# same.  Defaults follow the reference CLI: rank 4, condition 1e8,
+# sparsity 0.5 (psrc/generator/common.py:195-210).
+
+STRESS_KINDS = ("uniform", "normal", "log-uniform", "diagonal-dominant", "low-rank", "ill-conditioned",
+                "sparse", "zero-columns")
+
+
+def stress_matrix(kind: str, m: int, n: int, rng: np.random.Generator, rank: int = 4, condition: float = 1e8,
+                  sparsity: float = 0.5, lo: float = -1.0, hi: float = 1.0) -> np.ndarray:
+    """One m x n matrix of the given stress kind."""
+    if kind in ("uniform", "normal", "log-uniform"):
+        if kind == "uniform":
+            x = rng.uniform(lo, hi, (m, n))
+        elif kind == "normal":
+            x = rng.standard_normal((m, n))
+        else:
+            amin = max(min(abs(lo), abs(hi)), 1e-300)
+            amax = max(abs(lo), abs(hi), amin)
+            mag = np.exp(rng.uniform(np.log(amin), np.log(amax), (m, n)))
+            x = np.where(rng.random((m, n)) < 0.5, mag, -mag)
+        rows = np.arange(1, m + 1)[:, None]
+        cols = np.arange(1, n + 1)[None, :]
+        return x + 0.05 * np.sin(0.013 * rows * cols)
+    if kind == "diagonal-dominant":
+        scale = max(abs(lo), abs(hi), 1.0)
+        x = 0.01 * rng.uniform(lo, hi, (m, n))
+        k = min(m, n)
+        x[np.arange(k), np.arange(k)] += scale + np.abs(rng.uniform(lo, hi, k))
+        return x
+    if kind in ("low-rank", "ill-conditioned"):
+        r = min(rank, m, n)
+        left = rng.standard_normal((m, r))
+        right = rng.standard_normal((r, n))
+        if kind == "low-rank":
+            return left @ right + 1e-6 * rng.standard_normal((m, n))
+        scales = condition ** (-np.arange(r) / max(r - 1, 1))
+        return (left * scales) @ right + 1e-12 * rng.standard_normal((m, n))
+    if kind == "sparse":
+        x = rng.uniform(lo, hi, (m, n))
+        x[rng.random((m, n)) < sparsity] = 0.0
+        return x
+    if kind == "zero-columns":
+        x = rng.uniform(lo, hi, (m, n))
+        stride = max(2, n // 4)
+        x[:, np.arange(n) % stride == 0] = 0.0
+        return x
+    raise ValueError(f"unsupported stress kind: {kind}")
+
+
+def stress_mix(count: int, shapes, seed: int = BASE_SEED, kinds=STRESS_KINDS, **kw):
+    """count matrices cycling through kinds and shapes ((m, n) pairs, m >= n);
+    matrix k uses seed + k.  Returns (mats, kinds_used)."""
+    mats, used = [], []
+    for k in range(count):
+        rng = np.random.default_rng(seed + k)
+        kind = kinds[k % len(kinds)]
+        m, n = shapes[(k // len(kinds)) % len(shapes)]
+        mats.append(np.ascontiguousarray(stress_matrix(kind, m, n, rng, **kw)))
+        used.append(kind)
+    return mats, used
