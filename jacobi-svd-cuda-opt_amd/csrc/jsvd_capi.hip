@@ -1001,9 +1001,28 @@ jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out)
     JSVD_CHECK(device_context(&ctx));
     std::lock_guard<std::mutex> lock(ctx->mu);
     hipStream_t st = ctx->stream;
-    hipEvent_t e0, e1;
-    JSVD_HIP(hipEventCreate(&e0));
-    JSVD_HIP(hipEventCreate(&e1));
+    // Events are released and a bucket's class choice is restored on every
+    // return path (a failed candidate must not stay installed).
+    struct Scope
+    {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int bucket = -1, saved = -1;
+        ~Scope()
+        {
+            if (e0)
+                (void)hipEventDestroy(e0);
+            if (e1)
+                (void)hipEventDestroy(e1);
+            if (bucket >= 0)
+            {
+                std::lock_guard<std::mutex> tl(g_tune_mu);
+                g_tuned[bucket] = saved;
+            }
+        }
+    } scope;
+    JSVD_HIP(hipEventCreate(&scope.e0));
+    JSVD_HIP(hipEventCreate(&scope.e1));
+    hipEvent_t e0 = scope.e0, e1 = scope.e1;
     int64_t lo = 0;
     for (int b = 0; b < kTuneBuckets; ++b)
     {
@@ -1043,6 +1062,11 @@ jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out)
         JSVD_HIP(hipMemcpyAsync(ctx->a.ptr, host.data(), sizeof(double) * total, hipMemcpyHostToDevice, st));
         double t_default = -1.0, t_best = 1e300;
         int best = -1;
+        {
+            std::lock_guard<std::mutex> tl(g_tune_mu);
+            scope.bucket = b;
+            scope.saved = g_tuned[b];
+        }
         for (int cls : cands)
         {
             {
@@ -1072,13 +1096,12 @@ jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out)
         }
         {
             std::lock_guard<std::mutex> tl(g_tune_mu);
-            g_tuned[b] = best;
+            g_tuned[b] = best; // the winner stays installed
+            scope.bucket = -1;
         }
         if (t_default > 0.0 && t_best > 0.0)
             out->estimated_best_speedup = std::max(out->estimated_best_speedup, t_default / t_best);
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return JSVD_OK;
 }
 
