@@ -196,8 +196,10 @@ namespace jsvd
             }
         }
 
+        // Inlined into the kernel: as a call, the kernel-argument struct it
+        // takes by reference would be copied to private (scratch) memory.
         template <int BS, int G, int R, int VMAX2, bool VLOG, int PASSES>
-        __device__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
+        __device__ __forceinline__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
         {
             constexpr int GROUPS = BS / G;
             const int m = w.m, n = w.n, tid = threadIdx.x;
