@@ -76,26 +76,30 @@ namespace jsvd
                                                       int lane, double eps, double *__restrict__ tlog)
         {
             double2 xp[R], xq[R];
-            double pp = 0.0, qq = 0.0, pq = 0.0;
 #pragma unroll
             for (int j = 0; j < R; ++j)
             {
                 xp[j] = ap[lane + j * G];
                 xq[j] = aq[lane + j * G];
             }
+            // Two FMA chains per dot (even and odd rows), joined at the end.
+            double pp0 = 0.0, pp1 = 0.0, qq0 = 0.0, qq1 = 0.0, pq0 = 0.0, pq1 = 0.0;
 #pragma unroll
             for (int j = 0; j < R; ++j)
             {
-                pp += xp[j].x * xp[j].x + xp[j].y * xp[j].y;
-                qq += xq[j].x * xq[j].x + xq[j].y * xq[j].y;
-                pq += xp[j].x * xq[j].x + xp[j].y * xq[j].y;
+                pp0 = fma(xp[j].x, xp[j].x, pp0);
+                pp1 = fma(xp[j].y, xp[j].y, pp1);
+                qq0 = fma(xq[j].x, xq[j].x, qq0);
+                qq1 = fma(xq[j].y, xq[j].y, qq1);
+                pq0 = fma(xp[j].x, xq[j].x, pq0);
+                pq1 = fma(xp[j].y, xq[j].y, pq1);
             }
+            double pp = pp0 + pp1, qq = qq0 + qq1, pq = pq0 + pq1;
             dev::group_sum3_dpp<G>(pp, qq, pq);
             // Threshold test first (reference jacobi_rotation_kernels.cuh:310-312),
-            // then the V loads, whose latency overlaps the rotation parameters.
-            const double rhs = eps * sqrt(fmax(pp, 0.0) * fmax(qq, 0.0));
-            const double apq = fabs(pq);
-            if (!(apq > rhs && apq > 1.0e-300))
+            // then the V loads, issued before the rotation parameters so
+            // that their latency overlaps that computation.
+            if (!dev::rotation_needed(pp, qq, pq, eps))
             {
                 if (VLOG && lane == 0)
                     *tlog = 0.0;
@@ -104,8 +108,8 @@ namespace jsvd
             double2 yp[VLOG ? 1 : VMAX2], yq[VLOG ? 1 : VMAX2];
             if constexpr (!VLOG)
             {
-                // All V loads issue before any use; indices past the matrix
-                // re-read its last row pair (never stored back).
+                // Indices past the matrix re-read its last row pair (never
+                // stored back).
 #pragma unroll
                 for (int j = 0; j < VMAX2; ++j)
                 {
@@ -113,47 +117,42 @@ namespace jsvd
                     yp[j] = vp[i];
                     yq[j] = vq[i];
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            const double zeta = (qq - pp) / (2.0 * pq);
-            const double root = sqrt(1.0 + zeta * zeta);
-            // One division: 1 / (|zeta| + root) equals both branches of the
-            // sign-split form bit for bit (also for zeta = -0, which takes +).
-            const double tmag = 1.0 / (fabs(zeta) + root);
-            const double t = (zeta >= 0.0) ? tmag : -tmag;
-            const double c = 1.0 / sqrt(1.0 + t * t);
-            const double s = t * c;
+            const double t = dev::rotation_tangent(pp, qq, pq);
+            double c, s;
+            dev::rotation_from_tangent(t, c, s);
             if (VLOG && lane == 0)
-                *tlog = (s != 0.0) ? t : 0.0;
-            if (s != 0.0)
+                *tlog = t;
+            // t = +-0 (a tangent below the double range) gives c = 1, s = 0:
+            // the updates below leave both columns unchanged.
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                const int i = lane + j * G;
+                double2 np, nq;
+                np.x = c * xp[j].x - s * xq[j].x;
+                np.y = c * xp[j].y - s * xq[j].y;
+                nq.x = s * xp[j].x + c * xq[j].x;
+                nq.y = s * xp[j].y + c * xq[j].y;
+                ap[i] = np;
+                aq[i] = nq;
+            }
+            if constexpr (!VLOG)
             {
 #pragma unroll
-                for (int j = 0; j < R; ++j)
+                for (int j = 0; j < VMAX2; ++j)
                 {
-                    const int i = lane + j * G;
-                    double2 np, nq;
-                    np.x = c * xp[j].x - s * xq[j].x;
-                    np.y = c * xp[j].y - s * xq[j].y;
-                    nq.x = s * xp[j].x + c * xq[j].x;
-                    nq.y = s * xp[j].y + c * xq[j].y;
-                    ap[i] = np;
-                    aq[i] = nq;
-                }
-                if constexpr (!VLOG)
-                {
-#pragma unroll
-                    for (int j = 0; j < VMAX2; ++j)
+                    if (j < vrows2)
                     {
-                        if (j < vrows2)
-                        {
-                            const int i = lane + j * G;
-                            double2 np, nq;
-                            np.x = c * yp[j].x - s * yq[j].x;
-                            np.y = c * yp[j].y - s * yq[j].y;
-                            nq.x = s * yp[j].x + c * yq[j].x;
-                            nq.y = s * yp[j].y + c * yq[j].y;
-                            vp[i] = np;
-                            vq[i] = nq;
-                        }
+                        const int i = lane + j * G;
+                        double2 np, nq;
+                        np.x = c * yp[j].x - s * yq[j].x;
+                        np.y = c * yp[j].y - s * yq[j].y;
+                        nq.x = s * yp[j].x + c * yq[j].x;
+                        nq.y = s * yp[j].y + c * yq[j].y;
+                        vp[i] = np;
+                        vq[i] = nq;
                     }
                 }
             }
@@ -167,10 +166,8 @@ namespace jsvd
         {
             if (t == 0.0)
                 return;
-            const double c = 1.0 / sqrt(1.0 + t * t);
-            const double s = t * c;
-            if (s == 0.0)
-                return;
+            double c, s;
+            dev::rotation_from_tangent(t, c, s);
             double2 yp[VMAX2], yq[VMAX2];
 #pragma unroll
             for (int j = 0; j < VMAX2; ++j)
@@ -269,9 +266,11 @@ namespace jsvd
                             if (p < n && q < n)
                             {
                                 double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + k : &dummy_t;
-                                any |= pair_class_f64<G, R, VMAX2, VLOG>(A2 + p * lda2, A2 + q * lda2,
-                                                                         V2 + p * ldv2, V2 + q * ldv2, vrows2, lane,
-                                                                         args.eps, slot);
+                                // 24-bit products (full-rate multiplies): column
+                                // indices and leading dimensions are far below 2^24.
+                                any |= pair_class_f64<G, R, VMAX2, VLOG>(
+                                    A2 + __umul24(p, lda2), A2 + __umul24(q, lda2), V2 + __umul24(p, ldv2),
+                                    V2 + __umul24(q, ldv2), vrows2, lane, args.eps, slot);
                             }
                             circ[i].advance(even);
                         }
