@@ -65,15 +65,15 @@ namespace jsvd
         }
 
         // Rotation of one pair held by G lanes; each lane owns R row pairs of
-        // A (R is the matrix's exact count, a template argument, so every row
-        // loop is unrolled without bounds tests and all loads are issued
-        // before any use).  Returns 1 when the pair counts as rotated
+        // A and VR of V (the matrix's exact counts, template arguments, so
+        // every row loop is unrolled without bounds tests and all loads are
+        // issued before any use).  Returns 1 when the pair counts as rotated
         // (reference flag semantics).  With VLOG the group's lane 0 records t
         // (0 when nothing is applied) at *tlog and V is not touched.
-        template <int G, int R, int VMAX2, bool VLOG>
+        template <int G, int R, int VR, bool VLOG>
         __device__ __forceinline__ int pair_class_f64(double2 *__restrict__ ap, double2 *__restrict__ aq,
-                                                      double2 *__restrict__ vp, double2 *__restrict__ vq, int vrows2,
-                                                      int lane, double eps, double *__restrict__ tlog)
+                                                      double2 *__restrict__ vp, double2 *__restrict__ vq, int lane,
+                                                      double eps, double *__restrict__ tlog)
         {
             double2 xp[R], xq[R];
 #pragma unroll
@@ -105,17 +105,14 @@ namespace jsvd
                     *tlog = 0.0;
                 return 0;
             }
-            double2 yp[VLOG ? 1 : VMAX2], yq[VLOG ? 1 : VMAX2];
+            double2 yp[VLOG ? 1 : VR], yq[VLOG ? 1 : VR];
             if constexpr (!VLOG)
             {
-                // Indices past the matrix re-read its last row pair (never
-                // stored back).
 #pragma unroll
-                for (int j = 0; j < VMAX2; ++j)
+                for (int j = 0; j < VR; ++j)
                 {
-                    const int i = lane + (j < vrows2 ? j : vrows2 - 1) * G;
-                    yp[j] = vp[i];
-                    yq[j] = vq[i];
+                    yp[j] = vp[lane + j * G];
+                    yq[j] = vq[lane + j * G];
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -141,45 +138,7 @@ namespace jsvd
             if constexpr (!VLOG)
             {
 #pragma unroll
-                for (int j = 0; j < VMAX2; ++j)
-                {
-                    if (j < vrows2)
-                    {
-                        const int i = lane + j * G;
-                        double2 np, nq;
-                        np.x = c * yp[j].x - s * yq[j].x;
-                        np.y = c * yp[j].y - s * yq[j].y;
-                        nq.x = s * yp[j].x + c * yq[j].x;
-                        nq.y = s * yp[j].y + c * yq[j].y;
-                        vp[i] = np;
-                        vq[i] = nq;
-                    }
-                }
-            }
-            return 1;
-        }
-
-        // Replays one logged rotation on the V columns of a pair.
-        template <int G, int VMAX2>
-        __device__ __forceinline__ void replay_pair_f64(double2 *__restrict__ vp, double2 *__restrict__ vq,
-                                                        int vrows2, int lane, double t)
-        {
-            if (t == 0.0)
-                return;
-            double c, s;
-            dev::rotation_from_tangent(t, c, s);
-            double2 yp[VMAX2], yq[VMAX2];
-#pragma unroll
-            for (int j = 0; j < VMAX2; ++j)
-            {
-                const int i = lane + (j < vrows2 ? j : vrows2 - 1) * G;
-                yp[j] = vp[i];
-                yq[j] = vq[i];
-            }
-#pragma unroll
-            for (int j = 0; j < VMAX2; ++j)
-            {
-                if (j < vrows2)
+                for (int j = 0; j < VR; ++j)
                 {
                     const int i = lane + j * G;
                     double2 np, nq;
@@ -191,11 +150,42 @@ namespace jsvd
                     vq[i] = nq;
                 }
             }
+            return 1;
+        }
+
+        // Replays one logged rotation on the V columns of a pair.
+        template <int G, int VR>
+        __device__ __forceinline__ void replay_pair_f64(double2 *__restrict__ vp, double2 *__restrict__ vq, int lane,
+                                                        double t)
+        {
+            if (t == 0.0)
+                return;
+            double c, s;
+            dev::rotation_from_tangent(t, c, s);
+            double2 yp[VR], yq[VR];
+#pragma unroll
+            for (int j = 0; j < VR; ++j)
+            {
+                yp[j] = vp[lane + j * G];
+                yq[j] = vq[lane + j * G];
+            }
+#pragma unroll
+            for (int j = 0; j < VR; ++j)
+            {
+                const int i = lane + j * G;
+                double2 np, nq;
+                np.x = c * yp[j].x - s * yq[j].x;
+                np.y = c * yp[j].y - s * yq[j].y;
+                nq.x = s * yp[j].x + c * yq[j].x;
+                nq.y = s * yp[j].y + c * yq[j].y;
+                vp[i] = np;
+                vq[i] = nq;
+            }
         }
 
         // Inlined into the kernel: as a call, the kernel-argument struct it
         // takes by reference would be copied to private (scratch) memory.
-        template <int BS, int G, int R, int VMAX2, bool VLOG, int PASSES>
+        template <int BS, int G, int R, int VR, bool VLOG, int PASSES>
         __device__ __forceinline__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
         {
             constexpr int GROUPS = BS / G;
@@ -240,7 +230,6 @@ namespace jsvd
             const int half = even >> 1;
             const int rounds = n < 2 ? 0 : even - 1;
             const int gid = tid / G, lane = tid & (G - 1);
-            const int vrows2 = ldv / (2 * G);
             double2 *A2 = reinterpret_cast<double2 *>(A);
             double2 *V2 = reinterpret_cast<double2 *>(V);
             const int lda2 = lda / 2, ldv2 = ldv / 2;
@@ -268,9 +257,9 @@ namespace jsvd
                                 double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + k : &dummy_t;
                                 // 24-bit products (full-rate multiplies): column
                                 // indices and leading dimensions are far below 2^24.
-                                any |= pair_class_f64<G, R, VMAX2, VLOG>(
+                                any |= pair_class_f64<G, R, VR, VLOG>(
                                     A2 + __umul24(p, lda2), A2 + __umul24(q, lda2), V2 + __umul24(p, ldv2),
-                                    V2 + __umul24(q, ldv2), vrows2, lane, args.eps, slot);
+                                    V2 + __umul24(q, ldv2), lane, args.eps, slot);
                             }
                             circ[i].advance(even);
                         }
@@ -365,7 +354,8 @@ namespace jsvd
                                 {
                                     const int p = circ[i].p, q = circ[i].q;
                                     if (p < n && q < n)
-                                        replay_pair_f64<G, VMAX2>(V2 + p * ldv2, V2 + q * ldv2, vrows2, lane, t);
+                                        replay_pair_f64<G, VR>(V2 + __umul24(p, ldv2), V2 + __umul24(q, ldv2),
+                                                                              lane, t);
                                     circ[i].advance(even);
                                 }
                             }
@@ -392,20 +382,36 @@ namespace jsvd
             }
         }
 
-        // Runs solve_class_f64 instantiated for the matrix's exact number of
-        // row pairs per lane (1 .. RMAX2).
+        // Runs solve_class_f64 instantiated for the matrix's exact numbers of
+        // row pairs per lane: R = 1 .. RMAX2 for A, VR = 1 .. min(R, VMAX2)
+        // for V (n <= m, so V never has more row pairs than A).
+        template <int BS, int G, int VMAX2, bool VLOG, int PASSES, int R, int VR>
+        __device__ __forceinline__ void solve_vrows(int vrows2, const WorkItem &w, double *lds, const ClassArgs &args)
+        {
+            if constexpr (VR < VMAX2 && VR < R)
+            {
+                if (vrows2 != VR)
+                {
+                    solve_vrows<BS, G, VMAX2, VLOG, PASSES, R, VR + 1>(vrows2, w, lds, args);
+                    return;
+                }
+            }
+            solve_class_f64<BS, G, R, VR, VLOG, PASSES>(w, lds, args);
+        }
+
         template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES, int R>
-        __device__ __forceinline__ void solve_rows(int rows2, const WorkItem &w, double *lds, const ClassArgs &args)
+        __device__ __forceinline__ void solve_rows(int rows2, int vrows2, const WorkItem &w, double *lds,
+                                                   const ClassArgs &args)
         {
             if constexpr (R < RMAX2)
             {
                 if (rows2 != R)
                 {
-                    solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, R + 1>(rows2, w, lds, args);
+                    solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, R + 1>(rows2, vrows2, w, lds, args);
                     return;
                 }
             }
-            solve_class_f64<BS, G, R, VMAX2, VLOG, PASSES>(w, lds, args);
+            solve_vrows<BS, G, VMAX2, VLOG, PASSES, R, 1>(vrows2, w, lds, args);
         }
 
         // counters[0]: next work-list entry; counters[1]: workgroups finished.
@@ -428,7 +434,8 @@ namespace jsvd
                 if (item >= args.count)
                     break;
                 const WorkItem w = args.items[item];
-                solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, 1>(padded_rows(w.m, G) / (2 * G), w, lds, args);
+                solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, 1>(padded_rows(w.m, G) / (2 * G),
+                                                                 padded_rows(w.n, G) / (2 * G), w, lds, args);
                 __syncthreads();
             }
             if (threadIdx.x == 0)
