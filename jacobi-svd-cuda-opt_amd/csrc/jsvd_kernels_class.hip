@@ -64,6 +64,32 @@ namespace jsvd
             return ((rows + 2 * G - 1) / (2 * G)) * (2 * G);
         }
 
+        // A lane's share of one column in LDS: blocks j = 0 .. N-1 of G row
+        // pairs (block j holds row pairs jG + lane).  With swap set, the
+        // blocks of each aligned pair (0,1), (2,3), ... trade places (the
+        // last block of an odd N stays).  A 16-byte LDS access serves 16
+        // lanes (256 bytes, all 64 banks) per cycle: two 8-lane groups.  When
+        // the column stride is a multiple of 256 bytes, the neighbouring
+        // columns of two such groups start on the same banks; swapping in
+        // every odd group puts their blocks 128 bytes apart instead.  Only the
+        // order in which a lane visits its blocks changes.
+        template <int G, int N>
+        struct ColumnShare
+        {
+            double2 *even, *odd, *last;
+            __device__ __forceinline__ ColumnShare(double2 *col, int lane, int swap)
+                : even(col + lane + swap * G), odd(col + lane + (1 - swap) * G), last(col + lane)
+            {
+            }
+            // j must be a compile-time constant after unrolling.
+            __device__ __forceinline__ double2 &operator[](int j) const
+            {
+                if ((N & 1) && j == N - 1)
+                    return last[j * G];
+                return (j & 1) ? odd[(j - 1) * G] : even[j * G];
+            }
+        };
+
         // Rotation of one pair held by G lanes; each lane owns R row pairs of
         // A and VR of V (the matrix's exact counts, template arguments, so
         // every row loop is unrolled without bounds tests and all loads are
@@ -71,16 +97,17 @@ namespace jsvd
         // (reference flag semantics).  With VLOG the group's lane 0 records t
         // (0 when nothing is applied) at *tlog and V is not touched.
         template <int G, int R, int VR, bool VLOG>
-        __device__ __forceinline__ int pair_class_f64(double2 *__restrict__ ap, double2 *__restrict__ aq,
-                                                      double2 *__restrict__ vp, double2 *__restrict__ vq, int lane,
-                                                      double eps, double *__restrict__ tlog)
+        __device__ __forceinline__ int pair_class_f64(double2 *__restrict__ acp, double2 *__restrict__ acq,
+                                                      double2 *__restrict__ vcp, double2 *__restrict__ vcq, int lane,
+                                                      int swap_a, int swap_v, double eps, double *__restrict__ tlog)
         {
+            const ColumnShare<G, R> ap(acp, lane, swap_a), aq(acq, lane, swap_a);
             double2 xp[R], xq[R];
 #pragma unroll
             for (int j = 0; j < R; ++j)
             {
-                xp[j] = ap[lane + j * G];
-                xq[j] = aq[lane + j * G];
+                xp[j] = ap[j];
+                xq[j] = aq[j];
             }
             // Two FMA chains per dot (even and odd rows), joined at the end.
             double pp0 = 0.0, pp1 = 0.0, qq0 = 0.0, qq1 = 0.0, pq0 = 0.0, pq1 = 0.0;
@@ -106,13 +133,14 @@ namespace jsvd
                 return 0;
             }
             double2 yp[VLOG ? 1 : VR], yq[VLOG ? 1 : VR];
+            const ColumnShare<G, VR> vp(vcp, lane, swap_v), vq(vcq, lane, swap_v);
             if constexpr (!VLOG)
             {
 #pragma unroll
                 for (int j = 0; j < VR; ++j)
                 {
-                    yp[j] = vp[lane + j * G];
-                    yq[j] = vq[lane + j * G];
+                    yp[j] = vp[j];
+                    yq[j] = vq[j];
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -126,28 +154,26 @@ namespace jsvd
 #pragma unroll
             for (int j = 0; j < R; ++j)
             {
-                const int i = lane + j * G;
                 double2 np, nq;
                 np.x = c * xp[j].x - s * xq[j].x;
                 np.y = c * xp[j].y - s * xq[j].y;
                 nq.x = s * xp[j].x + c * xq[j].x;
                 nq.y = s * xp[j].y + c * xq[j].y;
-                ap[i] = np;
-                aq[i] = nq;
+                ap[j] = np;
+                aq[j] = nq;
             }
             if constexpr (!VLOG)
             {
 #pragma unroll
                 for (int j = 0; j < VR; ++j)
                 {
-                    const int i = lane + j * G;
                     double2 np, nq;
                     np.x = c * yp[j].x - s * yq[j].x;
                     np.y = c * yp[j].y - s * yq[j].y;
                     nq.x = s * yp[j].x + c * yq[j].x;
                     nq.y = s * yp[j].y + c * yq[j].y;
-                    vp[i] = np;
-                    vq[i] = nq;
+                    vp[j] = np;
+                    vq[j] = nq;
                 }
             }
             return 1;
@@ -155,31 +181,31 @@ namespace jsvd
 
         // Replays one logged rotation on the V columns of a pair.
         template <int G, int VR>
-        __device__ __forceinline__ void replay_pair_f64(double2 *__restrict__ vp, double2 *__restrict__ vq, int lane,
-                                                        double t)
+        __device__ __forceinline__ void replay_pair_f64(double2 *__restrict__ vcp, double2 *__restrict__ vcq, int lane,
+                                                        int swap_v, double t)
         {
             if (t == 0.0)
                 return;
             double c, s;
             dev::rotation_from_tangent(t, c, s);
+            const ColumnShare<G, VR> vp(vcp, lane, swap_v), vq(vcq, lane, swap_v);
             double2 yp[VR], yq[VR];
 #pragma unroll
             for (int j = 0; j < VR; ++j)
             {
-                yp[j] = vp[lane + j * G];
-                yq[j] = vq[lane + j * G];
+                yp[j] = vp[j];
+                yq[j] = vq[j];
             }
 #pragma unroll
             for (int j = 0; j < VR; ++j)
             {
-                const int i = lane + j * G;
                 double2 np, nq;
                 np.x = c * yp[j].x - s * yq[j].x;
                 np.y = c * yp[j].y - s * yq[j].y;
                 nq.x = s * yp[j].x + c * yq[j].x;
                 nq.y = s * yp[j].y + c * yq[j].y;
-                vp[i] = np;
-                vq[i] = nq;
+                vp[j] = np;
+                vq[j] = nq;
             }
         }
 
@@ -233,6 +259,10 @@ namespace jsvd
             double2 *A2 = reinterpret_cast<double2 *>(A);
             double2 *V2 = reinterpret_cast<double2 *>(V);
             const int lda2 = lda / 2, ldv2 = ldv / 2;
+            // Block order per lane group (see ColumnShare): swapped in odd
+            // 8-lane groups when the column stride is a multiple of 256 bytes.
+            const int swap_a = (G == 8 && (gid & 1) && lda % 32 == 0) ? 1 : 0;
+            const int swap_v = (G == 8 && (gid & 1) && ldv % 32 == 0) ? 1 : 0;
             double *tlog = VLOG ? args.vlog + (int64_t)blockIdx.x * args.vlog_stride : nullptr;
             double dummy_t;
             dev::CirclePair circ[PASSES]; // pair positions of this group, one per pass
@@ -259,7 +289,7 @@ namespace jsvd
                                 // indices and leading dimensions are far below 2^24.
                                 any |= pair_class_f64<G, R, VR, VLOG>(
                                     A2 + __umul24(p, lda2), A2 + __umul24(q, lda2), V2 + __umul24(p, ldv2),
-                                    V2 + __umul24(q, ldv2), lane, args.eps, slot);
+                                    V2 + __umul24(q, ldv2), lane, swap_a, swap_v, args.eps, slot);
                             }
                             circ[i].advance(even);
                         }
@@ -355,7 +385,7 @@ namespace jsvd
                                     const int p = circ[i].p, q = circ[i].q;
                                     if (p < n && q < n)
                                         replay_pair_f64<G, VR>(V2 + __umul24(p, ldv2), V2 + __umul24(q, ldv2),
-                                                                              lane, t);
+                                                                              lane, swap_v, t);
                                     circ[i].advance(even);
                                 }
                             }
