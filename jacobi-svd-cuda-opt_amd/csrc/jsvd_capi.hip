@@ -233,6 +233,7 @@ struct JsvdBucket
     int64_t count = 0;
     double work = 0.0;      // sum of n^2 (m + n), for launch order
     int64_t max_n = 0;
+    int stream = 0;         // 0 = the caller's stream, k = auxiliary stream k-1
     // Rotation-log classes: one tangent log per resident workgroup.
     double *vlog = nullptr;
     int64_t vlog_stride = 0;
@@ -669,6 +670,22 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         if (any_mixed)
             JSVD_CHECK(plan->q_scratch.alloc(plan->v_elems));
         const size_t streams = std::min<size_t>(plan->buckets.size(), 4);
+        // Buckets to streams, longest first onto the least loaded stream;
+        // a bucket's length is its work per resident workgroup (a bucket of
+        // a few large matrices, one workgroup per CU, runs longest).
+        {
+            std::vector<size_t> by_len(plan->buckets.size());
+            std::iota(by_len.begin(), by_len.end(), size_t(0));
+            auto len = [&](size_t i) { return plan->buckets[i].work / std::max(1, plan->buckets[i].grid); };
+            std::stable_sort(by_len.begin(), by_len.end(), [&](size_t x, size_t y) { return len(x) > len(y); });
+            std::vector<double> load(streams, 0.0);
+            for (size_t i : by_len)
+            {
+                const size_t s = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+                plan->buckets[i].stream = static_cast<int>(s);
+                load[s] += len(i);
+            }
+        }
         if (streams > 1)
         {
             JSVD_HIP(hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming));
@@ -755,7 +772,7 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
     for (size_t b = 0; b < p->buckets.size(); ++b)
     {
         const JsvdBucket &bk = p->buckets[b];
-        const size_t lane = b % (p->aux.size() + 1);
+        const size_t lane = static_cast<size_t>(bk.stream);
         hipStream_t st = lane == 0 ? stream : p->aux[lane - 1];
         const jsvd::WorkItem *items = p->d_items.ptr + bk.first;
         unsigned int *ctr = p->d_counters.ptr + kCountersPerBucket * b;
