@@ -90,25 +90,44 @@ namespace jsvd
             }
         };
 
-        // Rotation of one pair held by G lanes; each lane owns R row pairs of
-        // A and VR of V (the matrix's exact counts, template arguments, so
-        // every row loop is unrolled without bounds tests and all loads are
-        // issued before any use).  Returns 1 when the pair counts as rotated
-        // (reference flag semantics).  With VLOG the group's lane 0 records t
-        // (0 when nothing is applied) at *tlog and V is not touched.
-        template <int G, int R, int VR, bool VLOG>
-        __device__ __forceinline__ int pair_class_f64(double2 *__restrict__ acp, double2 *__restrict__ acq,
-                                                      double2 *__restrict__ vcp, double2 *__restrict__ vcq, int lane,
-                                                      int swap_a, int swap_v, double eps, double *__restrict__ tlog)
+        template <int G, int N>
+        __device__ __forceinline__ void load_share(double2 (&x)[N], double2 *col, int lane, int swap)
         {
-            const ColumnShare<G, R> ap(acp, lane, swap_a), aq(acq, lane, swap_a);
-            double2 xp[R], xq[R];
+            const ColumnShare<G, N> c(col, lane, swap);
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                x[j] = c[j];
+        }
+
+        template <int G, int N>
+        __device__ __forceinline__ void store_share(const double2 (&x)[N], double2 *col, int lane, int swap)
+        {
+            const ColumnShare<G, N> c(col, lane, swap);
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+                c[j] = x[j];
+        }
+
+        // Rotation of one pair held by G lanes.  The top column (p) is held
+        // in registers by the lane group (xp for A, yp for V), the bottom
+        // column (q) is read from and written back to LDS.  Each lane owns R
+        // row pairs of A and VR of V (the matrix's exact counts, template
+        // arguments, so every row loop is unrolled without bounds tests and
+        // all loads are issued before any use).  Returns 1 when the pair
+        // counts as rotated (reference flag semantics).  With VLOG the
+        // group's lane 0 records t (0 when nothing is applied) at *tlog and
+        // V is not touched.
+        template <int G, int R, int VR, bool VLOG>
+        __device__ __forceinline__ int pair_class_f64(double2 (&xp)[R], double2 (&yp)[VLOG ? 1 : VR],
+                                                      double2 *__restrict__ acq, double2 *__restrict__ vcq,
+                                                      int lane, int swap_a, int swap_v, double eps,
+                                                      double *__restrict__ tlog)
+        {
+            const ColumnShare<G, R> aq(acq, lane, swap_a);
+            double2 xq[R];
 #pragma unroll
             for (int j = 0; j < R; ++j)
-            {
-                xp[j] = ap[j];
                 xq[j] = aq[j];
-            }
             // Two FMA chains per dot (even and odd rows), joined at the end.
             double pp0 = 0.0, pp1 = 0.0, qq0 = 0.0, qq1 = 0.0, pq0 = 0.0, pq1 = 0.0;
 #pragma unroll
@@ -132,16 +151,13 @@ namespace jsvd
                     *tlog = 0.0;
                 return 0;
             }
-            double2 yp[VLOG ? 1 : VR], yq[VLOG ? 1 : VR];
-            const ColumnShare<G, VR> vp(vcp, lane, swap_v), vq(vcq, lane, swap_v);
+            double2 yq[VLOG ? 1 : VR];
+            const ColumnShare<G, VR> vq(vcq, lane, swap_v);
             if constexpr (!VLOG)
             {
 #pragma unroll
                 for (int j = 0; j < VR; ++j)
-                {
-                    yp[j] = vp[j];
                     yq[j] = vq[j];
-                }
                 __builtin_amdgcn_sched_barrier(0);
             }
             const double t = dev::rotation_tangent(pp, qq, pq);
@@ -154,12 +170,12 @@ namespace jsvd
 #pragma unroll
             for (int j = 0; j < R; ++j)
             {
-                double2 np, nq;
-                np.x = c * xp[j].x - s * xq[j].x;
-                np.y = c * xp[j].y - s * xq[j].y;
-                nq.x = s * xp[j].x + c * xq[j].x;
-                nq.y = s * xp[j].y + c * xq[j].y;
-                ap[j] = np;
+                const double2 a = xp[j], b = xq[j];
+                xp[j].x = c * a.x - s * b.x;
+                xp[j].y = c * a.y - s * b.y;
+                double2 nq;
+                nq.x = s * a.x + c * b.x;
+                nq.y = s * a.y + c * b.y;
                 aq[j] = nq;
             }
             if constexpr (!VLOG)
@@ -167,12 +183,12 @@ namespace jsvd
 #pragma unroll
                 for (int j = 0; j < VR; ++j)
                 {
-                    double2 np, nq;
-                    np.x = c * yp[j].x - s * yq[j].x;
-                    np.y = c * yp[j].y - s * yq[j].y;
-                    nq.x = s * yp[j].x + c * yq[j].x;
-                    nq.y = s * yp[j].y + c * yq[j].y;
-                    vp[j] = np;
+                    const double2 a = yp[j], b = yq[j];
+                    yp[j].x = c * a.x - s * b.x;
+                    yp[j].y = c * a.y - s * b.y;
+                    double2 nq;
+                    nq.x = s * a.x + c * b.x;
+                    nq.y = s * a.y + c * b.y;
                     vq[j] = nq;
                 }
             }
@@ -265,14 +281,38 @@ namespace jsvd
             const int swap_v = (G == 8 && (gid & 1) && ldv % 32 == 0) ? 1 : 0;
             double *tlog = VLOG ? args.vlog + (int64_t)blockIdx.x * args.vlog_stride : nullptr;
             double dummy_t;
-            dev::CirclePair circ[PASSES]; // pair positions of this group, one per pass
+            // Pair positions of the circle (reference order): top[k] and
+            // bottom[k], k = 0 .. half-1; position 0's top is column 0.  Each
+            // round every other column moves one slot along the ring
+            // top[1] -> top[2] -> ... -> top[half-1] -> bottom[half-1] -> ... ->
+            // bottom[0] -> top[1].  Virtual group 0 (lane group 0, pass 0)
+            // stays at position 0; virtual group v >= 1 starts at position v
+            // and moves one position per round (half-1 -> 1), following its
+            // top column, which it holds in registers: only bottom columns
+            // travel through LDS.  A group that leaves position half-1 (its
+            // top column turns into bottom[half-1]) stores that column and,
+            // at position 1, loads the new top (the previous bottom[0]).
+            // Column indices >= n are the dummy column of an odd n.
+            const int ring = even - 1;
+            int pos[PASSES], ct[PASSES], cb[PASSES];
+            bool fresh[PASSES], dirty[PASSES];
+            double2 ta[PASSES][R];
+            double2 tv[PASSES][VLOG ? 1 : VR];
+#pragma unroll
+            for (int i = 0; i < PASSES; ++i)
+            {
+                const int k = gid + i * GROUPS;
+                pos[i] = k;
+                ct[i] = k;
+                cb[i] = even - 1 - k;
+                fresh[i] = true; // top column to be loaded
+                dirty[i] = false;
+            }
+            int z = even - 1; // bottom[0]'s column this round
             int executed = 0;
             for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
             {
                 int any = 0;
-#pragma unroll
-                for (int i = 0; i < PASSES; ++i)
-                    circ[i].start(gid + i * GROUPS, even);
                 for (int r = 0; r < rounds; ++r)
                 {
 #pragma unroll
@@ -281,25 +321,78 @@ namespace jsvd
                         const int k = gid + i * GROUPS;
                         if (k < half)
                         {
-                            const int p = circ[i].p, q = circ[i].q;
+                            const int p = ct[i], q = cb[i];
+                            if (fresh[i])
+                            {
+                                if (p < n)
+                                {
+                                    load_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
+                                    if constexpr (!VLOG)
+                                        load_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
+                                }
+                                fresh[i] = false;
+                                dirty[i] = false;
+                            }
                             if (p < n && q < n)
                             {
-                                double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + k : &dummy_t;
+                                double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + pos[i] : &dummy_t;
                                 // 24-bit products (full-rate multiplies): column
                                 // indices and leading dimensions are far below 2^24.
-                                any |= pair_class_f64<G, R, VR, VLOG>(
-                                    A2 + __umul24(p, lda2), A2 + __umul24(q, lda2), V2 + __umul24(p, ldv2),
-                                    V2 + __umul24(q, ldv2), lane, swap_a, swap_v, args.eps, slot);
+                                const int rot = pair_class_f64<G, R, VR, VLOG>(
+                                    ta[i], tv[i], A2 + __umul24(q, lda2), V2 + __umul24(q, ldv2), lane, swap_a,
+                                    swap_v, args.eps, slot);
+                                any |= rot;
+                                dirty[i] = dirty[i] || rot;
                             }
-                            circ[i].advance(even);
+                            if (k > 0)
+                            {
+                                if (pos[i] == half - 1)
+                                {
+                                    // The top column becomes bottom[half-1].
+                                    if (p < n && dirty[i])
+                                    {
+                                        store_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
+                                        if constexpr (!VLOG)
+                                            store_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
+                                    }
+                                    pos[i] = 1;
+                                    ct[i] = z;
+                                    cb[i] = z - 2 < 1 ? z - 2 + ring : z - 2;
+                                    fresh[i] = true;
+                                }
+                                else
+                                {
+                                    ++pos[i];
+                                    cb[i] = q - 2 < 1 ? q - 2 + ring : q - 2;
+                                }
+                            }
+                            else
+                            {
+                                cb[i] = q - 1 < 1 ? ring : q - 1;
+                            }
                         }
                     }
+                    z = z - 1 < 1 ? ring : z - 1;
                     __syncthreads();
                 }
                 executed = sweep + 1;
                 if (!__syncthreads_or(any))
                     break;
             }
+            // Held top columns back to LDS (those loaded and changed since).
+#pragma unroll
+            for (int i = 0; i < PASSES; ++i)
+            {
+                const int k = gid + i * GROUPS;
+                if (k < half && !fresh[i] && dirty[i] && ct[i] < n)
+                {
+                    store_share<G, R>(ta[i], A2 + __umul24(ct[i], lda2), lane, swap_a);
+                    if constexpr (!VLOG)
+                        store_share<G, VR>(tv[i], V2 + __umul24(ct[i], ldv2), lane, swap_v);
+                }
+            }
+            __syncthreads();
+            dev::CirclePair circ[PASSES]; // pair positions for the rotation-log replay
 
             // U / Sigma epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
             // sigma_j = ||a_j||; U_ij = a_ij / sigma_j if sigma_j > eps else 0.
