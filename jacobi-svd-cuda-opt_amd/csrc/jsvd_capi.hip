@@ -234,6 +234,7 @@ struct JsvdBucket
     double work = 0.0;      // sum of n^2 (m + n), for launch order
     int64_t max_n = 0;
     int stream = 0;         // 0 = the caller's stream, k = auxiliary stream k-1
+    int per_cu = 0;         // resident workgroups per CU
     // Rotation-log classes: one tangent log per resident workgroup.
     double *vlog = nullptr;
     int64_t vlog_stride = 0;
@@ -626,6 +627,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         if (occupancy <= 0)
             return fail(JSVD_EDEVICE, "work-list kernel cannot be resident with the requested LDS size.");
         const int64_t resident = int64_t(occupancy) * prop.multiProcessorCount;
+        b.per_cu = occupancy;
         b.grid = static_cast<int>(std::min<int64_t>(resident, b.count));
         if (b.f32cls >= 0)
         {
@@ -810,6 +812,9 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
             args.vlog = bk.vlog;
             args.vlog_stride = bk.vlog_stride;
             args.counters = ctr + 6;
+            // One workgroup per CU: each such matrix is a long single solve
+            // that other buckets' workgroups would otherwise slow down.
+            args.priority = bk.per_cu == 1 ? 1 : 0;
             if (mixed)
             {
                 args.b_cm = d_u;

@@ -58,6 +58,7 @@ namespace jsvd
         double *vlog;         // rotation-log classes: per-workgroup tangent log
         int64_t vlog_stride;  // doubles per workgroup log
         unsigned int *counters;
+        int priority;         // 1: the workgroup's waves issue at raised priority
     };
 
     // Shape classes of the LDS-resident kernel with compile-time lane groups.

@@ -548,6 +548,11 @@ namespace jsvd
             extern __shared__ double2 lds_pairs[];
             double *lds = reinterpret_cast<double *>(lds_pairs);
             __shared__ int s_item;
+            // Raised issue priority over co-resident workgroups of other
+            // buckets: set for the buckets whose matrices need a whole CU's
+            // LDS, the longest single solves of a plan.
+            if (args.priority)
+                __builtin_amdgcn_s_setprio(3);
             for (;;)
             {
                 if (threadIdx.x == 0)
