@@ -167,16 +167,18 @@ namespace jsvd
                 *tlog = t;
             // t = +-0 (a tangent below the double range) gives c = 1, s = 0:
             // the updates below leave both columns unchanged.
+            // The new bottom row pair first, into the registers of the old
+            // one (stored at once), then the new top in place: no register
+            // copies to carry the held top column into the next round.
 #pragma unroll
             for (int j = 0; j < R; ++j)
             {
                 const double2 a = xp[j], b = xq[j];
+                xq[j].x = s * a.x + c * b.x;
+                xq[j].y = s * a.y + c * b.y;
+                aq[j] = xq[j];
                 xp[j].x = c * a.x - s * b.x;
                 xp[j].y = c * a.y - s * b.y;
-                double2 nq;
-                nq.x = s * a.x + c * b.x;
-                nq.y = s * a.y + c * b.y;
-                aq[j] = nq;
             }
             if constexpr (!VLOG)
             {
@@ -184,12 +186,11 @@ namespace jsvd
                 for (int j = 0; j < VR; ++j)
                 {
                     const double2 a = yp[j], b = yq[j];
+                    yq[j].x = s * a.x + c * b.x;
+                    yq[j].y = s * a.y + c * b.y;
+                    vq[j] = yq[j];
                     yp[j].x = c * a.x - s * b.x;
                     yp[j].y = c * a.y - s * b.y;
-                    double2 nq;
-                    nq.x = s * a.x + c * b.x;
-                    nq.y = s * a.y + c * b.y;
-                    vq[j] = nq;
                 }
             }
             return 1;
@@ -294,8 +295,9 @@ namespace jsvd
             // at position 1, loads the new top (the previous bottom[0]).
             // Column indices >= n are the dummy column of an odd n.
             const int ring = even - 1;
-            int pos[PASSES], ct[PASSES], cb[PASSES];
-            bool fresh[PASSES], dirty[PASSES];
+            // Per pass: position, top and bottom column, and two flags kept
+            // as integers (0/1) rather than lane masks.
+            int pos[PASSES], ct[PASSES], cb[PASSES], fresh[PASSES], dirty[PASSES];
             double2 ta[PASSES][R];
             double2 tv[PASSES][VLOG ? 1 : VR];
 #pragma unroll
@@ -305,8 +307,8 @@ namespace jsvd
                 pos[i] = k;
                 ct[i] = k;
                 cb[i] = even - 1 - k;
-                fresh[i] = true; // top column to be loaded
-                dirty[i] = false;
+                fresh[i] = 1; // top column to be loaded
+                dirty[i] = 0;
             }
             int z = even - 1; // bottom[0]'s column this round
             int executed = 0;
@@ -315,6 +317,8 @@ namespace jsvd
                 int any = 0;
                 for (int r = 0; r < rounds; ++r)
                 {
+                    const int z1 = z - 1 < 1 ? ring : z - 1;
+                    const int z2 = z - 2 < 1 ? z - 2 + ring : z - 2;
 #pragma unroll
                     for (int i = 0; i < PASSES; ++i)
                     {
@@ -322,57 +326,43 @@ namespace jsvd
                         if (k < half)
                         {
                             const int p = ct[i], q = cb[i];
-                            if (fresh[i])
+                            if (fresh[i] & (p < n))
                             {
-                                if (p < n)
-                                {
-                                    load_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
-                                    if constexpr (!VLOG)
-                                        load_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
-                                }
-                                fresh[i] = false;
-                                dirty[i] = false;
+                                load_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
+                                if constexpr (!VLOG)
+                                    load_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
                             }
-                            if (p < n && q < n)
+                            int rot = 0;
+                            if ((p < n) & (q < n))
                             {
                                 double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + pos[i] : &dummy_t;
                                 // 24-bit products (full-rate multiplies): column
                                 // indices and leading dimensions are far below 2^24.
-                                const int rot = pair_class_f64<G, R, VR, VLOG>(
-                                    ta[i], tv[i], A2 + __umul24(q, lda2), V2 + __umul24(q, ldv2), lane, swap_a,
-                                    swap_v, args.eps, slot);
-                                any |= rot;
-                                dirty[i] = dirty[i] || rot;
+                                rot = pair_class_f64<G, R, VR, VLOG>(ta[i], tv[i], A2 + __umul24(q, lda2),
+                                                                     V2 + __umul24(q, ldv2), lane, swap_a, swap_v,
+                                                                     args.eps, slot);
                             }
-                            if (k > 0)
+                            any |= rot;
+                            dirty[i] = (dirty[i] & (fresh[i] ^ 1)) | rot;
+                            // Leaving position half-1: the top column becomes
+                            // bottom[half-1] (stored if changed); the group
+                            // restarts at position 1 with the previous bottom[0].
+                            const int wrap = (k > 0) & (pos[i] == half - 1);
+                            if (wrap & (p < n) & dirty[i])
                             {
-                                if (pos[i] == half - 1)
-                                {
-                                    // The top column becomes bottom[half-1].
-                                    if (p < n && dirty[i])
-                                    {
-                                        store_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
-                                        if constexpr (!VLOG)
-                                            store_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
-                                    }
-                                    pos[i] = 1;
-                                    ct[i] = z;
-                                    cb[i] = z - 2 < 1 ? z - 2 + ring : z - 2;
-                                    fresh[i] = true;
-                                }
-                                else
-                                {
-                                    ++pos[i];
-                                    cb[i] = q - 2 < 1 ? q - 2 + ring : q - 2;
-                                }
+                                store_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
+                                if constexpr (!VLOG)
+                                    store_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
                             }
-                            else
-                            {
-                                cb[i] = q - 1 < 1 ? ring : q - 1;
-                            }
+                            const int q1 = q - 1 < 1 ? ring : q - 1;
+                            const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
+                            cb[i] = k == 0 ? q1 : (wrap ? z2 : q2);
+                            ct[i] = wrap ? z : p;
+                            pos[i] = wrap ? 1 : (k > 0 ? pos[i] + 1 : 0);
+                            fresh[i] = wrap;
                         }
                     }
-                    z = z - 1 < 1 ? ring : z - 1;
+                    z = z1;
                     __syncthreads();
                 }
                 executed = sweep + 1;
