@@ -41,6 +41,7 @@ namespace jsvd
             int block, group, rmax2, vmax2;
             bool vlog;
             int passes; // pair positions per lane group and round
+            bool split; // half the waves rotate A, the other half V one round later
         };
 
         // Shape classes.  The default for a shape is the first fitting class
@@ -56,6 +57,7 @@ namespace jsvd
             {1024, 16, 4, 4, true, 1}, // 6: n <= 128, m <= 128, V by rotation log
             {512, 16, 8, 8, true, 4},  // 7: n <= 256, m <= 256, V by rotation log
             {512, 16, 4, 2, true, 1},  // 8: n <= 64, m <= 128, V by rotation log (two workgroups per CU)
+            {512, 8, 8, 4, false, 1, true}, // 9: n <= 64, m <= 128: four waves on A, four on V
         };
         constexpr int kClassCount = sizeof(kClasses) / sizeof(kClasses[0]);
 
@@ -226,9 +228,155 @@ namespace jsvd
             }
         }
 
+        // Sweeps with the waves split in two roles.  The first BS/2 threads
+        // (A waves) run the circle on A exactly as the one-role loop does,
+        // with V left out: each pair's tangent t (0 when not rotated) goes to
+        // a two-round ring T in LDS.  The other BS/2 threads (V waves) follow
+        // one round behind with the same positions and held top columns, and
+        // apply c, s recomputed from t to V.  V's loads, stores and
+        // arithmetic thereby leave the A waves' critical path (they overlap
+        // the next round's dots and rotation parameters), and each role
+        // holds only its own columns in registers.  c and s come from t by
+        // the same expressions in both roles, so V receives the rotations
+        // applied to A bit for bit.  Returns the number of sweeps.
+        template <int BS, int G, int R, int VR>
+        __device__ __forceinline__ int sweeps_split_f64(int n, int lda, int ldv, double *A, double *V, double *T,
+                                                        const ClassArgs &args)
+        {
+            constexpr int HALF_BS = BS / 2;
+            const int tid = threadIdx.x;
+            const int role = tid >= HALF_BS ? 1 : 0; // 0: A waves, 1: V waves
+            const int t2 = tid - role * HALF_BS;
+            const int k = t2 / G, lane = t2 & (G - 1);
+            const int even = n + (n & 1), half = even >> 1, ring = even - 1;
+            const int rounds = n < 2 ? 0 : even - 1;
+            double2 *A2 = reinterpret_cast<double2 *>(A);
+            double2 *V2 = reinterpret_cast<double2 *>(V);
+            const int lda2 = lda / 2, ldv2 = ldv / 2;
+            const int swap_a = (G == 8 && (k & 1) && lda % 32 == 0) ? 1 : 0;
+            const int swap_v = (G == 8 && (k & 1) && ldv % 32 == 0) ? 1 : 0;
+            // Circle positions and held top columns as in the one-role loop;
+            // each role advances its own copy after handling a round.
+            int pos = k, ct = k, cb = even - 1 - k, fresh = 1, dirty = 0, z = even - 1;
+            // The held top column: R row pairs of A in the A waves, the first
+            // VR (VR <= R) of V in the V waves (one array, so the two roles
+            // share registers).
+            double2 held[R];
+            double2 unused[1];
+            auto load_v = [&](double2 *col) {
+                const ColumnShare<G, VR> c(col, lane, swap_v);
+#pragma unroll
+                for (int j = 0; j < VR; ++j)
+                    held[j] = c[j];
+            };
+            auto store_v = [&](double2 *col) {
+                const ColumnShare<G, VR> c(col, lane, swap_v);
+#pragma unroll
+                for (int j = 0; j < VR; ++j)
+                    c[j] = held[j];
+            };
+            auto advance = [&](int p, int q) {
+                const int z1 = z - 1 < 1 ? ring : z - 1;
+                const int z2 = z - 2 < 1 ? z - 2 + ring : z - 2;
+                const int wrap = (k > 0) & (pos == half - 1);
+                if (wrap & (p < n) & dirty)
+                {
+                    if (role == 0)
+                        store_share<G, R>(held, A2 + __umul24(p, lda2), lane, swap_a);
+                    else
+                        store_v(V2 + __umul24(p, ldv2));
+                }
+                const int q1 = q - 1 < 1 ? ring : q - 1;
+                const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
+                cb = k == 0 ? q1 : (wrap ? z2 : q2);
+                ct = wrap ? z : p;
+                pos = wrap ? 1 : (k > 0 ? pos + 1 : 0);
+                fresh = wrap;
+                z = z1;
+            };
+            // V waves: apply the rotations logged for round `gr` at slot gr & 1.
+            auto v_round = [&](int gr) {
+                const int p = ct, q = cb;
+                if (fresh & (p < n))
+                    load_v(V2 + __umul24(p, ldv2));
+                int rot = 0;
+                if ((p < n) & (q < n))
+                {
+                    const double t = T[(gr & 1) * half + pos];
+                    if (t != 0.0)
+                    {
+                        const ColumnShare<G, VR> vq(V2 + __umul24(q, ldv2), lane, swap_v);
+                        double2 yq[VR];
+#pragma unroll
+                        for (int j = 0; j < VR; ++j)
+                            yq[j] = vq[j];
+                        double c, s;
+                        dev::rotation_from_tangent(t, c, s);
+#pragma unroll
+                        for (int j = 0; j < VR; ++j)
+                        {
+                            const double2 a = held[j], b = yq[j];
+                            yq[j].x = s * a.x + c * b.x;
+                            yq[j].y = s * a.y + c * b.y;
+                            vq[j] = yq[j];
+                            held[j].x = c * a.x - s * b.x;
+                            held[j].y = c * a.y - s * b.y;
+                        }
+                        rot = 1;
+                    }
+                }
+                dirty = (dirty & (fresh ^ 1)) | rot;
+                advance(p, q);
+            };
+            int executed = 0, gr = 0;
+            for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
+            {
+                int any = 0;
+                for (int r = 0; r < rounds; ++r, ++gr)
+                {
+                    if (k < half)
+                    {
+                        if (role == 0)
+                        {
+                            const int p = ct, q = cb;
+                            if (fresh & (p < n))
+                                load_share<G, R>(held, A2 + __umul24(p, lda2), lane, swap_a);
+                            int rot = 0;
+                            if ((p < n) & (q < n))
+                                rot = pair_class_f64<G, R, VR, true>(held, unused, A2 + __umul24(q, lda2), nullptr,
+                                                                     lane, swap_a, 0, args.eps,
+                                                                     T + (gr & 1) * half + pos);
+                            any |= rot;
+                            dirty = (dirty & (fresh ^ 1)) | rot;
+                            advance(p, q);
+                        }
+                        else if (gr > 0)
+                            v_round(gr - 1);
+                    }
+                    __syncthreads();
+                }
+                executed = sweep + 1;
+                if (!__syncthreads_or(any))
+                    break;
+            }
+            // The V waves' last round, then every held top column to LDS.
+            if (role == 1 && k < half && gr > 0)
+                v_round(gr - 1);
+            __syncthreads();
+            if (k < half && !fresh && dirty && ct < n)
+            {
+                if (role == 0)
+                    store_share<G, R>(held, A2 + __umul24(ct, lda2), lane, swap_a);
+                else
+                    store_v(V2 + __umul24(ct, ldv2));
+            }
+            __syncthreads();
+            return executed;
+        }
+
         // Inlined into the kernel: as a call, the kernel-argument struct it
         // takes by reference would be copied to private (scratch) memory.
-        template <int BS, int G, int R, int VR, bool VLOG, int PASSES>
+        template <int BS, int G, int R, int VR, bool VLOG, int PASSES, bool SPLIT>
         __device__ __forceinline__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
         {
             constexpr int GROUPS = BS / G;
@@ -294,94 +442,102 @@ namespace jsvd
             // top column turns into bottom[half-1]) stores that column and,
             // at position 1, loads the new top (the previous bottom[0]).
             // Column indices >= n are the dummy column of an odd n.
-            const int ring = even - 1;
-            // Per pass: position, top and bottom column, and two flags kept
-            // as integers (0/1) rather than lane masks.
-            int pos[PASSES], ct[PASSES], cb[PASSES], fresh[PASSES], dirty[PASSES];
-            double2 ta[PASSES][R];
-            double2 tv[PASSES][VLOG ? 1 : VR];
-#pragma unroll
-            for (int i = 0; i < PASSES; ++i)
-            {
-                const int k = gid + i * GROUPS;
-                pos[i] = k;
-                ct[i] = k;
-                cb[i] = even - 1 - k;
-                fresh[i] = 1; // top column to be loaded
-                dirty[i] = 0;
-            }
-            int z = even - 1; // bottom[0]'s column this round
             int executed = 0;
-            for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
+            if constexpr (SPLIT)
             {
-                int any = 0;
-                for (int r = 0; r < rounds; ++r)
-                {
-                    const int z1 = z - 1 < 1 ? ring : z - 1;
-                    const int z2 = z - 2 < 1 ? z - 2 + ring : z - 2;
+                static_assert(!VLOG && PASSES == 1, "split roles carry V in LDS, one pair per group");
+                executed = sweeps_split_f64<BS, G, R, VR>(n, lda, ldv, A, V, S + n, args);
+            }
+            else
+            {
+                const int ring = even - 1;
+                // Per pass: position, top and bottom column, and two flags kept
+                // as integers (0/1) rather than lane masks.
+                int pos[PASSES], ct[PASSES], cb[PASSES], fresh[PASSES], dirty[PASSES];
+                double2 ta[PASSES][R];
+                double2 tv[PASSES][VLOG ? 1 : VR];
 #pragma unroll
-                    for (int i = 0; i < PASSES; ++i)
+                for (int i = 0; i < PASSES; ++i)
+                {
+                    const int k = gid + i * GROUPS;
+                    pos[i] = k;
+                    ct[i] = k;
+                    cb[i] = even - 1 - k;
+                    fresh[i] = 1; // top column to be loaded
+                    dirty[i] = 0;
+                }
+                int z = even - 1; // bottom[0]'s column this round
+                for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
+                {
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
                     {
-                        const int k = gid + i * GROUPS;
-                        if (k < half)
-                        {
-                            const int p = ct[i], q = cb[i];
-                            if (fresh[i] & (p < n))
-                            {
-                                load_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
-                                if constexpr (!VLOG)
-                                    load_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
-                            }
-                            int rot = 0;
-                            if ((p < n) & (q < n))
-                            {
-                                double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + pos[i] : &dummy_t;
-                                // 24-bit products (full-rate multiplies): column
-                                // indices and leading dimensions are far below 2^24.
-                                rot = pair_class_f64<G, R, VR, VLOG>(ta[i], tv[i], A2 + __umul24(q, lda2),
-                                                                     V2 + __umul24(q, ldv2), lane, swap_a, swap_v,
-                                                                     args.eps, slot);
-                            }
-                            any |= rot;
-                            dirty[i] = (dirty[i] & (fresh[i] ^ 1)) | rot;
-                            // Leaving position half-1: the top column becomes
-                            // bottom[half-1] (stored if changed); the group
-                            // restarts at position 1 with the previous bottom[0].
-                            const int wrap = (k > 0) & (pos[i] == half - 1);
-                            if (wrap & (p < n) & dirty[i])
-                            {
-                                store_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
-                                if constexpr (!VLOG)
-                                    store_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
-                            }
-                            const int q1 = q - 1 < 1 ? ring : q - 1;
-                            const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
-                            cb[i] = k == 0 ? q1 : (wrap ? z2 : q2);
-                            ct[i] = wrap ? z : p;
-                            pos[i] = wrap ? 1 : (k > 0 ? pos[i] + 1 : 0);
-                            fresh[i] = wrap;
-                        }
-                    }
-                    z = z1;
-                    __syncthreads();
-                }
-                executed = sweep + 1;
-                if (!__syncthreads_or(any))
-                    break;
-            }
-            // Held top columns back to LDS (those loaded and changed since).
+                        const int z1 = z - 1 < 1 ? ring : z - 1;
+                        const int z2 = z - 2 < 1 ? z - 2 + ring : z - 2;
 #pragma unroll
-            for (int i = 0; i < PASSES; ++i)
-            {
-                const int k = gid + i * GROUPS;
-                if (k < half && !fresh[i] && dirty[i] && ct[i] < n)
-                {
-                    store_share<G, R>(ta[i], A2 + __umul24(ct[i], lda2), lane, swap_a);
-                    if constexpr (!VLOG)
-                        store_share<G, VR>(tv[i], V2 + __umul24(ct[i], ldv2), lane, swap_v);
+                        for (int i = 0; i < PASSES; ++i)
+                        {
+                            const int k = gid + i * GROUPS;
+                            if (k < half)
+                            {
+                                const int p = ct[i], q = cb[i];
+                                if (fresh[i] & (p < n))
+                                {
+                                    load_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
+                                    if constexpr (!VLOG)
+                                        load_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
+                                }
+                                int rot = 0;
+                                if ((p < n) & (q < n))
+                                {
+                                    double *slot = VLOG ? tlog + ((int64_t)sweep * rounds + r) * half + pos[i] : &dummy_t;
+                                    // 24-bit products (full-rate multiplies): column
+                                    // indices and leading dimensions are far below 2^24.
+                                    rot = pair_class_f64<G, R, VR, VLOG>(ta[i], tv[i], A2 + __umul24(q, lda2),
+                                                                         V2 + __umul24(q, ldv2), lane, swap_a, swap_v,
+                                                                         args.eps, slot);
+                                }
+                                any |= rot;
+                                dirty[i] = (dirty[i] & (fresh[i] ^ 1)) | rot;
+                                // Leaving position half-1: the top column becomes
+                                // bottom[half-1] (stored if changed); the group
+                                // restarts at position 1 with the previous bottom[0].
+                                const int wrap = (k > 0) & (pos[i] == half - 1);
+                                if (wrap & (p < n) & dirty[i])
+                                {
+                                    store_share<G, R>(ta[i], A2 + __umul24(p, lda2), lane, swap_a);
+                                    if constexpr (!VLOG)
+                                        store_share<G, VR>(tv[i], V2 + __umul24(p, ldv2), lane, swap_v);
+                                }
+                                const int q1 = q - 1 < 1 ? ring : q - 1;
+                                const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
+                                cb[i] = k == 0 ? q1 : (wrap ? z2 : q2);
+                                ct[i] = wrap ? z : p;
+                                pos[i] = wrap ? 1 : (k > 0 ? pos[i] + 1 : 0);
+                                fresh[i] = wrap;
+                            }
+                        }
+                        z = z1;
+                        __syncthreads();
+                    }
+                    executed = sweep + 1;
+                    if (!__syncthreads_or(any))
+                        break;
                 }
+                // Held top columns back to LDS (those loaded and changed since).
+#pragma unroll
+                for (int i = 0; i < PASSES; ++i)
+                {
+                    const int k = gid + i * GROUPS;
+                    if (k < half && !fresh[i] && dirty[i] && ct[i] < n)
+                    {
+                        store_share<G, R>(ta[i], A2 + __umul24(ct[i], lda2), lane, swap_a);
+                        if constexpr (!VLOG)
+                            store_share<G, VR>(tv[i], V2 + __umul24(ct[i], ldv2), lane, swap_v);
+                    }
+                }
+                __syncthreads();
             }
-            __syncthreads();
             dev::CirclePair circ[PASSES]; // pair positions for the rotation-log replay
 
             // U / Sigma epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
@@ -498,21 +654,21 @@ namespace jsvd
         // Runs solve_class_f64 instantiated for the matrix's exact numbers of
         // row pairs per lane: R = 1 .. RMAX2 for A, VR = 1 .. min(R, VMAX2)
         // for V (n <= m, so V never has more row pairs than A).
-        template <int BS, int G, int VMAX2, bool VLOG, int PASSES, int R, int VR>
+        template <int BS, int G, int VMAX2, bool VLOG, int PASSES, bool SPLIT, int R, int VR>
         __device__ __forceinline__ void solve_vrows(int vrows2, const WorkItem &w, double *lds, const ClassArgs &args)
         {
             if constexpr (VR < VMAX2 && VR < R)
             {
                 if (vrows2 != VR)
                 {
-                    solve_vrows<BS, G, VMAX2, VLOG, PASSES, R, VR + 1>(vrows2, w, lds, args);
+                    solve_vrows<BS, G, VMAX2, VLOG, PASSES, SPLIT, R, VR + 1>(vrows2, w, lds, args);
                     return;
                 }
             }
-            solve_class_f64<BS, G, R, VR, VLOG, PASSES>(w, lds, args);
+            solve_class_f64<BS, G, R, VR, VLOG, PASSES, SPLIT>(w, lds, args);
         }
 
-        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES, int R>
+        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES, bool SPLIT, int R>
         __device__ __forceinline__ void solve_rows(int rows2, int vrows2, const WorkItem &w, double *lds,
                                                    const ClassArgs &args)
         {
@@ -520,17 +676,17 @@ namespace jsvd
             {
                 if (rows2 != R)
                 {
-                    solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, R + 1>(rows2, vrows2, w, lds, args);
+                    solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, SPLIT, R + 1>(rows2, vrows2, w, lds, args);
                     return;
                 }
             }
-            solve_vrows<BS, G, VMAX2, VLOG, PASSES, R, 1>(vrows2, w, lds, args);
+            solve_vrows<BS, G, VMAX2, VLOG, PASSES, SPLIT, R, 1>(vrows2, w, lds, args);
         }
 
         // counters[0]: next work-list entry; counters[1]: workgroups finished.
         // The last workgroup to finish resets both, so the next launch on the
         // stream starts from a clean work-list without a memset.
-        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES>
+        template <int BS, int G, int RMAX2, int VMAX2, bool VLOG, int PASSES, bool SPLIT>
         __global__ __launch_bounds__(BS) void lds_class_f64(ClassArgs args)
         {
             // Declared as double2 so the dynamic LDS block is 16-byte aligned
@@ -552,7 +708,7 @@ namespace jsvd
                 if (item >= args.count)
                     break;
                 const WorkItem w = args.items[item];
-                solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, 1>(padded_rows(w.m, G) / (2 * G),
+                solve_rows<BS, G, RMAX2, VMAX2, VLOG, PASSES, SPLIT, 1>(padded_rows(w.m, G) / (2 * G),
                                                                  padded_rows(w.n, G) / (2 * G), w, lds, args);
                 __syncthreads();
             }
@@ -572,24 +728,26 @@ namespace jsvd
         {
             constexpr ClassSpec c = kClasses[I];
             return reinterpret_cast<const void *>(
-                &lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog, c.passes>);
+                &lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog, c.passes, c.split>);
         }
 
         template <int I>
         void class_launch_t(int grid, int lds_bytes, const ClassArgs &args, hipStream_t stream)
         {
             constexpr ClassSpec c = kClasses[I];
-            lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog, c.passes>
+            lds_class_f64<c.block, c.group, c.rmax2, c.vmax2, c.vlog, c.passes, c.split>
                 <<<grid, c.block, lds_bytes, stream>>>(args);
         }
 
         using LaunchFn = void (*)(int, int, const ClassArgs &, hipStream_t);
         const LaunchFn kClassLaunch[] = {class_launch_t<0>, class_launch_t<1>, class_launch_t<2>,
                                          class_launch_t<3>, class_launch_t<4>, class_launch_t<5>,
-                                         class_launch_t<6>, class_launch_t<7>, class_launch_t<8>};
+                                         class_launch_t<6>, class_launch_t<7>, class_launch_t<8>,
+                                         class_launch_t<9>};
         const void *(*const kClassSymbol[])() = {class_symbol_t<0>, class_symbol_t<1>, class_symbol_t<2>,
                                                  class_symbol_t<3>, class_symbol_t<4>, class_symbol_t<5>,
-                                                 class_symbol_t<6>, class_symbol_t<7>, class_symbol_t<8>};
+                                                 class_symbol_t<6>, class_symbol_t<7>, class_symbol_t<8>,
+                                                 class_symbol_t<9>};
         static_assert(sizeof(kClassLaunch) / sizeof(kClassLaunch[0]) == kClassCount);
         static_assert(sizeof(kClassSymbol) / sizeof(kClassSymbol[0]) == kClassCount);
     } // namespace
@@ -611,7 +769,9 @@ namespace jsvd
         const int64_t ldv = ((n + 2 * g - 1) / (2 * g)) * (2 * g);
         if (kClasses[cls].vlog)
             return (std::max(lda, ldv) * n + n) * 8;
-        return (lda * n + ldv * n + n) * 8;
+        // Split roles: a two-round tangent ring after sigma's scratch.
+        const int64_t ring = kClasses[cls].split ? 2 * ((n + 1) / 2) : 0;
+        return (lda * n + ldv * n + n + ring) * 8;
     }
 
     bool class_fits(int cls, int64_t m, int64_t n)
@@ -620,7 +780,7 @@ namespace jsvd
             return false;
         const ClassSpec &c = kClasses[cls];
         const int64_t half = (n + 1) / 2;
-        const int64_t groups = c.block / c.group;
+        const int64_t groups = (c.split ? c.block / 2 : c.block) / c.group;
         // One wave: one pair per group per round.  Multi-wave classes may
         // loop over pairs (up to two passes, four with the rotation log).
         const int64_t passes = c.passes;
@@ -641,8 +801,12 @@ namespace jsvd
         // one DPP level shorter; against class 2 (n <= 32), the 8-lane
         // groups keep the LDS accesses free of bank conflicts and four waves
         // share one workgroup's barriers (7% faster on the uneven
-        // 4096-matrix batch than class 2 first).
-        static constexpr int kOrder[] = {0, 1, 4, 2, 3, 5, 6, 7, 8};
+        // 4096-matrix batch than class 2 first).  The split-role class 9 is
+        // 9% faster per round than class 4 for a 128 x 64 matrix alone on its
+        // CU, but its eight waves displace other buckets' workgroups: on the
+        // uneven batch it loses 2% as the default for whole-CU matrices, so
+        // it comes last (the autotuner may still choose it per bucket).
+        static constexpr int kOrder[] = {0, 1, 4, 2, 3, 5, 6, 7, 8, 9};
         static_assert(sizeof(kOrder) / sizeof(kOrder[0]) == kClassCount, "every class once");
         for (int i : kOrder)
             if (class_fits(i, m, n))
