@@ -374,11 +374,272 @@ namespace jsvd
             return executed;
         }
 
+        // Compact layout for the one-role classes that keep V in LDS.  Each
+        // pair group holds its top column (A and V) in registers, so LDS only
+        // stores the bottom row of the circle: half + 2 column slots of A
+        // and of V instead of n.  The bottom column at position P in global
+        // round r lives in slot (r + P) mod (half + 2) (it stays in its slot
+        // while it moves down the bottom row); the group that leaves
+        // position half-1 stores its top column in slot (r - 2) mod
+        // (half + 2), freed a round earlier, and at position 1 loads the new
+        // top from slot (r - 1) mod (half + 2), bottom[0]'s slot of the
+        // previous round.  With one pair (n <= 2) the bottom stays in slot 0.
+        // Same pairs in the same order as the reference circle; about half
+        // the LDS of the full layout, so more workgroups share a CU.
+        template <int BS, int G, int R, int VR, int PASSES>
+        __device__ __forceinline__ void solve_compact_f64(const WorkItem &w, double *lds, const ClassArgs &args)
+        {
+            constexpr int GROUPS = BS / G;
+            const int m = w.m, n = w.n, tid = threadIdx.x;
+            const int lda = padded_rows(m, G), ldv = padded_rows(n, G);
+            const int even = n + (n & 1), half = even >> 1, ring = even - 1;
+            const int rounds = n < 2 ? 0 : even - 1;
+            const int ns = half + 2;
+            double *As = lds;                                     // ns slots of lda rows
+            double *Vs = As + ns * lda;                           // ns slots of ldv rows
+            double *S = Vs + ns * ldv;                            // sigma scratch (n)
+            int *colmap = reinterpret_cast<int *>(S + n);         // column held by each slot
+            const int gid = tid / G, lane = tid & (G - 1);
+            double2 *A2 = reinterpret_cast<double2 *>(As);
+            double2 *V2 = reinterpret_cast<double2 *>(Vs);
+            const int lda2 = lda / 2, ldv2 = ldv / 2;
+            const int swap_a = (G == 8 && (gid & 1) && lda % 32 == 0) ? 1 : 0;
+            const int swap_v = (G == 8 && (gid & 1) && ldv % 32 == 0) ? 1 : 0;
+
+            // Slots 0 .. cnt-1 <- columns col(0 .. cnt-1) of the start matrix
+            // and of the initial V (zeros for the dummy column and padding).
+            auto fill = [&](int cnt, auto col) {
+                for (int idx = tid; idx < cnt * lda; idx += BS)
+                {
+                    const int sl = idx / lda, i = idx - sl * lda, j = col(sl);
+                    double x = 0.0;
+                    if (i < m && j < n)
+                        x = args.b_cm ? args.b_cm[w.a_off + (int64_t)j * m + i] : args.a[w.a_off + (int64_t)i * n + j];
+                    As[idx] = x;
+                }
+                for (int idx = tid; idx < cnt * ldv; idx += BS)
+                {
+                    const int sl = idx / ldv, i = idx - sl * ldv, j = col(sl);
+                    double x = 0.0;
+                    if (i < n && j < n)
+                        x = args.q_init ? args.q_init[w.v_off + (int64_t)j * n + i] : (i == j ? 1.0 : 0.0);
+                    Vs[idx] = x;
+                }
+            };
+
+            int pos[PASSES], ct[PASSES], cb[PASSES], fresh[PASSES];
+            double2 ta[PASSES][R];
+            double2 tv[PASSES][VR];
+            // The top row (position k holds column k) is staged through slots
+            // 0 .. half-1 into registers, then the bottom row (position P
+            // holds column even-1-P) fills slots 0 .. half-1.
+            fill(half, [](int sl) { return sl; });
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < PASSES; ++i)
+            {
+                const int k = gid + i * GROUPS;
+                pos[i] = k;
+                ct[i] = k;
+                cb[i] = even - 1 - k;
+                fresh[i] = 0;
+                if (k < half && k < n)
+                {
+                    load_share<G, R>(ta[i], A2 + __umul24(k, lda2), lane, swap_a);
+                    load_share<G, VR>(tv[i], V2 + __umul24(k, ldv2), lane, swap_v);
+                }
+            }
+            __syncthreads();
+            fill(half, [even](int sl) { return even - 1 - sl; });
+            __syncthreads();
+
+            int z = even - 1; // bottom[0]'s column this round
+            int gs = 0;       // global round mod ns
+            double dummy_t;
+            int executed = 0;
+            for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
+            {
+                int any = 0;
+                for (int r = 0; r < rounds; ++r)
+                {
+                    const int z1 = z - 1 < 1 ? ring : z - 1;
+                    const int z2 = z - 2 < 1 ? z - 2 + ring : z - 2;
+                    const int ls = gs == 0 ? ns - 1 : gs - 1;       // fresh top loads
+                    const int ws = gs < 2 ? gs + ns - 2 : gs - 2;   // wrap stores
+#pragma unroll
+                    for (int i = 0; i < PASSES; ++i)
+                    {
+                        const int k = gid + i * GROUPS;
+                        if (k < half)
+                        {
+                            const int p = ct[i], q = cb[i];
+                            if (fresh[i] & (p < n))
+                            {
+                                load_share<G, R>(ta[i], A2 + __umul24(ls, lda2), lane, swap_a);
+                                load_share<G, VR>(tv[i], V2 + __umul24(ls, ldv2), lane, swap_v);
+                            }
+                            int bsl = gs + pos[i];
+                            bsl = half == 1 ? 0 : (bsl >= ns ? bsl - ns : bsl);
+                            int rot = 0;
+                            if ((p < n) & (q < n))
+                                rot = pair_class_f64<G, R, VR, false>(ta[i], tv[i], A2 + __umul24(bsl, lda2),
+                                                                      V2 + __umul24(bsl, ldv2), lane, swap_a, swap_v,
+                                                                      args.eps, &dummy_t);
+                            any |= rot;
+                            // Leaving position half-1: the top column becomes
+                            // bottom[half-1] in slot ws; the group restarts at
+                            // position 1 with the previous bottom[0].
+                            const int wrap = (k > 0) & (pos[i] == half - 1);
+                            if (wrap & (p < n))
+                            {
+                                store_share<G, R>(ta[i], A2 + __umul24(ws, lda2), lane, swap_a);
+                                store_share<G, VR>(tv[i], V2 + __umul24(ws, ldv2), lane, swap_v);
+                            }
+                            const int q1 = q - 1 < 1 ? ring : q - 1;
+                            const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
+                            cb[i] = k == 0 ? q1 : (wrap ? z2 : q2);
+                            ct[i] = wrap ? z : p;
+                            pos[i] = wrap ? 1 : (k > 0 ? pos[i] + 1 : 0);
+                            fresh[i] = wrap;
+                        }
+                    }
+                    z = z1;
+                    gs = gs + 1 == ns ? 0 : gs + 1;
+                    __syncthreads();
+                }
+                executed = sweep + 1;
+                if (!__syncthreads_or(any))
+                    break;
+            }
+
+            // Epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
+            // sigma_j = ||a_j||; U_ij = a_ij / sigma_j if sigma_j > eps else 0.
+            // Pending top loads first, so that every top is in registers.
+            const int ls = gs == 0 ? ns - 1 : gs - 1;
+#pragma unroll
+            for (int i = 0; i < PASSES; ++i)
+            {
+                const int k = gid + i * GROUPS;
+                if (k < half && fresh[i] && ct[i] < n)
+                {
+                    load_share<G, R>(ta[i], A2 + __umul24(ls, lda2), lane, swap_a);
+                    load_share<G, VR>(tv[i], V2 + __umul24(ls, ldv2), lane, swap_v);
+                }
+            }
+            // Column norms: the held top from registers, the bottom from its slot.
+#pragma unroll
+            for (int i = 0; i < PASSES; ++i)
+            {
+                const int k = gid + i * GROUPS;
+                if (k < half)
+                {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                        acc = fma(ta[i][j].y, ta[i][j].y, fma(ta[i][j].x, ta[i][j].x, acc));
+                    acc = dev::group_sum_dpp<G>(acc);
+                    int bsl = gs + pos[i];
+                    bsl = half == 1 ? 0 : (bsl >= ns ? bsl - ns : bsl);
+                    double accb = 0.0;
+                    for (int row = lane; row < m; row += G)
+                    {
+                        const double x = As[bsl * lda + row];
+                        accb = fma(x, x, accb);
+                    }
+                    accb = dev::group_sum_dpp<G>(accb);
+                    if (lane == 0)
+                    {
+                        if (ct[i] < n)
+                        {
+                            const double sj = sqrt(fmax(acc, 0.0));
+                            S[ct[i]] = sj;
+                            args.sigma[w.s_off + ct[i]] = sj;
+                        }
+                        if (cb[i] < n)
+                        {
+                            const double sj = sqrt(fmax(accb, 0.0));
+                            S[cb[i]] = sj;
+                            args.sigma[w.s_off + cb[i]] = sj;
+                        }
+                        colmap[bsl] = cb[i];
+                    }
+                }
+            }
+            __syncthreads();
+            double *ug = args.u + w.a_off;
+            double *vg = args.v + w.v_off;
+            // U and V columns of the slots listed in colmap[0 .. cnt-1]
+            // (other slots: colmap >= n), row by row.
+            auto write_out = [&](int cnt, const int *cols) {
+                for (int idx = tid; idx < cnt * m; idx += BS)
+                {
+                    const int row = idx / cnt, sl = idx - row * cnt, j = cols[sl];
+                    if (j < n)
+                    {
+                        const double sj = S[j];
+                        ug[row * n + j] = (sj > args.eps) ? (As[sl * lda + row] / sj) : 0.0;
+                    }
+                }
+                for (int idx = tid; idx < cnt * n; idx += BS)
+                {
+                    const int row = idx / cnt, sl = idx - row * cnt, j = cols[sl];
+                    if (j < n)
+                        vg[row * n + j] = Vs[sl * ldv + row];
+                }
+            };
+            // Slots not holding a bottom column this round hold none.
+            if (tid < ns)
+            {
+                bool bottom = false;
+                for (int P = 0; P < half; ++P)
+                {
+                    int bsl = gs + P;
+                    bsl = half == 1 ? 0 : (bsl >= ns ? bsl - ns : bsl);
+                    bottom = bottom || bsl == tid;
+                }
+                if (!bottom)
+                    colmap[tid] = n;
+            }
+            __syncthreads();
+            write_out(ns, colmap);
+            __syncthreads();
+            // Then the held tops: position k's column to slot k.
+#pragma unroll
+            for (int i = 0; i < PASSES; ++i)
+            {
+                const int k = gid + i * GROUPS;
+                if (k < half)
+                {
+                    if (ct[i] < n)
+                    {
+                        store_share<G, R>(ta[i], A2 + __umul24(k, lda2), lane, swap_a);
+                        store_share<G, VR>(tv[i], V2 + __umul24(k, ldv2), lane, swap_v);
+                    }
+                    if (lane == 0)
+                        colmap[k] = ct[i];
+                }
+            }
+            __syncthreads();
+            write_out(half, colmap);
+            if (tid == 0 && args.sweeps)
+            {
+                if (args.add_sweeps)
+                    args.sweeps[w.index] += executed;
+                else
+                    args.sweeps[w.index] = executed;
+            }
+        }
+
         // Inlined into the kernel: as a call, the kernel-argument struct it
         // takes by reference would be copied to private (scratch) memory.
         template <int BS, int G, int R, int VR, bool VLOG, int PASSES, bool SPLIT>
         __device__ __forceinline__ void solve_class_f64(const WorkItem &w, double *lds, const ClassArgs &args)
         {
+            if constexpr (!VLOG && !SPLIT)
+            {
+                solve_compact_f64<BS, G, R, VR, PASSES>(w, lds, args);
+                return;
+            }
             constexpr int GROUPS = BS / G;
             const int m = w.m, n = w.n, tid = threadIdx.x;
             const int lda = padded_rows(m, G), ldv = padded_rows(n, G);
@@ -769,9 +1030,15 @@ namespace jsvd
         const int64_t ldv = ((n + 2 * g - 1) / (2 * g)) * (2 * g);
         if (kClasses[cls].vlog)
             return (std::max(lda, ldv) * n + n) * 8;
-        // Split roles: a two-round tangent ring after sigma's scratch.
-        const int64_t ring = kClasses[cls].split ? 2 * ((n + 1) / 2) : 0;
-        return (lda * n + ldv * n + n + ring) * 8;
+        if (kClasses[cls].split)
+        {
+            // A and V in full, sigma, a two-round tangent ring.
+            return (lda * n + ldv * n + n + 2 * ((n + 1) / 2)) * 8;
+        }
+        // Compact layout: (n+1)/2 + 2 slots of A and of V, sigma, and one
+        // int per slot.
+        const int64_t ns = (n + 1) / 2 + 2;
+        return (ns * (lda + ldv) + n + (ns + 1) / 2) * 8;
     }
 
     bool class_fits(int cls, int64_t m, int64_t n)
