@@ -508,6 +508,26 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     std::vector<std::pair<GroupKey, std::vector<jsvd::WorkItem>>> groups;
     bool any_mixed = false;
     static const int kSlots[] = {32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
+    // Workgroups per CU a kernel's registers allow (LDS aside), per class and
+    // for the generic kernel: buckets are keyed by the resident count a matrix
+    // actually gets, so matrices that differ only in an LDS-allowed count
+    // above the register limit share one bucket (and its work-list).
+    std::vector<int> reg_slots(jsvd::class_count() + 1, 0);
+    auto register_slots = [&](int cls, int block) -> int {
+        int &slot = reg_slots[cls >= 0 ? cls : jsvd::class_count()];
+        if (slot == 0)
+        {
+            const void *fn = cls >= 0 ? jsvd::lds_class_f64_symbol(cls) : jsvd::lds_batch_f64_symbol(block);
+            int occ = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, 0) != hipSuccess)
+            {
+                (void)hipGetLastError();
+                occ = 32;
+            }
+            slot = occ > 0 ? occ : 32;
+        }
+        return slot;
+    };
     for (int64_t k = 0; k < count; ++k)
     {
         plan->a_off[k] = ao;
@@ -529,8 +549,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             w.s_off = plan->s_off[k];
             const int block = cls >= 0 ? jsvd::class_block(cls) : generic_block;
             int slots = 1;
+            const int reg = register_slots(cls, block);
             for (int sl : kSlots)
-                if (need * sl <= jsvd::kMaxLdsBytes && (block / 64) * sl <= 32)
+                if (need * sl <= jsvd::kMaxLdsBytes && (block / 64) * sl <= 32 && sl <= reg)
                 {
                     slots = sl;
                     break;
