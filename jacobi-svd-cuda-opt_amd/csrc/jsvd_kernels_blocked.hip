@@ -225,10 +225,9 @@ namespace jsvd
             __syncthreads();
             const bool full = round == 0;
             const int inner = full ? W - 1 : B;
-            int rotated = 0;
+            int rotated = 0; // this thread saw a rotation (threads < B)
             for (int rr = 0; rr < inner; ++rr)
             {
-                int rot = 0;
                 if (tid < B)
                 {
                     int p, q;
@@ -242,17 +241,24 @@ namespace jsvd
                         p = tid;
                         q = B + (tid + rr) % B;
                     }
-                    double c, s;
-                    rot = dev::rotation(G[p * LDG + p], G[q * LDG + q], G[p * LDG + q], eps, c, s) ? 1 : 0;
+                    // Rotation parameters from the refined hardware
+                    // estimates (one dependent chain of about half the FP64
+                    // operations of IEEE division and square root).
+                    const double gpp = G[p * LDG + p], gqq = G[q * LDG + q], gpq = G[p * LDG + q];
+                    double c = 1.0, s = 0.0;
+                    if (dev::rotation_needed(gpp, gqq, gpq, eps))
+                    {
+                        dev::rotation_from_tangent(dev::rotation_tangent(gpp, gqq, gpq), c, s);
+                        rotated = 1;
+                    }
                     rc[tid] = c;
                     rs[tid] = s;
                     rp[tid] = p;
                     rq[tid] = q;
                 }
-                rot = __syncthreads_or(rot);
-                if (!rot)
-                    continue;
-                rotated = 1;
+                // One barrier per inner round: rounds without a rotation
+                // still run the (skipping) update loops.
+                __syncthreads();
                 // G' = J^T G J; J restricted to pair k is [[c, s], [-s, c]] on
                 // columns (p, q).
 #pragma unroll
@@ -308,6 +314,7 @@ namespace jsvd
                 }
                 __syncthreads();
             }
+            rotated = __syncthreads_or(rotated);
             if (rotated)
             {
                 double *q = qout + int64_t(pair) * (W * W);
