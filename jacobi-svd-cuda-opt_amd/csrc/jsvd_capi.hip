@@ -394,18 +394,32 @@ namespace
         JSVD_HIP(jsvd::launch_transpose_f64(a, p.ws_a.ptr, m, n, nullptr, 0.0, stream));
         JSVD_HIP(jsvd::launch_identity_f64(p.ws_v.ptr, n, stream));
         JSVD_HIP(jsvd::launch_fill_i32(p.ws_flags.ptr, 0, p.cfg.max_sweeps, stream));
-        int occupancy = 0;
-        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occupancy, jsvd::persistent_sweeps_f64_symbol(p.big_block), p.big_block, 0));
         hipDeviceProp_t prop;
         JSVD_HIP(hipGetDeviceProperties(&prop, p.device));
-        const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
         const int half = (n + (n & 1)) / 2;
-        const int grid = std::max(1, std::min(resident, half));
         int32_t *sweeps_slot = d_sweeps ? d_sweeps + k : nullptr;
-        JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n,
-                                                    p.cfg.epsilon, p.cfg.max_sweeps, p.ws_flags.ptr,
-                                                    p.ws_bar.ptr, sweeps_slot, stream));
+        int occupancy = 0;
+        if (m <= jsvd::wave_sweeps_max_rows())
+        {
+            // One wave per pair, four per workgroup: a quarter of the grid
+            // barrier's arrivals and no workgroup barrier inside a round.
+            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, jsvd::wave_sweeps_f64_symbol(m), 256, 0));
+            const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
+            const int grid = std::max(1, std::min(resident, (half + 3) / 4));
+            JSVD_HIP(jsvd::launch_wave_sweeps_f64(grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
+                                                  p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
+                                                  stream));
+        }
+        else
+        {
+            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occupancy, jsvd::persistent_sweeps_f64_symbol(p.big_block), p.big_block, 0));
+            const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
+            const int grid = std::max(1, std::min(resident, half));
+            JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n,
+                                                        p.cfg.epsilon, p.cfg.max_sweeps, p.ws_flags.ptr,
+                                                        p.ws_bar.ptr, sweeps_slot, stream));
+        }
         return finish_large(p, k, m, n, m, n, d_u, d_sigma, d_v, stream);
     }
 

@@ -117,6 +117,14 @@ namespace jsvd
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
                                             int32_t *sweeps_out, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
+    // One wave per column pair (workgroups of four waves), for m up to
+    // wave_sweeps_max_rows(); same arguments as the one-workgroup-per-pair
+    // kernel above.
+    int wave_sweeps_max_rows();
+    const void *wave_sweeps_f64_symbol(int64_t m);
+    hipError_t launch_wave_sweeps_f64(int grid, double *a_cm, double *v_cm, int64_t m, int n, double eps,
+                                      int max_sweeps, int32_t *flags, unsigned int *barrier, int32_t *sweeps_out,
+                                      hipStream_t stream);
 
     // Blocked Jacobi over one column-major matrix whose n columns are padded
     // with zero columns to nv = nb * block and whose m rows are padded with

@@ -369,6 +369,92 @@ namespace jsvd
                 *sweeps_out = executed;
         }
 
+        // One wave per column pair for a matrix of up to 64 * RW rows: lane l
+        // holds rows l + 64 j (j < RW) of both columns in registers, the three
+        // dots are reduced across the wave (DPP and lane shuffles), so a
+        // round needs no workgroup barrier; four waves per workgroup, a grid
+        // barrier between rounds (fewer arrivals than one workgroup per
+        // pair).  Rotation parameters from the refined hardware estimates.
+        // Same pairs and order as the reference circle.
+        template <int RW>
+        __global__ __launch_bounds__(256) void wave_sweeps_f64(double *__restrict__ A, double *__restrict__ V,
+                                                               int64_t m, int n, double eps, int max_sweeps,
+                                                               int32_t *__restrict__ flags,
+                                                               unsigned int *__restrict__ bar,
+                                                               int32_t *__restrict__ sweeps_out)
+        {
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            const int even = n + (n & 1), half = even >> 1;
+            const int rounds = n < 2 ? 0 : even - 1;
+            const int stride = gridDim.x * 4;
+            int executed = 0;
+            for (int sweep = 0; sweep < max_sweeps; ++sweep)
+            {
+                int any = 0;
+                for (int r = 0; r < rounds; ++r)
+                {
+                    for (int k = blockIdx.x * 4 + wave; k < half; k += stride)
+                    {
+                        const int p = dev::column_at(k, r, n, even);
+                        const int q = dev::column_at(even - 1 - k, r, n, even);
+                        if (p < 0 || q < 0)
+                            continue;
+                        double *ap = A + (int64_t)p * m;
+                        double *aq = A + (int64_t)q * m;
+                        double xp[RW], xq[RW];
+#pragma unroll
+                        for (int j = 0; j < RW; ++j)
+                        {
+                            const int64_t i = lane + 64 * j;
+                            xp[j] = i < m ? ap[i] : 0.0;
+                            xq[j] = i < m ? aq[i] : 0.0;
+                        }
+                        double pp = 0.0, qq = 0.0, pq = 0.0;
+#pragma unroll
+                        for (int j = 0; j < RW; ++j)
+                        {
+                            pp = fma(xp[j], xp[j], pp);
+                            qq = fma(xq[j], xq[j], qq);
+                            pq = fma(xp[j], xq[j], pq);
+                        }
+                        dev::group_sum3_dpp<64>(pp, qq, pq);
+                        if (!dev::rotation_needed(pp, qq, pq, eps))
+                            continue;
+                        any = 1;
+                        double c, s;
+                        dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+#pragma unroll
+                        for (int j = 0; j < RW; ++j)
+                        {
+                            const int64_t i = lane + 64 * j;
+                            if (i < m)
+                            {
+                                ap[i] = c * xp[j] - s * xq[j];
+                                aq[i] = s * xp[j] + c * xq[j];
+                            }
+                        }
+                        double *vp = V + (int64_t)p * n;
+                        double *vq = V + (int64_t)q * n;
+                        for (int i = lane; i < n; i += 64)
+                        {
+                            const double x = vp[i], y = vq[i];
+                            vp[i] = c * x - s * y;
+                            vq[i] = s * x + c * y;
+                        }
+                    }
+                    if (r == rounds - 1 && any && lane == 0)
+                        __hip_atomic_fetch_or(&flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    dev::grid_barrier(bar, gridDim.x);
+                }
+                executed = sweep + 1;
+                const int f = __hip_atomic_load(&flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (f == 0)
+                    break;
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0 && sweeps_out)
+                *sweeps_out = executed;
+        }
+
         // Tiled transpose through a padded LDS tile (32 x 33 doubles: the
         // extra column puts the 32 elements of a tile column in distinct bank
         // pairs).  Block 32 x 8.
@@ -449,6 +535,7 @@ namespace jsvd
 
     constexpr int kLdsRmax = 16;
     constexpr int kPersistentRmax = 16;
+    constexpr int kWaveRowsMax = 8; // wave-per-pair kernel: up to 512 rows
 
     const void *lds_batch_f64_symbol(int block)
     {
@@ -504,6 +591,54 @@ namespace jsvd
     // sizes `grid` to the occupancy-guaranteed resident count and issues all
     // barrier kernels of a device on one stream, so two of them never share
     // the GPU; any other kernel that delays residency finishes on its own.
+    int wave_sweeps_max_rows()
+    {
+        return 64 * kWaveRowsMax;
+    }
+
+    const void *wave_sweeps_f64_symbol(int64_t m)
+    {
+        const int rw = static_cast<int>((m + 63) / 64);
+        switch (rw)
+        {
+        case 1: return reinterpret_cast<const void *>(&wave_sweeps_f64<1>);
+        case 2: return reinterpret_cast<const void *>(&wave_sweeps_f64<2>);
+        case 3: return reinterpret_cast<const void *>(&wave_sweeps_f64<3>);
+        case 4: return reinterpret_cast<const void *>(&wave_sweeps_f64<4>);
+        case 5: return reinterpret_cast<const void *>(&wave_sweeps_f64<5>);
+        case 6: return reinterpret_cast<const void *>(&wave_sweeps_f64<6>);
+        case 7: return reinterpret_cast<const void *>(&wave_sweeps_f64<7>);
+        case 8: return reinterpret_cast<const void *>(&wave_sweeps_f64<8>);
+        default: return nullptr;
+        }
+    }
+
+    hipError_t launch_wave_sweeps_f64(int grid, double *a_cm, double *v_cm, int64_t m, int n, double eps,
+                                      int max_sweeps, int32_t *flags, unsigned int *barrier, int32_t *sweeps_out,
+                                      hipStream_t stream)
+    {
+        const int rw = static_cast<int>((m + 63) / 64);
+#define JSVD_WAVE_CASE(RW)                                                                                         \
+    case RW:                                                                                                       \
+        wave_sweeps_f64<RW><<<grid, 256, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier,          \
+                                                      sweeps_out);                                                \
+        break;
+        switch (rw)
+        {
+            JSVD_WAVE_CASE(1)
+            JSVD_WAVE_CASE(2)
+            JSVD_WAVE_CASE(3)
+            JSVD_WAVE_CASE(4)
+            JSVD_WAVE_CASE(5)
+            JSVD_WAVE_CASE(6)
+            JSVD_WAVE_CASE(7)
+            JSVD_WAVE_CASE(8)
+        default: return hipErrorInvalidValue;
+        }
+#undef JSVD_WAVE_CASE
+        return hipGetLastError();
+    }
+
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
                                             int32_t *sweeps_out, hipStream_t stream)
