@@ -401,11 +401,11 @@ namespace
         int occupancy = 0;
         if (m <= jsvd::wave_sweeps_max_rows())
         {
-            // One wave per pair, four per workgroup: a quarter of the grid
+            // One wave per pair, eight per workgroup: an eighth of the grid
             // barrier's arrivals and no workgroup barrier inside a round.
-            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, jsvd::wave_sweeps_f64_symbol(m), 256, 0));
+            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, jsvd::wave_sweeps_f64_symbol(m), 512, 0));
             const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
-            const int grid = std::max(1, std::min(resident, (half + 3) / 4));
+            const int grid = std::max(1, std::min(resident, (half + 7) / 8));
             JSVD_HIP(jsvd::launch_wave_sweeps_f64(grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
                                                   p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
                                                   stream));

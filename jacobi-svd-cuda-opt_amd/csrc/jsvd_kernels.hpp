@@ -117,7 +117,7 @@ namespace jsvd
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
                                             int32_t *sweeps_out, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
-    // One wave per column pair (workgroups of four waves), for m up to
+    // One wave per column pair (workgroups of eight waves), for m up to
     // wave_sweeps_max_rows(); same arguments as the one-workgroup-per-pair
     // kernel above.
     int wave_sweeps_max_rows();

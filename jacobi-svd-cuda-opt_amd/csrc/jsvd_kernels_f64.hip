@@ -372,12 +372,12 @@ namespace jsvd
         // One wave per column pair for a matrix of up to 64 * RW rows: lane l
         // holds rows l + 64 j (j < RW) of both columns in registers, the three
         // dots are reduced across the wave (DPP and lane shuffles), so a
-        // round needs no workgroup barrier; four waves per workgroup, a grid
-        // barrier between rounds (fewer arrivals than one workgroup per
-        // pair).  Rotation parameters from the refined hardware estimates.
+        // round needs no workgroup barrier; eight waves per workgroup, a grid
+        // barrier between rounds (an eighth of the arrivals of one workgroup
+        // per pair; four or sixteen waves per workgroup measured slower).  Rotation parameters from the refined hardware estimates.
         // Same pairs and order as the reference circle.
         template <int RW>
-        __global__ __launch_bounds__(256) void wave_sweeps_f64(double *__restrict__ A, double *__restrict__ V,
+        __global__ __launch_bounds__(512) void wave_sweeps_f64(double *__restrict__ A, double *__restrict__ V,
                                                                int64_t m, int n, double eps, int max_sweeps,
                                                                int32_t *__restrict__ flags,
                                                                unsigned int *__restrict__ bar,
@@ -386,14 +386,14 @@ namespace jsvd
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
             const int even = n + (n & 1), half = even >> 1;
             const int rounds = n < 2 ? 0 : even - 1;
-            const int stride = gridDim.x * 4;
+            const int stride = gridDim.x * 8;
             int executed = 0;
             for (int sweep = 0; sweep < max_sweeps; ++sweep)
             {
                 int any = 0;
                 for (int r = 0; r < rounds; ++r)
                 {
-                    for (int k = blockIdx.x * 4 + wave; k < half; k += stride)
+                    for (int k = blockIdx.x * 8 + wave; k < half; k += stride)
                     {
                         const int p = dev::column_at(k, r, n, even);
                         const int q = dev::column_at(even - 1 - k, r, n, even);
@@ -620,7 +620,7 @@ namespace jsvd
         const int rw = static_cast<int>((m + 63) / 64);
 #define JSVD_WAVE_CASE(RW)                                                                                         \
     case RW:                                                                                                       \
-        wave_sweeps_f64<RW><<<grid, 256, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier,          \
+        wave_sweeps_f64<RW><<<grid, 512, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier,          \
                                                       sweeps_out);                                                \
         break;
         switch (rw)
