@@ -1,6 +1,6 @@
 // jsvd_device.hpp -- device helpers shared by the gfx950 Jacobi kernels:
-// circle-method schedule, Rutishauser rotation, wave64 reductions and the
-// grid barrier of the persistent sweep kernel.
+// circle-method schedule, Rutishauser rotation parameters, wave64
+// reductions and the grid barrier of the persistent sweep kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,31 +43,6 @@ namespace jsvd::dev
             q = (q == 1) ? even - 1 : q - 1;
         }
     };
-
-    // Rutishauser rotation for the 2x2 Gram [[pp, pq], [pq, qq]] with the
-    // reference's threshold |pq| > eps*sqrt(pp*qq) and |pq| > 1e-300
-    // (src/domain/jacobi_rotation_kernels.cuh:305-326).  Returns true when the
-    // pair counts as rotated.  Always evaluated in FP64.
-    __device__ __forceinline__ bool rotation(double pp, double qq, double pq, double eps, double &c, double &s)
-    {
-        const double rhs = eps * sqrt(fmax(pp, 0.0) * fmax(qq, 0.0));
-        const double apq = fabs(pq);
-        if (apq > rhs && apq > 1.0e-300)
-        {
-            const double zeta = (qq - pp) / (2.0 * pq);
-            const double root = sqrt(1.0 + zeta * zeta);
-            // One division: 1 / (|zeta| + root) equals both branches of the
-            // sign-split form bit for bit (also for zeta = -0, which takes +).
-            const double tmag = 1.0 / (fabs(zeta) + root);
-            const double t = (zeta >= 0.0) ? tmag : -tmag;
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = t * c;
-            return true;
-        }
-        c = 1.0;
-        s = 0.0;
-        return false;
-    }
 
     // 1/sqrt(x) for x in [0.25, 2]: the hardware estimate (relative error
     // about 2^-22) refined by two Newton steps y <- y + y (1 - x y^2) / 2.
@@ -137,6 +112,22 @@ namespace jsvd::dev
         const double rhs = eps * __builtin_amdgcn_sqrt(fmax(pp, 0.0) * fmax(qq, 0.0));
         const double apq = fabs(pq);
         return apq > rhs && apq > 1.0e-300;
+    }
+
+    // rotation_needed, then c and s from rotation_tangent; c = 1, s = 0
+    // when the pair does not rotate.  Returns true when the pair counts as
+    // rotated.
+    __device__ __forceinline__ bool rotation_refined(double pp, double qq, double pq, double eps, double &c,
+                                                     double &s)
+    {
+        if (!rotation_needed(pp, qq, pq, eps))
+        {
+            c = 1.0;
+            s = 0.0;
+            return false;
+        }
+        rotation_from_tangent(rotation_tangent(pp, qq, pq), c, s);
+        return true;
     }
 
     // Butterfly all-reduce over aligned groups of G lanes (G a power of two,

@@ -65,7 +65,7 @@ namespace jsvd
                 qq = dev::group_sum(qq, G);
                 pq = dev::group_sum(pq, G);
                 double c, s;
-                const bool rot = dev::rotation(pp, qq, pq, eps, c, s);
+                const bool rot = dev::rotation_refined(pp, qq, pq, eps, c, s);
                 if (s != 0.0)
                 {
 #pragma unroll
@@ -101,7 +101,7 @@ namespace jsvd
             qq = dev::group_sum(qq, G);
             pq = dev::group_sum(pq, G);
             double c, s;
-            const bool rot = dev::rotation(pp, qq, pq, eps, c, s);
+            const bool rot = dev::rotation_refined(pp, qq, pq, eps, c, s);
             if (s != 0.0)
             {
                 for (int i = lane; i < m; i += G)
@@ -319,8 +319,12 @@ namespace jsvd
                             qq += red[1][w2];
                             pq += red[2][w2];
                         }
-                        double c, s;
-                        any |= dev::rotation(pp, qq, pq, eps, c, s) ? 1 : 0;
+                        double c = 1.0, s = 0.0;
+                        if (dev::rotation_needed(pp, qq, pq, eps))
+                        {
+                            any = 1;
+                            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+                        }
                         if (s != 0.0)
                         {
                             if (in_regs)
