@@ -82,8 +82,10 @@ def test_lds_path_against_oracle(shape):
 @pytest.mark.parametrize("shape,threads", [((256, 256), 256), ((300, 150), 256), ((300, 151), 512),
                                            ((5000, 40), 256), ((1024, 129), 1024)])
 def test_persistent_path_against_oracle(shape, threads):
-    # These shapes exceed the LDS budget, so they run the cooperative
-    # persistent kernel (5000 x 40 also exercises its row-streaming branch).
+    # These shapes exceed the LDS budget, so they run the grid-barrier
+    # kernels: one wave per pair up to 512 rows (256 x 256, 300 x 15x), one
+    # workgroup per pair beyond (1024 x 129; 5000 x 40 also exercises its
+    # row-streaming branch).
     _note(f"persistent path shape {shape} threads {threads}")
     rng = np.random.default_rng(7 + shape[0] + shape[1])
     a = rng.standard_normal(shape)
