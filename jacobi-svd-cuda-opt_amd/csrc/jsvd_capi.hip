@@ -202,17 +202,30 @@ namespace
         return -1;
     }
 
-    int preferred_class(int64_t m, int64_t n)
+    // Class pinned by the autotuner for this column count, or -1.
+    int tuned_class(int64_t m, int64_t n)
     {
         const int b = tune_bucket(n);
-        if (b >= 0)
-        {
-            std::lock_guard<std::mutex> lock(g_tune_mu);
-            const int c = g_tuned[b];
-            if (c >= 0 && jsvd::class_fits(c, m, n))
-                return c;
-        }
-        return jsvd::select_class_f64(m, n);
+        if (b < 0)
+            return -1;
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        const int c = g_tuned[b];
+        return (c >= 0 && jsvd::class_fits(c, m, n)) ? c : -1;
+    }
+
+    // join_four_wave: the plan already has four-wave (class 4) matrices;
+    // the one-wave classes' matrices (n <= 16) then join that work-list,
+    // where the longest-first order runs them at its end instead of in a
+    // bucket that waits for registers behind it (1% on the uneven batch).
+    int preferred_class(int64_t m, int64_t n, bool join_four_wave)
+    {
+        const int t = tuned_class(m, n);
+        if (t >= 0)
+            return t;
+        const int c = jsvd::select_class_f64(m, n);
+        if (join_four_wave && (c == 0 || c == 1) && jsvd::class_fits(4, m, n))
+            return 4;
+        return c;
     }
 } // namespace
 
@@ -522,6 +535,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     std::vector<std::pair<GroupKey, std::vector<jsvd::WorkItem>>> groups;
     bool any_mixed = false;
     static const int kSlots[] = {32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
+    bool any_four_wave = false;
+    for (int64_t k = 0; k < count && !any_four_wave; ++k)
+        any_four_wave = tuned_class(m[k], n[k]) < 0 && jsvd::select_class_f64(m[k], n[k]) == 4;
     // Workgroups per CU a kernel's registers allow (LDS aside), per class and
     // for the generic kernel: buckets are keyed by the resident count a matrix
     // actually gets, so matrices that differ only in an LDS-allowed count
@@ -550,7 +566,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         ao += m[k] * n[k];
         vo += n[k] * n[k];
         so += n[k];
-        const int cls = preferred_class(m[k], n[k]);
+        const int cls = preferred_class(m[k], n[k], any_four_wave);
         const int64_t need = cls >= 0 ? jsvd::class_lds_bytes(cls, m[k], n[k]) : jsvd::lds_bytes_f64(m[k], n[k]);
         if (need <= jsvd::kMaxLdsBytes)
         {
