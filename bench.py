@@ -4,8 +4,10 @@
 Workload (one step = one pass of the hot path over one batch): 4096
 matrices, n ~ U{8..64}, m ~ U{n..2n}, A = Q1 diag(sigma) Q2^T with Haar
 factors and sigma log-spaced 1 .. 1e-6, eps = 1e-15, at most 30 sweeps,
-solved by one device work-list plan.  Inputs are resident in HBM before the
-timed region; outputs stay on the device.
+solved by one device work-list plan.  Before the timed region the shape
+class of each column-count bucket is picked by timing whole plans of this
+batch (jsvd_autotune_batch; --no-tune keeps the defaults).  Inputs are
+resident in HBM before the timed region; outputs stay on the device.
 
 Prints one JSON line: matrices/s, the work-list kernels' FP64 throughput
 against the MI355X peak and their LDS stores against the measured LDS store
@@ -18,7 +20,7 @@ beside it), the plain solve of one 256 x 256 matrix beside the oracle, and
 the adjoint kernel on 512 matrices of 96 x 64 (FP64 and FP32).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed] [--no-blocked] [--no-single]
-                    [--no-backward]
+                    [--no-backward] [--no-tune]
     python bench.py --traffic-probe    # one 256 MiB copy + one plan execution,
                                         # for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 """
@@ -312,6 +314,8 @@ def main():
     ap.add_argument("--no-blocked", action="store_true", help="skip the 8192 x 2048 blocked solve")
     ap.add_argument("--no-single", action="store_true", help="skip the single 256 x 256 plain solve")
     ap.add_argument("--no-backward", action="store_true", help="skip the 512 x 96x64 adjoint")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default shape classes (skip the batch autotune before the timed region)")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -332,6 +336,13 @@ def main():
     flat, ms, ns = gen.pack(mats)
     plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
     bufs = device_buffers(plan, flat, torch)
+    tune = None
+    if not args.no_tune:
+        # Shape classes per column-count bucket from whole-plan timings of this
+        # batch (outside the timed region); the timed plan is built afterwards.
+        tune = jsvd.autotune_batch(ms, ns, *bufs[:4], reps=3, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+        plan.close()
+        plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
     wall, kernel_ms = time_plan(plan, bufs, args.steps, args.warmup, torch)
 
     # Sanity of the last step's results (outside the timed region).
@@ -363,7 +374,10 @@ def main():
         "config": {"workload": "batched_uneven_fp64", "matrices": int(len(ms)), "n": "U{8..64}",
                    "m": "U{n..2n}", "sigma": "logspace(0,-6,n)", "epsilon": EPS, "max_sweeps": MAX_SWEEPS,
                    "mean_sweeps": float(sweeps.mean()), "max_sigma_err_normwise": worst,
-                   "buckets": info["buckets"]},
+                   "buckets": info["buckets"],
+                   "shape_classes": ({"per_bucket": tune["classes"], "plans_timed": tune["samples"],
+                                      "ms_default": tune["ms_initial"], "ms_tuned": tune["ms_tuned"]}
+                                     if tune else "default")},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,

@@ -173,6 +173,29 @@ jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out);
 int32_t jsvd_tune_class(int32_t bucket, int32_t cls);
 int32_t jsvd_class_count(void);
 
+/* Shape-class autotune on a caller's own batch (no reference counterpart:
+ * the reference tunes only its layout threshold).  Times whole executions of
+ * a plan for the count matrices m[k] x n[k] on device buffers of the packed
+ * layout (d_a is read, d_u / d_sigma / d_v are overwritten), one warm-up
+ * and `reps` timed runs per candidate, on `stream`.  Buckets present in the
+ * batch are visited smallest first; each keeps its current class unless the
+ * default (-1) or a class that fits all of its matrices makes the whole plan
+ * at least 1% faster.  The winners stay installed in the table above, so
+ * plans created afterwards use them; the report gives the table and the
+ * plan's time before and after.  Blocks the host until the timing is done. */
+#define JSVD_TUNE_BUCKETS 7
+typedef struct jsvd_batch_tune_report
+{
+    int32_t classes[JSVD_TUNE_BUCKETS]; /* class per bucket after tuning (-1 = default) */
+    int32_t samples;                    /* plans timed */
+    double ms_initial;                  /* mean ms per execution with the table as it was */
+    double ms_tuned;                    /* mean ms per execution with the tuned table */
+} jsvd_batch_tune_report;
+
+jsvd_status jsvd_autotune_batch(int64_t count, const int64_t *m, const int64_t *n, const jsvd_config *cfg,
+                                const double *d_a, double *d_u, double *d_sigma, double *d_v, int32_t reps,
+                                void *stream, jsvd_batch_tune_report *out);
+
 #ifdef __cplusplus
 }
 #endif

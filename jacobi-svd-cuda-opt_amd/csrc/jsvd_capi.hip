@@ -193,6 +193,7 @@ namespace
     constexpr int64_t kTuneUpper[kTuneBuckets] = {8, 16, 32, 48, 64, 96, 128};
     std::mutex g_tune_mu;
     int g_tuned[kTuneBuckets] = {-1, -1, -1, -1, -1, -1, -1};
+    static_assert(kTuneBuckets == JSVD_TUNE_BUCKETS, "bucket table size is part of the C ABI");
 
     int tune_bucket(int64_t n)
     {
@@ -1174,6 +1175,140 @@ jsvd_status jsvd_autotune(const jsvd_config *cfg, jsvd_autotune_report *out)
         }
         if (t_default > 0.0 && t_best > 0.0)
             out->estimated_best_speedup = std::max(out->estimated_best_speedup, t_default / t_best);
+    }
+    return JSVD_OK;
+}
+
+jsvd_status jsvd_autotune_batch(int64_t count, const int64_t *m, const int64_t *n, const jsvd_config *cfg,
+                                const double *d_a, double *d_u, double *d_sigma, double *d_v, int32_t reps,
+                                void *stream, jsvd_batch_tune_report *out)
+{
+    g_last_error.clear();
+    if (count <= 0 || !m || !n || !d_a || !d_u || !d_sigma || !d_v || !out)
+        return fail(JSVD_EINVAL, "Input batch description is invalid.");
+    if (reps < 1 || reps > 1000)
+        return fail(JSVD_EINVAL, "reps must be in [1, 1000].");
+    for (int64_t k = 0; k < count; ++k)
+        JSVD_CHECK(validate_shape(static_cast<uint64_t>(m[k]), static_cast<uint64_t>(n[k])));
+    jsvd_config c;
+    jsvd_config_default(&c);
+    if (cfg)
+        c = *cfg;
+    JSVD_CHECK(validate_config(c));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+
+    // Matrices per bucket and the classes that fit every one of them.
+    std::vector<std::vector<int>> cands(kTuneBuckets);
+    std::vector<char> present(kTuneBuckets, 0);
+    for (int64_t k = 0; k < count; ++k)
+    {
+        const int b = tune_bucket(n[k]);
+        if (b >= 0)
+            present[b] = 1;
+    }
+    for (int b = 0; b < kTuneBuckets; ++b)
+    {
+        if (!present[b])
+            continue;
+        for (int cls = 0; cls < jsvd::class_count(); ++cls)
+        {
+            bool fits = true;
+            for (int64_t k = 0; k < count && fits; ++k)
+                if (tune_bucket(n[k]) == b)
+                    fits = jsvd::class_fits(cls, m[k], n[k]);
+            if (fits)
+                cands[b].push_back(cls);
+        }
+    }
+
+    struct Scope
+    {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        int bucket = -1, saved = -1;
+        ~Scope()
+        {
+            if (e0)
+                (void)hipEventDestroy(e0);
+            if (e1)
+                (void)hipEventDestroy(e1);
+            if (bucket >= 0)
+            {
+                std::lock_guard<std::mutex> tl(g_tune_mu);
+                g_tuned[bucket] = saved;
+            }
+        }
+    } scope;
+    JSVD_HIP(hipEventCreate(&scope.e0));
+    JSVD_HIP(hipEventCreate(&scope.e1));
+    out->samples = 0;
+    // One warm-up, then the mean of `reps` executions of a plan built from
+    // the current table.
+    auto time_plan = [&](double &ms) -> jsvd_status {
+        jsvd_plan *plan = nullptr;
+        JSVD_CHECK(jsvd_plan_create(count, m, n, &c, &plan));
+        std::unique_ptr<jsvd_plan, void (*)(jsvd_plan *)> guard(plan, jsvd_plan_destroy);
+        JSVD_CHECK(jsvd_plan_execute_f64(plan, d_a, d_u, d_sigma, d_v, nullptr, st));
+        JSVD_HIP(hipEventRecord(scope.e0, st));
+        for (int r = 0; r < reps; ++r)
+            JSVD_CHECK(jsvd_plan_execute_f64(plan, d_a, d_u, d_sigma, d_v, nullptr, st));
+        JSVD_HIP(hipEventRecord(scope.e1, st));
+        JSVD_HIP(hipEventSynchronize(scope.e1));
+        float total = 0.0f;
+        JSVD_HIP(hipEventElapsedTime(&total, scope.e0, scope.e1));
+        ms = double(total) / reps;
+        out->samples += 1;
+        return JSVD_OK;
+    };
+    // The first timing runs while the clocks ramp up and is discarded.
+    double t_cur = 0.0;
+    JSVD_CHECK(time_plan(t_cur));
+    JSVD_CHECK(time_plan(t_cur));
+    out->ms_initial = t_cur;
+    // Coordinate descent over the buckets, smallest column counts first: each
+    // bucket keeps its current choice unless a candidate (the default -1 or a
+    // class fitting all its matrices) is at least 1% faster on the whole plan,
+    // so that timing noise does not flip choices.
+    constexpr double kMargin = 0.99;
+    for (int b = 0; b < kTuneBuckets; ++b)
+    {
+        if (!present[b])
+            continue;
+        int best;
+        {
+            std::lock_guard<std::mutex> tl(g_tune_mu);
+            best = g_tuned[b];
+            scope.bucket = b;
+            scope.saved = best;
+        }
+        std::vector<int> trial{-1};
+        trial.insert(trial.end(), cands[b].begin(), cands[b].end());
+        for (int cls : trial)
+        {
+            if (cls == best)
+                continue;
+            {
+                std::lock_guard<std::mutex> tl(g_tune_mu);
+                g_tuned[b] = cls;
+            }
+            double t = 0.0;
+            JSVD_CHECK(time_plan(t));
+            if (t < kMargin * t_cur)
+            {
+                t_cur = t;
+                best = cls;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> tl(g_tune_mu);
+            g_tuned[b] = best;
+            scope.bucket = -1;
+        }
+    }
+    out->ms_tuned = t_cur;
+    {
+        std::lock_guard<std::mutex> tl(g_tune_mu);
+        for (int b = 0; b < kTuneBuckets; ++b)
+            out->classes[b] = g_tuned[b];
     }
     return JSVD_OK;
 }

@@ -19,6 +19,7 @@ from ._lib import (JsvdConfig, JsvdError, JsvdInvalidArgument, LAYOUT_AUTO, LAYO
 __all__ = [
     "JsvdConfig", "JsvdError", "JsvdInvalidArgument", "PRECISION_FP64", "PRECISION_MIXED", "LAYOUT_AUTO",
     "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets",
+    "autotune_batch", "tune_class",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -139,3 +140,37 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def tune_class(bucket: int, cls: int) -> int:
+    """Set the shape class of a column-count bucket (n <= 8, 16, 32, 48, 64,
+    96, 128: buckets 0..6; -1 = default) for plans created afterwards;
+    returns the previous setting."""
+    prev = load().jsvd_tune_class(bucket, cls)
+    if prev == -2:
+        raise JsvdInvalidArgument(_lib.JSVD_EINVAL, f"bucket {bucket} is out of range")
+    return prev
+
+
+def autotune_batch(ms, ns, a, u, s, v, reps: int = 3, cfg: JsvdConfig | None = None, stream=None, **overrides):
+    """Time whole plan executions of this batch on device tensors of the
+    packed layout (``jsvd_autotune_batch``) and keep, per column-count
+    bucket, the fastest shape class; plans created afterwards use the table.
+    Returns {"classes": [...], "samples": k, "ms_initial": t0, "ms_tuned": t1}."""
+    import torch
+
+    ms = np.asarray(ms, dtype=np.int64)
+    ns = np.asarray(ns, dtype=np.int64)
+    _, _, _, (a_elems, v_elems, s_elems) = pack_offsets(ms, ns)
+    for t, nel in ((a, a_elems), (u, a_elems), (s, s_elems), (v, v_elems)):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() >= nel):
+            raise ValueError("autotune_batch needs contiguous float64 CUDA tensors of the packed sizes")
+    c = _as_cfg(cfg, overrides)
+    st = stream if stream is not None else torch.cuda.current_stream()
+    rep = _lib.JsvdBatchTuneReport()
+    _lib.check(load().jsvd_autotune_batch(len(ms), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                          ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ctypes.byref(c),
+                                          a.data_ptr(), u.data_ptr(), s.data_ptr(), v.data_ptr(), int(reps),
+                                          st.cuda_stream, ctypes.byref(rep)))
+    return {"classes": list(rep.classes), "samples": int(rep.samples), "ms_initial": float(rep.ms_initial),
+            "ms_tuned": float(rep.ms_tuned)}

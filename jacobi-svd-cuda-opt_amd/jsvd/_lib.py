@@ -65,6 +65,18 @@ class JsvdAutotuneReport(ctypes.Structure):
     ]
 
 
+TUNE_BUCKETS = 7
+
+
+class JsvdBatchTuneReport(ctypes.Structure):
+    _fields_ = [
+        ("classes", ctypes.c_int32 * TUNE_BUCKETS),
+        ("samples", ctypes.c_int32),
+        ("ms_initial", ctypes.c_double),
+        ("ms_tuned", ctypes.c_double),
+    ]
+
+
 class JsvdError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(message)
@@ -102,6 +114,8 @@ SIGNATURES = {
     "jsvd_autotune": (ctypes.c_int, [ctypes.POINTER(JsvdConfig), ctypes.POINTER(JsvdAutotuneReport)]),
     "jsvd_tune_class": (ctypes.c_int32, [ctypes.c_int32, ctypes.c_int32]),
     "jsvd_class_count": (ctypes.c_int32, []),
+    "jsvd_autotune_batch": (ctypes.c_int, [ctypes.c_int64, _i64p, _i64p, ctypes.POINTER(JsvdConfig), _vp, _vp, _vp,
+                                           _vp, ctypes.c_int32, _vp, ctypes.POINTER(JsvdBatchTuneReport)]),
 }
 
 _lib = None

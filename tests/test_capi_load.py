@@ -64,3 +64,25 @@ def test_validation_before_device(shape, over, msg):
     with pytest.raises(jsvd.JsvdInvalidArgument) as e:
         jsvd.svd(np.zeros(shape), **over)
     assert str(e.value) == msg
+
+
+def test_autotune_batch_rejects_bad_arguments_before_device():
+    lib = jsvd.load()
+    rep = _lib.JsvdBatchTuneReport()
+    cfg = jsvd.default_config()
+    m = (ctypes.c_int64 * 1)(8)
+    n = (ctypes.c_int64 * 1)(4)
+    # count 0, null buffers and reps out of range are refused without a device call.
+    assert lib.jsvd_autotune_batch(0, m, n, ctypes.byref(cfg), 1, 1, 1, 1, 1, None, ctypes.byref(rep)) == 1
+    assert lib.jsvd_autotune_batch(1, m, n, ctypes.byref(cfg), None, 1, 1, 1, 1, None, ctypes.byref(rep)) == 1
+    assert lib.jsvd_autotune_batch(1, m, n, ctypes.byref(cfg), 1, 1, 1, 1, 0, None, ctypes.byref(rep)) == 1
+    bad = (ctypes.c_int64 * 1)(2)
+    assert lib.jsvd_autotune_batch(1, bad, n, ctypes.byref(cfg), 1, 1, 1, 1, 1, None, ctypes.byref(rep)) == 1
+    assert b"rows >= columns" in lib.jsvd_last_error()
+
+
+def test_tune_class_table_round_trip():
+    assert jsvd.tune_class(3, 4) == -1
+    assert jsvd.tune_class(3, -1) == 4
+    with pytest.raises(jsvd.JsvdInvalidArgument):
+        jsvd.tune_class(7, 0)

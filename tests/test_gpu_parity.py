@@ -228,6 +228,45 @@ def test_batch_plan_device_full_uneven_batch():
     plan.close()
 
 
+def test_autotune_batch_keeps_parity():
+    # jsvd_autotune_batch times whole plans on the caller's batch and installs
+    # the fastest class per bucket; a plan built afterwards must still match
+    # the oracle, and the reported table must hold valid choices.
+    import torch
+
+    _note("autotune_batch, 300 uneven matrices")
+    mats, _ = gen.uneven_batch(300, seed=91)
+    flat, ms, ns = gen.pack(mats)
+    _, _, _, (a_el, v_el, s_el) = jsvd.pack_offsets(ms, ns)
+    dev = torch.device("cuda:0")
+    a = torch.from_numpy(flat).to(dev)
+    u = torch.empty(a_el, dtype=torch.float64, device=dev)
+    s = torch.empty(s_el, dtype=torch.float64, device=dev)
+    v = torch.empty(v_el, dtype=torch.float64, device=dev)
+    try:
+        rep = jsvd.autotune_batch(ms, ns, a, u, s, v, reps=1, epsilon=1e-13, max_sweeps=30)
+        _note(f"tuned classes {rep['classes']} {rep['ms_initial']:.3f} -> {rep['ms_tuned']:.3f} ms")
+        ncls = jsvd.load().jsvd_class_count()
+        assert all(-1 <= c < ncls for c in rep["classes"])
+        assert rep["samples"] >= 2 and 0 < rep["ms_tuned"] <= rep["ms_initial"]
+        plan = jsvd.BatchPlan(ms, ns, epsilon=1e-13, max_sweeps=30)
+        sw = torch.zeros(len(ms), dtype=torch.int32, device=dev)
+        plan.execute(a, u, s, v, sw)
+        torch.cuda.synchronize()
+        uh, sh, vh, swh = (t.cpu().numpy() for t in (u, s, v, sw))
+        subset = list(range(0, len(mats), 5))
+        ref = oracle.solve_batch([mats[k] for k in subset], 1e-13, 30)
+        for k, r in zip(subset, ref):
+            m, n = int(ms[k]), int(ns[k])
+            g = (uh[plan.a_off[k]:plan.a_off[k] + m * n].reshape(m, n), sh[plan.s_off[k]:plan.s_off[k] + n],
+                 vh[plan.v_off[k]:plan.v_off[k] + n * n].reshape(n, n), int(swh[k]))
+            _assert_parity(mats[k], g, r, 1e-13, f"tuned{k}")
+        plan.close()
+    finally:
+        for b in range(7):
+            jsvd.tune_class(b, -1)
+
+
 def test_batch_plan_repeat_execution():
     # The work-list counters reset themselves at the end of each launch: a
     # second execution of the same plan must give identical results.
