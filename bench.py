@@ -229,6 +229,16 @@ def plain_single(torch, jsvd, gen, cpu):
     out = {"workload": "256x256 N(0,1), eps 1e-15, 30 sweeps", "gpu_ms": kernel_ms,
            "sweeps": int(bufs[4].item())}
     plan.close()
+    # 48 such matrices (seeds base + k) in one plan: grouped launches of the
+    # same kernel, several matrices side by side.
+    mats = [gen.gaussian_square(256, seed=gen.BASE_SEED + k) for k in range(48)]
+    flat, ms, ns = gen.pack(mats)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    bufs = device_buffers(plan, flat, torch)
+    wall, batch_ms = time_plan(plan, bufs, 3, 1, torch)
+    out["batch_48"] = {"gpu_ms": batch_ms, "matrices_per_s": 48 / (batch_ms * 1e-3),
+                       "mean_sweeps": float(bufs[4].double().mean().item())}
+    plan.close()
     if cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
@@ -237,6 +247,12 @@ def plain_single(torch, jsvd, gen, cpu):
         ref = oracle.solve(a, EPS, MAX_SWEEPS, threads=1)
         out["oracle_ms_1_thread"] = (time.perf_counter() - t0) * 1e3
         out["oracle_sweeps"] = int(ref[3])
+        cores = min(os.cpu_count() or 1, 64)
+        t0 = time.perf_counter()
+        oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
+        dt = time.perf_counter() - t0
+        out["batch_48"]["oracle_matrices_per_s"] = 48 / dt
+        out["batch_48"]["oracle_cores"] = cores
     return out
 
 

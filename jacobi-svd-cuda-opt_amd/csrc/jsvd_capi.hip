@@ -289,6 +289,24 @@ struct jsvd_plan
     DevBuf<unsigned int> ws_bar;
     int big_block = 256;
     hipEvent_t big_in = nullptr, big_out = nullptr;
+    // Large matrices with m <= wave_sweeps_max_rows() (and no blocked
+    // path): the grouped wave kernel, one column-major workspace per matrix.
+    // Entries are ordered by row-register count (one launch each), longest
+    // first inside a launch.
+    struct WaveLaunch
+    {
+        int64_t m_max = 0;
+        int first = 0, count = 0;
+        int members = 1; // workgroups per group; 0 = one workgroup per matrix
+        int groups = 1;
+    };
+    std::vector<int64_t> wave;                 // batch index per entry
+    std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry
+    std::vector<WaveLaunch> wave_launches;
+    DevBuf<jsvd::WaveMatrix> d_wave;
+    DevBuf<double> wv_a, wv_v;
+    DevBuf<int32_t> wv_flags;
+    DevBuf<unsigned int> wv_bars;
     // Blocked Jacobi (cfg.block_width 16 or 32): per-round partial Grams,
     // rotations Q and flags, sized for the largest blocked matrix; the sweep
     // flag is read back through pinned host memory once per sweep.
@@ -413,28 +431,159 @@ namespace
         const int half = (n + (n & 1)) / 2;
         int32_t *sweeps_slot = d_sweeps ? d_sweeps + k : nullptr;
         int occupancy = 0;
-        if (m <= jsvd::wave_sweeps_max_rows())
-        {
-            // One wave per pair, eight per workgroup: an eighth of the grid
-            // barrier's arrivals and no workgroup barrier inside a round.
-            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, jsvd::wave_sweeps_f64_symbol(m), 512, 0));
-            const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
-            const int grid = std::max(1, std::min(resident, (half + 7) / 8));
-            JSVD_HIP(jsvd::launch_wave_sweeps_f64(grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
-                                                  p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
-                                                  stream));
-        }
-        else
-        {
-            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &occupancy, jsvd::persistent_sweeps_f64_symbol(p.big_block), p.big_block, 0));
-            const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
-            const int grid = std::max(1, std::min(resident, half));
-            JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n,
-                                                        p.cfg.epsilon, p.cfg.max_sweeps, p.ws_flags.ptr,
-                                                        p.ws_bar.ptr, sweeps_slot, stream));
-        }
+        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &occupancy, jsvd::persistent_sweeps_f64_symbol(p.big_block), p.big_block, 0));
+        const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
+        const int grid = std::max(1, std::min(resident, half));
+        JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
+                                                    p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
+                                                    stream));
         return finish_large(p, k, m, n, m, n, d_u, d_sigma, d_v, stream);
+    }
+
+    // A launch of `count` matrices whose widest has `max_half` column pairs
+    // per round runs one workgroup per matrix when it has more matrices than
+    // max(4, max_half / 12): alone, a sixteen-wave workgroup takes half / 16
+    // pairs per wave and round, while a group's time grows with the number
+    // of groups that share the L2 write-backs of their barriers.  Measured on
+    // MI355X, the two forms cross at about 7 matrices of 160 x 150, 10 of
+    // 256 x 256 and 40 of 512 x 512.
+    bool wave_block_mode(int64_t count, int64_t max_half)
+    {
+        return count > std::max<int64_t>(4, max_half / 12);
+    }
+
+    // Every grouped-wave matrix of the plan: row-major A into its
+    // column-major workspace, V <- I, the grouped sweep launches, then sigma,
+    // U = A / sigma and V back to the caller's row-major layout.
+    jsvd_status plan_wave(jsvd_plan &p, const double *d_a, double *d_u, double *d_sigma, double *d_v,
+                          int32_t *d_sweeps, hipStream_t stream)
+    {
+        const int64_t entries = static_cast<int64_t>(p.wave.size());
+        JSVD_HIP(jsvd::launch_fill_i32(p.wv_flags.ptr, 0, entries * p.cfg.max_sweeps, stream));
+        for (int64_t e = 0; e < entries; ++e)
+        {
+            const int64_t k = p.wave[e];
+            const int64_t m = p.m[k];
+            const int n = static_cast<int>(p.n[k]);
+            JSVD_HIP(jsvd::launch_transpose_f64(d_a + p.a_off[k], p.wv_a.ptr + p.wave_a_off[e], m, n, nullptr, 0.0,
+                                                stream));
+            JSVD_HIP(jsvd::launch_identity_f64(p.wv_v.ptr + p.wave_v_off[e], n, stream));
+        }
+        for (const auto &l : p.wave_launches)
+        {
+            jsvd::WaveGroupArgs args{};
+            args.mats = p.d_wave.ptr + l.first;
+            args.count = l.count;
+            args.members = l.members;
+            args.eps = p.cfg.epsilon;
+            args.max_sweeps = p.cfg.max_sweeps;
+            args.bars = p.wv_bars.ptr;
+            args.sweeps = d_sweeps;
+            if (l.members == 0)
+                JSVD_HIP(jsvd::launch_wave_block_sweeps_f64(l.m_max, l.count, args, stream));
+            else
+                JSVD_HIP(jsvd::launch_wave_group_sweeps_f64(l.m_max, l.groups, args, stream));
+        }
+        for (int64_t e = 0; e < entries; ++e)
+        {
+            const int64_t k = p.wave[e];
+            const int64_t m = p.m[k];
+            const int n = static_cast<int>(p.n[k]);
+            const double *a_cm = p.wv_a.ptr + p.wave_a_off[e];
+            double *sigma = d_sigma + p.s_off[k];
+            JSVD_HIP(jsvd::launch_column_norms_f64(a_cm, m, sigma, m, n, stream));
+            JSVD_HIP(jsvd::launch_transpose_ld_f64(a_cm, m, d_u + p.a_off[k], n, n, m, sigma, p.cfg.epsilon, stream));
+            JSVD_HIP(jsvd::launch_transpose_ld_f64(p.wv_v.ptr + p.wave_v_off[e], n, d_v + p.v_off[k], n, n, n,
+                                                   nullptr, 0.0, stream));
+        }
+        return JSVD_OK;
+    }
+
+    // Orders the grouped-wave entries (row-register count, then longest
+    // first), sizes each launch and allocates the per-matrix workspaces.
+    // A launch of many matrices (wave_block_mode) runs one workgroup per
+    // matrix (members = 0): no cross-workgroup barrier, and the dispatcher
+    // hands the longest matrices out first as CUs free up.  A smaller launch
+    // runs groups of workgroups with barriers of their own: every
+    // workgroup of it must be resident at once (the group barriers wait for
+    // each other's members), so it has at most the occupancy-guaranteed
+    // resident count of workgroups; a group gets one wave per column pair of
+    // its largest matrix when the GPU has room for a group per matrix at
+    // that size, else fewer members (each wave then takes several pairs per
+    // round).
+    jsvd_status plan_wave_setup(jsvd_plan &p, int cus)
+    {
+        auto rw = [&](int64_t k) { return (p.m[k] + 63) / 64; };
+        auto cost = [&](int64_t k) { return double(p.n[k]) * p.n[k] * (p.m[k] + p.n[k]); };
+        std::stable_sort(p.wave.begin(), p.wave.end(), [&](int64_t x, int64_t y) {
+            if (rw(x) != rw(y))
+                return rw(x) > rw(y);
+            return cost(x) > cost(y);
+        });
+        const int64_t entries = static_cast<int64_t>(p.wave.size());
+        p.wave_a_off.resize(entries);
+        p.wave_v_off.resize(entries);
+        int64_t a_total = 0, v_total = 0;
+        for (int64_t e = 0; e < entries; ++e)
+        {
+            const int64_t k = p.wave[e];
+            p.wave_a_off[e] = a_total;
+            p.wave_v_off[e] = v_total;
+            a_total += p.m[k] * p.n[k];
+            v_total += p.n[k] * p.n[k];
+        }
+        int max_groups = 1;
+        for (int64_t e = 0; e < entries;)
+        {
+            jsvd_plan::WaveLaunch l;
+            l.first = static_cast<int>(e);
+            int64_t max_half = 1;
+            while (e < entries && rw(p.wave[e]) == rw(p.wave[l.first]))
+            {
+                const int64_t k = p.wave[e];
+                l.m_max = std::max(l.m_max, p.m[k]);
+                max_half = std::max<int64_t>(max_half, (p.n[k] + 1) / 2);
+                ++e;
+            }
+            l.count = static_cast<int>(e - l.first);
+            if (wave_block_mode(l.count, max_half))
+            {
+                l.members = 0;
+                l.groups = l.count;
+                p.wave_launches.push_back(l);
+                continue;
+            }
+            int occupancy = 0;
+            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max), 512, 0));
+            const int64_t resident = int64_t(std::max(1, occupancy)) * cus;
+            const int64_t need = (max_half + 7) / 8;
+            const int64_t members = std::max<int64_t>(1, std::min(need, resident / l.count));
+            l.members = static_cast<int>(std::min(members, resident));
+            l.groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(l.count, resident / l.members)));
+            max_groups = std::max(max_groups, l.groups);
+            p.wave_launches.push_back(l);
+        }
+        JSVD_CHECK(p.wv_a.alloc(a_total));
+        JSVD_CHECK(p.wv_v.alloc(v_total));
+        JSVD_CHECK(p.wv_flags.alloc(entries * p.cfg.max_sweeps));
+        JSVD_CHECK(p.wv_bars.alloc(2 * max_groups));
+        JSVD_HIP(hipMemset(p.wv_bars.ptr, 0, sizeof(unsigned int) * 2 * max_groups));
+        std::vector<jsvd::WaveMatrix> mats(entries);
+        for (int64_t e = 0; e < entries; ++e)
+        {
+            const int64_t k = p.wave[e];
+            mats[e].a = p.wv_a.ptr + p.wave_a_off[e];
+            mats[e].v = p.wv_v.ptr + p.wave_v_off[e];
+            mats[e].flags = p.wv_flags.ptr + e * p.cfg.max_sweeps;
+            mats[e].m = static_cast<int32_t>(p.m[k]);
+            mats[e].n = static_cast<int32_t>(p.n[k]);
+            mats[e].index = static_cast<int32_t>(k);
+        }
+        JSVD_CHECK(p.d_wave.alloc(entries));
+        JSVD_HIP(hipMemcpy(p.d_wave.ptr, mats.data(), sizeof(jsvd::WaveMatrix) * entries, hipMemcpyHostToDevice));
+        return JSVD_OK;
     }
 
     struct DeviceContext
@@ -604,6 +753,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             }
             it->second.push_back(w);
         }
+        else if (!uses_blocked(cfg, n[k]) && m[k] <= jsvd::wave_sweeps_max_rows())
+            plan->wave.push_back(k);
         else
         {
             plan->large.push_back(k);
@@ -778,6 +929,13 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                                              static_cast<int>(need)));
         }
     }
+    if (!plan->wave.empty())
+        JSVD_CHECK(plan_wave_setup(*plan, prop.multiProcessorCount));
+    if (!plan->large.empty() || !plan->wave.empty())
+    {
+        JSVD_HIP(hipEventCreateWithFlags(&plan->big_in, hipEventDisableTiming));
+        JSVD_HIP(hipEventCreateWithFlags(&plan->big_out, hipEventDisableTiming));
+    }
     if (!plan->large.empty())
     {
         plan->big_block = persistent_kernel_block(cfg.threads_per_block);
@@ -787,8 +945,6 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         JSVD_CHECK(plan->ws_flags.alloc(cfg.max_sweeps));
         JSVD_CHECK(plan->ws_bar.alloc(2));
         JSVD_HIP(hipMemset(plan->ws_bar.ptr, 0, sizeof(unsigned int) * 2));
-        JSVD_HIP(hipEventCreateWithFlags(&plan->big_in, hipEventDisableTiming));
-        JSVD_HIP(hipEventCreateWithFlags(&plan->big_out, hipEventDisableTiming));
         if (plan->blocked > 0)
         {
             JSVD_CHECK(plan->bk_gram.alloc(big_gram));
@@ -886,7 +1042,7 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
         JSVD_HIP(hipEventRecord(p->join[i], p->aux[i]));
         JSVD_HIP(hipStreamWaitEvent(stream, p->join[i], 0));
     }
-    if (!p->large.empty())
+    if (!p->large.empty() || !p->wave.empty())
     {
         // Large matrices run on the device's barrier stream, ordered after
         // the caller's prior work and before its later work.
@@ -894,6 +1050,8 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
         JSVD_CHECK(barrier_stream(p->device, &bs));
         JSVD_HIP(hipEventRecord(p->big_in, stream));
         JSVD_HIP(hipStreamWaitEvent(bs, p->big_in, 0));
+        if (!p->wave.empty())
+            JSVD_CHECK(plan_wave(*p, d_a, d_u, d_sigma, d_v, d_sweeps, bs));
         for (int64_t k : p->large)
             JSVD_CHECK(plan_large(*p, k, d_a, d_u, d_sigma, d_v, d_sweeps, bs));
         JSVD_HIP(hipEventRecord(p->big_out, bs));
@@ -908,7 +1066,7 @@ jsvd_status jsvd_plan_get_info(const jsvd_plan *p, jsvd_plan_info *info)
         return fail(JSVD_EINVAL, "plan or info pointer is null.");
     info->count = p->count;
     info->lds_matrices = static_cast<int64_t>(p->items.size());
-    info->large_matrices = static_cast<int64_t>(p->large.size());
+    info->large_matrices = static_cast<int64_t>(p->large.size() + p->wave.size());
     info->a_elements = p->a_elems;
     info->v_elements = p->v_elems;
     info->sigma_elements = p->s_elems;

@@ -235,33 +235,37 @@ namespace jsvd::dev
         return x;
     }
 
-    // Sense-free generation barrier across all workgroups of a launch whose
-    // grid is co-resident (grid <= resident workgroups, one such launch on
-    // the GPU at a time).  bar[0] counts arrivals, bar[1] is the generation.  The
-    // agent-scope fences write back and invalidate the per-XCD L2 so that
-    // columns written in one round are seen by every XCD in the next.
+    // Sense-free generation barrier across `nblocks` co-resident workgroups
+    // (a launch's grid, or one group of it; grid <= resident workgroups, one
+    // such launch on the GPU at a time).  bar[0] counts arrivals, bar[1] is
+    // the generation.  Each workgroup's arrival is a release RMW (its
+    // columns, written before the workgroup barrier, reach the point of
+    // agent coherence first); the last arrival acquires every other
+    // arrival's writes through the RMW chain, then publishes the new
+    // generation with a release store; the others poll with relaxed loads
+    // (an acquire load would invalidate this XCD's L2 on every iteration)
+    // and acquire once, after the generation has changed.  So each
+    // workgroup pays one L2 write-back and one invalidate per barrier.
     __device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned int nblocks)
     {
         __syncthreads();
         if (threadIdx.x == 0)
         {
             const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
             const unsigned int arrived =
-                __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (arrived == nblocks - 1)
             {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
             else
             {
-                // Poll with relaxed loads (an acquire load would invalidate
-                // this XCD's L2 on every iteration); one acquire fence below.
                 while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
                     __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
         __syncthreads();
     }

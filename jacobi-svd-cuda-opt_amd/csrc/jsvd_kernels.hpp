@@ -117,14 +117,39 @@ namespace jsvd
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
                                             int32_t *sweeps_out, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
-    // One wave per column pair (workgroups of eight waves), for m up to
-    // wave_sweeps_max_rows(); same arguments as the one-workgroup-per-pair
-    // kernel above.
+    // Matrices too large for LDS with m up to wave_sweeps_max_rows(): one
+    // wave per column pair (workgroups of eight waves), solved by groups of
+    // `members` workgroups, each group with its own barrier, so one launch
+    // runs `groups` independent solves at once.
+    struct WaveMatrix
+    {
+        double *a;      // column-major m x n workspace (leading dimension m)
+        double *v;      // column-major n x n workspace, identity at launch
+        int32_t *flags; // max_sweeps sweep flags, zero at launch
+        int32_t m;
+        int32_t n;
+        int32_t index;  // sweeps output slot
+        int32_t pad;
+    };
+    struct WaveGroupArgs
+    {
+        const WaveMatrix *mats; // longest first; group g takes g, g + groups, ...
+        int count;
+        int members;            // workgroups per group
+        double eps;
+        int max_sweeps;
+        unsigned int *bars;     // two words per group (arrivals, generation)
+        int32_t *sweeps;        // optional
+    };
     int wave_sweeps_max_rows();
-    const void *wave_sweeps_f64_symbol(int64_t m);
-    hipError_t launch_wave_sweeps_f64(int grid, double *a_cm, double *v_cm, int64_t m, int n, double eps,
-                                      int max_sweeps, int32_t *flags, unsigned int *barrier, int32_t *sweeps_out,
-                                      hipStream_t stream);
+    // Kernel instance for matrices of up to m rows (all matrices of one
+    // launch share it: ceil(m / 64) row registers per lane).
+    const void *wave_group_sweeps_f64_symbol(int64_t m);
+    hipError_t launch_wave_group_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream);
+    // The same solves, one workgroup of sixteen waves per matrix (no
+    // cross-workgroup barrier; `members` and `bars` are unused): the
+    // throughput form for many matrices.
+    hipError_t launch_wave_block_sweeps_f64(int64_t m, int grid, const WaveGroupArgs &args, hipStream_t stream);
 
     // Blocked Jacobi over one column-major matrix whose n columns are padded
     // with zero columns to nv = nb * block and whose m rows are padded with

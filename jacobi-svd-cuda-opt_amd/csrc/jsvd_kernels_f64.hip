@@ -373,90 +373,190 @@ namespace jsvd
                 *sweeps_out = executed;
         }
 
-        // One wave per column pair for a matrix of up to 64 * RW rows: lane l
-        // holds rows l + 64 j (j < RW) of both columns in registers, the three
-        // dots are reduced across the wave (DPP and lane shuffles), so a
-        // round needs no workgroup barrier; eight waves per workgroup, a grid
-        // barrier between rounds (an eighth of the arrivals of one workgroup
-        // per pair; four or sixteen waves per workgroup measured slower).  Rotation parameters from the refined hardware estimates.
+        // Matrices too large for LDS with at most 64 * RW rows, solved by
+        // groups of `members` workgroups: group g takes matrices g, g +
+        // groups, ... of the list (sorted longest first by the host), and its
+        // workgroups meet at a barrier of their own after every round, so
+        // the groups of one launch run independent solves side by side.  One
+        // wave per column pair: lane l holds rows l + 64 j (j < RW) of both
+        // columns in registers, the three dots are reduced across the wave
+        // (DPP and lane shuffles), so a round needs no workgroup barrier;
+        // eight waves per workgroup (an eighth of the barrier arrivals of one
+        // workgroup per pair; four or sixteen waves measured slower).  A wave
+        // takes pairs k, k + 8 members, ... of each round, so the arithmetic
+        // of a pair, and with it the result, does not depend on the group
+        // size.  Rotation parameters from the refined hardware estimates.
         // Same pairs and order as the reference circle.
         template <int RW>
-        __global__ __launch_bounds__(512) void wave_sweeps_f64(double *__restrict__ A, double *__restrict__ V,
-                                                               int64_t m, int n, double eps, int max_sweeps,
-                                                               int32_t *__restrict__ flags,
-                                                               unsigned int *__restrict__ bar,
-                                                               int32_t *__restrict__ sweeps_out)
+        __global__ __launch_bounds__(512) void wave_group_sweeps_f64(WaveGroupArgs g)
         {
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            const int even = n + (n & 1), half = even >> 1;
-            const int rounds = n < 2 ? 0 : even - 1;
-            const int stride = gridDim.x * 8;
-            int executed = 0;
-            for (int sweep = 0; sweep < max_sweeps; ++sweep)
+            const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
+            const int groups = gridDim.x / g.members;
+            unsigned int *bar = g.bars + 2 * group;
+            const int stride = g.members * 8;
+            for (int idx = group; idx < g.count; idx += groups)
             {
-                int any = 0;
-                for (int r = 0; r < rounds; ++r)
+                const WaveMatrix w = g.mats[idx];
+                double *__restrict__ A = w.a;
+                double *__restrict__ V = w.v;
+                const int64_t m = w.m;
+                const int n = w.n;
+                const int even = n + (n & 1), half = even >> 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                int executed = 0;
+                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
                 {
-                    for (int k = blockIdx.x * 8 + wave; k < half; k += stride)
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
                     {
-                        const int p = dev::column_at(k, r, n, even);
-                        const int q = dev::column_at(even - 1 - k, r, n, even);
-                        if (p < 0 || q < 0)
-                            continue;
-                        double *ap = A + (int64_t)p * m;
-                        double *aq = A + (int64_t)q * m;
-                        double xp[RW], xq[RW];
-#pragma unroll
-                        for (int j = 0; j < RW; ++j)
+                        for (int k = member * 8 + wave; k < half; k += stride)
                         {
-                            const int64_t i = lane + 64 * j;
-                            xp[j] = i < m ? ap[i] : 0.0;
-                            xq[j] = i < m ? aq[i] : 0.0;
-                        }
-                        double pp = 0.0, qq = 0.0, pq = 0.0;
+                            const int p = dev::column_at(k, r, n, even);
+                            const int q = dev::column_at(even - 1 - k, r, n, even);
+                            if (p < 0 || q < 0)
+                                continue;
+                            double *ap = A + (int64_t)p * m;
+                            double *aq = A + (int64_t)q * m;
+                            double xp[RW], xq[RW];
 #pragma unroll
-                        for (int j = 0; j < RW; ++j)
-                        {
-                            pp = fma(xp[j], xp[j], pp);
-                            qq = fma(xq[j], xq[j], qq);
-                            pq = fma(xp[j], xq[j], pq);
-                        }
-                        dev::group_sum3_dpp<64>(pp, qq, pq);
-                        if (!dev::rotation_needed(pp, qq, pq, eps))
-                            continue;
-                        any = 1;
-                        double c, s;
-                        dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
-#pragma unroll
-                        for (int j = 0; j < RW; ++j)
-                        {
-                            const int64_t i = lane + 64 * j;
-                            if (i < m)
+                            for (int j = 0; j < RW; ++j)
                             {
-                                ap[i] = c * xp[j] - s * xq[j];
-                                aq[i] = s * xp[j] + c * xq[j];
+                                const int64_t i = lane + 64 * j;
+                                xp[j] = i < m ? ap[i] : 0.0;
+                                xq[j] = i < m ? aq[i] : 0.0;
+                            }
+                            double pp = 0.0, qq = 0.0, pq = 0.0;
+#pragma unroll
+                            for (int j = 0; j < RW; ++j)
+                            {
+                                pp = fma(xp[j], xp[j], pp);
+                                qq = fma(xq[j], xq[j], qq);
+                                pq = fma(xp[j], xq[j], pq);
+                            }
+                            dev::group_sum3_dpp<64>(pp, qq, pq);
+                            if (!dev::rotation_needed(pp, qq, pq, g.eps))
+                                continue;
+                            any = 1;
+                            double c, s;
+                            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+#pragma unroll
+                            for (int j = 0; j < RW; ++j)
+                            {
+                                const int64_t i = lane + 64 * j;
+                                if (i < m)
+                                {
+                                    ap[i] = c * xp[j] - s * xq[j];
+                                    aq[i] = s * xp[j] + c * xq[j];
+                                }
+                            }
+                            double *vp = V + (int64_t)p * n;
+                            double *vq = V + (int64_t)q * n;
+                            for (int i = lane; i < n; i += 64)
+                            {
+                                const double x = vp[i], y = vq[i];
+                                vp[i] = c * x - s * y;
+                                vq[i] = s * x + c * y;
                             }
                         }
-                        double *vp = V + (int64_t)p * n;
-                        double *vq = V + (int64_t)q * n;
-                        for (int i = lane; i < n; i += 64)
-                        {
-                            const double x = vp[i], y = vq[i];
-                            vp[i] = c * x - s * y;
-                            vq[i] = s * x + c * y;
-                        }
+                        if (r == rounds - 1 && any && lane == 0)
+                            __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        dev::grid_barrier(bar, g.members);
                     }
-                    if (r == rounds - 1 && any && lane == 0)
-                        __hip_atomic_fetch_or(&flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    dev::grid_barrier(bar, gridDim.x);
+                    executed = sweep + 1;
+                    const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (f == 0)
+                        break;
                 }
-                executed = sweep + 1;
-                const int f = __hip_atomic_load(&flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (f == 0)
-                    break;
+                if (member == 0 && threadIdx.x == 0 && g.sweeps)
+                    g.sweeps[w.index] = executed;
             }
-            if (blockIdx.x == 0 && threadIdx.x == 0 && sweeps_out)
-                *sweeps_out = executed;
+        }
+
+        // The same solves with one workgroup of sixteen waves per matrix:
+        // workgroup w takes matrices w, w + grid, ... of the list.  A and V
+        // stay in global memory (L2 / Infinity Cache), but only this
+        // workgroup touches them, so a round ends with a workgroup barrier
+        // alone (no cross-CU write-back or invalidate).  Wave k takes pairs
+        // k, k + 16, ... of a round; the pair arithmetic is the grouped
+        // kernel's, so results equal it bitwise.
+        template <int RW>
+        __global__ __launch_bounds__(1024) void wave_block_sweeps_f64(WaveGroupArgs g)
+        {
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            for (int idx = blockIdx.x; idx < g.count; idx += gridDim.x)
+            {
+                const WaveMatrix w = g.mats[idx];
+                double *__restrict__ A = w.a;
+                double *__restrict__ V = w.v;
+                const int64_t m = w.m;
+                const int n = w.n;
+                const int even = n + (n & 1), half = even >> 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                int executed = 0;
+                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
+                {
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        for (int k = wave; k < half; k += 16)
+                        {
+                            const int p = dev::column_at(k, r, n, even);
+                            const int q = dev::column_at(even - 1 - k, r, n, even);
+                            if (p < 0 || q < 0)
+                                continue;
+                            double *ap = A + (int64_t)p * m;
+                            double *aq = A + (int64_t)q * m;
+                            double xp[RW], xq[RW];
+#pragma unroll
+                            for (int j = 0; j < RW; ++j)
+                            {
+                                const int64_t i = lane + 64 * j;
+                                xp[j] = i < m ? ap[i] : 0.0;
+                                xq[j] = i < m ? aq[i] : 0.0;
+                            }
+                            double pp = 0.0, qq = 0.0, pq = 0.0;
+#pragma unroll
+                            for (int j = 0; j < RW; ++j)
+                            {
+                                pp = fma(xp[j], xp[j], pp);
+                                qq = fma(xq[j], xq[j], qq);
+                                pq = fma(xp[j], xq[j], pq);
+                            }
+                            dev::group_sum3_dpp<64>(pp, qq, pq);
+                            if (!dev::rotation_needed(pp, qq, pq, g.eps))
+                                continue;
+                            any = 1;
+                            double c, s;
+                            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+#pragma unroll
+                            for (int j = 0; j < RW; ++j)
+                            {
+                                const int64_t i = lane + 64 * j;
+                                if (i < m)
+                                {
+                                    ap[i] = c * xp[j] - s * xq[j];
+                                    aq[i] = s * xp[j] + c * xq[j];
+                                }
+                            }
+                            double *vp = V + (int64_t)p * n;
+                            double *vq = V + (int64_t)q * n;
+                            for (int i = lane; i < n; i += 64)
+                            {
+                                const double x = vp[i], y = vq[i];
+                                vp[i] = c * x - s * y;
+                                vq[i] = s * x + c * y;
+                            }
+                        }
+                        __syncthreads();
+                    }
+                    executed = sweep + 1;
+                    if (!__syncthreads_or(any))
+                        break;
+                }
+                if (threadIdx.x == 0 && g.sweeps)
+                    g.sweeps[w.index] = executed;
+            }
         }
 
         // Tiled transpose through a padded LDS tile (32 x 33 doubles: the
@@ -600,32 +700,57 @@ namespace jsvd
         return 64 * kWaveRowsMax;
     }
 
-    const void *wave_sweeps_f64_symbol(int64_t m)
+    const void *wave_group_sweeps_f64_symbol(int64_t m)
     {
         const int rw = static_cast<int>((m + 63) / 64);
         switch (rw)
         {
-        case 1: return reinterpret_cast<const void *>(&wave_sweeps_f64<1>);
-        case 2: return reinterpret_cast<const void *>(&wave_sweeps_f64<2>);
-        case 3: return reinterpret_cast<const void *>(&wave_sweeps_f64<3>);
-        case 4: return reinterpret_cast<const void *>(&wave_sweeps_f64<4>);
-        case 5: return reinterpret_cast<const void *>(&wave_sweeps_f64<5>);
-        case 6: return reinterpret_cast<const void *>(&wave_sweeps_f64<6>);
-        case 7: return reinterpret_cast<const void *>(&wave_sweeps_f64<7>);
-        case 8: return reinterpret_cast<const void *>(&wave_sweeps_f64<8>);
+        case 1: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<1>);
+        case 2: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<2>);
+        case 3: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<3>);
+        case 4: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<4>);
+        case 5: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<5>);
+        case 6: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<6>);
+        case 7: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<7>);
+        case 8: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<8>);
         default: return nullptr;
         }
     }
 
-    hipError_t launch_wave_sweeps_f64(int grid, double *a_cm, double *v_cm, int64_t m, int n, double eps,
-                                      int max_sweeps, int32_t *flags, unsigned int *barrier, int32_t *sweeps_out,
-                                      hipStream_t stream)
+    hipError_t launch_wave_block_sweeps_f64(int64_t m, int grid, const WaveGroupArgs &args, hipStream_t stream)
     {
+        if (grid < 1 || args.count < 1)
+            return hipErrorInvalidValue;
         const int rw = static_cast<int>((m + 63) / 64);
 #define JSVD_WAVE_CASE(RW)                                                                                         \
     case RW:                                                                                                       \
-        wave_sweeps_f64<RW><<<grid, 512, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier,          \
-                                                      sweeps_out);                                                \
+        wave_block_sweeps_f64<RW><<<grid, 1024, 0, stream>>>(args);                                                \
+        break;
+        switch (rw)
+        {
+            JSVD_WAVE_CASE(1)
+            JSVD_WAVE_CASE(2)
+            JSVD_WAVE_CASE(3)
+            JSVD_WAVE_CASE(4)
+            JSVD_WAVE_CASE(5)
+            JSVD_WAVE_CASE(6)
+            JSVD_WAVE_CASE(7)
+            JSVD_WAVE_CASE(8)
+        default: return hipErrorInvalidValue;
+        }
+#undef JSVD_WAVE_CASE
+        return hipGetLastError();
+    }
+
+    hipError_t launch_wave_group_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream)
+    {
+        if (groups < 1 || args.members < 1 || args.count < 1)
+            return hipErrorInvalidValue;
+        const int rw = static_cast<int>((m + 63) / 64);
+        const unsigned int grid = static_cast<unsigned int>(groups) * static_cast<unsigned int>(args.members);
+#define JSVD_WAVE_CASE(RW)                                                                                         \
+    case RW:                                                                                                       \
+        wave_group_sweeps_f64<RW><<<grid, 512, 0, stream>>>(args);                                                 \
         break;
         switch (rw)
         {

@@ -94,6 +94,61 @@ def test_persistent_path_against_oracle(shape, threads):
     _assert_parity(a, got, ref, 1e-13, str(shape))
 
 
+def test_grouped_wave_batch_against_oracle_and_single_solves():
+    # Matrices too large for LDS with at most 512 rows go to the wave-per-pair
+    # kernels, one launch per row-register count: the 121 matrices of up to
+    # 192 rows run one workgroup per matrix, the few wider ones grouped
+    # launches (groups of workgroups with barriers of their own).  The pair
+    # arithmetic is the same in both forms and for any group size, so every
+    # matrix must equal its single solve bitwise, and the oracle to tolerance.
+    _note("grouped wave batch")
+    rng = np.random.default_rng(515)
+    shapes = [(256, 256), (300, 150), (200, 180), (512, 100), (257, 200), (192, 192), (300, 151)]
+    shapes += [(160, 150)] * 120 + [(64, 40), (30, 20), (128, 64)]
+    mats = [rng.standard_normal(sh) for sh in shapes]
+    ms = [a.shape[0] for a in mats]
+    ns = [a.shape[1] for a in mats]
+    plan = jsvd.BatchPlan(ms, ns, epsilon=1e-13, max_sweeps=30)
+    assert plan.info["large_matrices"] == len(shapes) - 3
+    plan.close()
+    got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
+    ref = oracle.solve_batch(mats, 1e-13, 30)
+    for k, (a, g, r) in enumerate(zip(mats, got, ref)):
+        _assert_parity(a, g, r, 1e-13, f"grouped{k} {a.shape}")
+    for k in list(range(7)) + [7, 70, 126, 128]:
+        single = jsvd.svd(mats[k], epsilon=1e-13, max_sweeps=30)
+        for x, y in zip(single[:3], got[k][:3]):
+            np.testing.assert_array_equal(x, y)
+        assert single[3] == got[k][3]
+
+
+def test_grouped_wave_launch_with_narrow_groups():
+    # 20 matrices of 512 x 512 stay in one grouped launch, and 20 groups of one
+    # wave per pair (32 workgroups each) exceed the resident workgroups, so
+    # the groups get fewer workgroups and each wave takes several pairs per
+    # round.  Each matrix must equal its single solve bitwise; sigma is
+    # checked against LAPACK (the oracle needs seconds per matrix here).
+    # eps = 1e-14 (above the rounding floor of 512-row dots, about 2.5e-15):
+    # at 1e-13, 512 columns each within eps of orthogonal can sum to
+    # ||U^T U - I||_F above the 9e-12 bound.
+    _note("grouped wave launch, 20 x 512x512")
+    mats = [gen.gaussian_square(512, seed=gen.BASE_SEED + k) for k in range(20)]
+    got = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30)
+    for k, (a, g) in enumerate(zip(mats, got)):
+        u, s, v, sw = g
+        ref = np.linalg.svd(a, compute_uv=False)
+        assert np.max(np.abs(np.sort(s)[::-1] - ref)) <= 1e-12 * ref[0], f"sigma {k}"
+        tol = 1e-12 * (512 / 64 + 1)
+        assert metrics.orthogonality(u, s > 1e-14) <= tol, f"U orthogonality {k}"
+        assert metrics.orthogonality(v) <= tol, f"V orthogonality {k}"
+        assert metrics.residual(a, u, s, v) <= tol, f"residual {k}"
+    for k in (0, 19):
+        single = jsvd.svd(mats[k], epsilon=1e-14, max_sweeps=30)
+        for x, y in zip(single[:3], got[k][:3]):
+            np.testing.assert_array_equal(x, y)
+        assert single[3] == got[k][3]
+
+
 def test_every_shape_class_against_oracle():
     # The autotuner may assign any fitting class to a column-count bucket:
     # force each class in turn on a batch whose shapes it fits.
