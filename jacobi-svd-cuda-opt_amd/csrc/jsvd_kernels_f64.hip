@@ -373,6 +373,71 @@ namespace jsvd
                 *sweeps_out = executed;
         }
 
+        // One column pair of a column-major m x n matrix (m <= 64 RW, n <= m)
+        // by one wave: lane l loads rows l + 64 j of both A columns and of
+        // both V columns at once (V's loads do not wait for the rotation
+        // parameters), reduces the three dots across the wave, and applies
+        // the rotation when the pair is not yet orthogonal.  Returns 1 when
+        // it rotated.
+        template <int RW>
+        __device__ __forceinline__ int wave_pair_f64(double *__restrict__ A, double *__restrict__ V, int64_t m,
+                                                     int n, int p, int q, int lane, double eps)
+        {
+            double *ap = A + (int64_t)p * m;
+            double *aq = A + (int64_t)q * m;
+            double *vp = V + (int64_t)p * n;
+            double *vq = V + (int64_t)q * n;
+            double xp[RW], xq[RW], yp[RW], yq[RW];
+#pragma unroll
+            for (int j = 0; j < RW; ++j)
+            {
+                const int64_t i = lane + 64 * j;
+                xp[j] = i < m ? ap[i] : 0.0;
+                xq[j] = i < m ? aq[i] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < RW; ++j)
+            {
+                const int i = lane + 64 * j;
+                yp[j] = i < n ? vp[i] : 0.0;
+                yq[j] = i < n ? vq[i] : 0.0;
+            }
+            double pp = 0.0, qq = 0.0, pq = 0.0;
+#pragma unroll
+            for (int j = 0; j < RW; ++j)
+            {
+                pp = fma(xp[j], xp[j], pp);
+                qq = fma(xq[j], xq[j], qq);
+                pq = fma(xp[j], xq[j], pq);
+            }
+            dev::group_sum3_dpp<64>(pp, qq, pq);
+            if (!dev::rotation_needed(pp, qq, pq, eps))
+                return 0;
+            double c, s;
+            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+#pragma unroll
+            for (int j = 0; j < RW; ++j)
+            {
+                const int64_t i = lane + 64 * j;
+                if (i < m)
+                {
+                    ap[i] = c * xp[j] - s * xq[j];
+                    aq[i] = s * xp[j] + c * xq[j];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < RW; ++j)
+            {
+                const int i = lane + 64 * j;
+                if (i < n)
+                {
+                    vp[i] = c * yp[j] - s * yq[j];
+                    vq[i] = s * yp[j] + c * yq[j];
+                }
+            }
+            return 1;
+        }
+
         // Matrices too large for LDS with at most 64 * RW rows, solved by
         // groups of `members` workgroups: group g takes matrices g, g +
         // groups, ... of the list (sorted longest first by the host), and its
@@ -416,48 +481,7 @@ namespace jsvd
                             const int q = dev::column_at(even - 1 - k, r, n, even);
                             if (p < 0 || q < 0)
                                 continue;
-                            double *ap = A + (int64_t)p * m;
-                            double *aq = A + (int64_t)q * m;
-                            double xp[RW], xq[RW];
-#pragma unroll
-                            for (int j = 0; j < RW; ++j)
-                            {
-                                const int64_t i = lane + 64 * j;
-                                xp[j] = i < m ? ap[i] : 0.0;
-                                xq[j] = i < m ? aq[i] : 0.0;
-                            }
-                            double pp = 0.0, qq = 0.0, pq = 0.0;
-#pragma unroll
-                            for (int j = 0; j < RW; ++j)
-                            {
-                                pp = fma(xp[j], xp[j], pp);
-                                qq = fma(xq[j], xq[j], qq);
-                                pq = fma(xp[j], xq[j], pq);
-                            }
-                            dev::group_sum3_dpp<64>(pp, qq, pq);
-                            if (!dev::rotation_needed(pp, qq, pq, g.eps))
-                                continue;
-                            any = 1;
-                            double c, s;
-                            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
-#pragma unroll
-                            for (int j = 0; j < RW; ++j)
-                            {
-                                const int64_t i = lane + 64 * j;
-                                if (i < m)
-                                {
-                                    ap[i] = c * xp[j] - s * xq[j];
-                                    aq[i] = s * xp[j] + c * xq[j];
-                                }
-                            }
-                            double *vp = V + (int64_t)p * n;
-                            double *vq = V + (int64_t)q * n;
-                            for (int i = lane; i < n; i += 64)
-                            {
-                                const double x = vp[i], y = vq[i];
-                                vp[i] = c * x - s * y;
-                                vq[i] = s * x + c * y;
-                            }
+                            any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
                         }
                         if (r == rounds - 1 && any && lane == 0)
                             __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -479,7 +503,8 @@ namespace jsvd
         // workgroup touches them, so a round ends with a workgroup barrier
         // alone (no cross-CU write-back or invalidate).  Wave k takes pairs
         // k, k + 16, ... of a round; the pair arithmetic is the grouped
-        // kernel's, so results equal it bitwise.
+        // kernel's, so results equal it bitwise.  Bound by the CU's L2
+        // bandwidth: a round of a 256 x 256 matrix moves 2 MB through one CU.
         template <int RW>
         __global__ __launch_bounds__(1024) void wave_block_sweeps_f64(WaveGroupArgs g)
         {
@@ -505,48 +530,7 @@ namespace jsvd
                             const int q = dev::column_at(even - 1 - k, r, n, even);
                             if (p < 0 || q < 0)
                                 continue;
-                            double *ap = A + (int64_t)p * m;
-                            double *aq = A + (int64_t)q * m;
-                            double xp[RW], xq[RW];
-#pragma unroll
-                            for (int j = 0; j < RW; ++j)
-                            {
-                                const int64_t i = lane + 64 * j;
-                                xp[j] = i < m ? ap[i] : 0.0;
-                                xq[j] = i < m ? aq[i] : 0.0;
-                            }
-                            double pp = 0.0, qq = 0.0, pq = 0.0;
-#pragma unroll
-                            for (int j = 0; j < RW; ++j)
-                            {
-                                pp = fma(xp[j], xp[j], pp);
-                                qq = fma(xq[j], xq[j], qq);
-                                pq = fma(xp[j], xq[j], pq);
-                            }
-                            dev::group_sum3_dpp<64>(pp, qq, pq);
-                            if (!dev::rotation_needed(pp, qq, pq, g.eps))
-                                continue;
-                            any = 1;
-                            double c, s;
-                            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
-#pragma unroll
-                            for (int j = 0; j < RW; ++j)
-                            {
-                                const int64_t i = lane + 64 * j;
-                                if (i < m)
-                                {
-                                    ap[i] = c * xp[j] - s * xq[j];
-                                    aq[i] = s * xp[j] + c * xq[j];
-                                }
-                            }
-                            double *vp = V + (int64_t)p * n;
-                            double *vq = V + (int64_t)q * n;
-                            for (int i = lane; i < n; i += 64)
-                            {
-                                const double x = vp[i], y = vq[i];
-                                vp[i] = c * x - s * y;
-                                vq[i] = s * x + c * y;
-                            }
+                            any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
                         }
                         __syncthreads();
                     }
