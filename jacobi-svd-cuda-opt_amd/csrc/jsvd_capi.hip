@@ -514,7 +514,10 @@ namespace
     // round).
     jsvd_status plan_wave_setup(jsvd_plan &p, int cus)
     {
-        auto rw = [&](int64_t k) { return (p.m[k] + 63) / 64; };
+        // Row-register count; every matrix beyond the register kernels'
+        // height shares one streamed launch.
+        const int64_t rw_max = jsvd::wave_sweeps_max_rows() / 64;
+        auto rw = [&](int64_t k) { return std::min<int64_t>((p.m[k] + 63) / 64, rw_max + 1); };
         auto cost = [&](int64_t k) { return double(p.n[k]) * p.n[k] * (p.m[k] + p.n[k]); };
         std::stable_sort(p.wave.begin(), p.wave.end(), [&](int64_t x, int64_t y) {
             if (rw(x) != rw(y))
@@ -547,7 +550,7 @@ namespace
                 ++e;
             }
             l.count = static_cast<int>(e - l.first);
-            if (wave_block_mode(l.count, max_half))
+            if (l.m_max > jsvd::wave_sweeps_max_rows() || wave_block_mode(l.count, max_half))
             {
                 l.members = 0;
                 l.groups = l.count;
@@ -685,6 +688,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     std::vector<std::pair<GroupKey, std::vector<jsvd::WorkItem>>> groups;
     bool any_mixed = false;
     static const int kSlots[] = {32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
+    std::vector<int64_t> tall;
     bool any_four_wave = false;
     for (int64_t k = 0; k < count && !any_four_wave; ++k)
         any_four_wave = tuned_class(m[k], n[k]) < 0 && jsvd::select_class_f64(m[k], n[k]) == 4;
@@ -755,6 +759,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         }
         else if (!uses_blocked(cfg, n[k]) && m[k] <= jsvd::wave_sweeps_max_rows())
             plan->wave.push_back(k);
+        else if (!uses_blocked(cfg, n[k]))
+            tall.push_back(k);
         else
         {
             plan->large.push_back(k);
@@ -775,6 +781,22 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             big_n = std::max(big_n, n[k]);
         }
     }
+    // Plain-path matrices of more than wave_sweeps_max_rows() rows: many of
+    // them run one workgroup per matrix (rows streamed) beside the other
+    // wave-path matrices; a few run one at a time on the grid-barrier kernel.
+    int64_t tall_half = 1;
+    for (int64_t k : tall)
+        tall_half = std::max<int64_t>(tall_half, (n[k] + 1) / 2);
+    if (wave_block_mode(static_cast<int64_t>(tall.size()), tall_half))
+        plan->wave.insert(plan->wave.end(), tall.begin(), tall.end());
+    else
+        for (int64_t k : tall)
+        {
+            plan->large.push_back(k);
+            big_mn = std::max(big_mn, m[k] * n[k]);
+            big_nn = std::max(big_nn, n[k] * n[k]);
+            big_n = std::max(big_n, n[k]);
+        }
     plan->a_elems = ao;
     plan->v_elems = vo;
     plan->s_elems = so;

@@ -148,7 +148,9 @@ namespace jsvd
     hipError_t launch_wave_group_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream);
     // The same solves, one workgroup of sixteen waves per matrix (no
     // cross-workgroup barrier; `members` and `bars` are unused): the
-    // throughput form for many matrices.
+    // throughput form for many matrices.  Matrices of more than
+    // wave_sweeps_max_rows() rows are accepted here (rows streamed; `m` is
+    // then the launch's largest row count).
     hipError_t launch_wave_block_sweeps_f64(int64_t m, int grid, const WaveGroupArgs &args, hipStream_t stream);
 
     // Blocked Jacobi over one column-major matrix whose n columns are padded

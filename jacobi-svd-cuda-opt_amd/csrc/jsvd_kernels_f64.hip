@@ -543,6 +543,109 @@ namespace jsvd
             }
         }
 
+        // One column pair of any height by one wave, rows streamed in chunks
+        // of 256 (4 per lane): the dots accumulate rows lane + 64 j in
+        // increasing j, the same FMA chain as wave_pair_f64, so both give the
+        // same result bitwise; the update re-reads the columns.
+        __device__ __forceinline__ int wave_pair_stream_f64(double *__restrict__ A, double *__restrict__ V,
+                                                            int64_t m, int n, int p, int q, int lane, double eps)
+        {
+            constexpr int CH = 4;
+            double *ap = A + (int64_t)p * m;
+            double *aq = A + (int64_t)q * m;
+            double pp = 0.0, qq = 0.0, pq = 0.0;
+            for (int64_t r0 = 0; r0 < m; r0 += 64 * CH)
+            {
+                double xp[CH], xq[CH];
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    const int64_t i = r0 + lane + 64 * j;
+                    xp[j] = i < m ? ap[i] : 0.0;
+                    xq[j] = i < m ? aq[i] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    pp = fma(xp[j], xp[j], pp);
+                    qq = fma(xq[j], xq[j], qq);
+                    pq = fma(xp[j], xq[j], pq);
+                }
+            }
+            dev::group_sum3_dpp<64>(pp, qq, pq);
+            if (!dev::rotation_needed(pp, qq, pq, eps))
+                return 0;
+            double c, s;
+            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+            for (int64_t r0 = 0; r0 < m; r0 += 64 * CH)
+            {
+                double xp[CH], xq[CH];
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    const int64_t i = r0 + lane + 64 * j;
+                    xp[j] = i < m ? ap[i] : 0.0;
+                    xq[j] = i < m ? aq[i] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    const int64_t i = r0 + lane + 64 * j;
+                    if (i < m)
+                    {
+                        ap[i] = c * xp[j] - s * xq[j];
+                        aq[i] = s * xp[j] + c * xq[j];
+                    }
+                }
+            }
+            double *vp = V + (int64_t)p * n;
+            double *vq = V + (int64_t)q * n;
+            for (int i = lane; i < n; i += 64)
+            {
+                const double x = vp[i], y = vq[i];
+                vp[i] = c * x - s * y;
+                vq[i] = s * x + c * y;
+            }
+            return 1;
+        }
+
+        // wave_block_sweeps_f64 for matrices of any height: one workgroup of
+        // sixteen waves per matrix, rows streamed (wave_pair_stream_f64).
+        __global__ __launch_bounds__(1024) void wave_block_stream_f64(WaveGroupArgs g)
+        {
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            for (int idx = blockIdx.x; idx < g.count; idx += gridDim.x)
+            {
+                const WaveMatrix w = g.mats[idx];
+                const int64_t m = w.m;
+                const int n = w.n;
+                const int even = n + (n & 1), half = even >> 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                int executed = 0;
+                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
+                {
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        for (int k = wave; k < half; k += 16)
+                        {
+                            const int p = dev::column_at(k, r, n, even);
+                            const int q = dev::column_at(even - 1 - k, r, n, even);
+                            if (p < 0 || q < 0)
+                                continue;
+                            any |= wave_pair_stream_f64(w.a, w.v, m, n, p, q, lane, g.eps);
+                        }
+                        __syncthreads();
+                    }
+                    executed = sweep + 1;
+                    if (!__syncthreads_or(any))
+                        break;
+                }
+                if (threadIdx.x == 0 && g.sweeps)
+                    g.sweeps[w.index] = executed;
+            }
+        }
+
         // Tiled transpose through a padded LDS tile (32 x 33 doubles: the
         // extra column puts the 32 elements of a tile column in distinct bank
         // pairs).  Block 32 x 8.
@@ -705,6 +808,11 @@ namespace jsvd
     {
         if (grid < 1 || args.count < 1)
             return hipErrorInvalidValue;
+        if (m > 64 * kWaveRowsMax)
+        {
+            wave_block_stream_f64<<<grid, 1024, 0, stream>>>(args);
+            return hipGetLastError();
+        }
         const int rw = static_cast<int>((m + 63) / 64);
 #define JSVD_WAVE_CASE(RW)                                                                                         \
     case RW:                                                                                                       \

@@ -122,6 +122,21 @@ def test_grouped_wave_batch_against_oracle_and_single_solves():
         assert single[3] == got[k][3]
 
 
+def test_tall_batch_streamed_against_oracle():
+    # More than max(4, pairs / 12) plain-path matrices taller than 512 rows
+    # run one workgroup per matrix with their rows streamed in 256-row
+    # chunks (one launch); a single such matrix stays on the grid-barrier
+    # kernel.  Uneven heights, odd and even widths.
+    _note("tall batch, streamed rows")
+    rng = np.random.default_rng(611)
+    shapes = [(1024, 64)] * 8 + [(700, 90), (513, 77), (2000, 40), (600, 129), (1500, 100), (777, 31)]
+    mats = [rng.standard_normal(sh) for sh in shapes]
+    got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
+    ref = oracle.solve_batch(mats, 1e-13, 30)
+    for k, (a, g, r) in enumerate(zip(mats, got, ref)):
+        _assert_parity(a, g, r, 1e-13, f"tall{k} {a.shape}")
+
+
 def test_grouped_wave_launch_with_narrow_groups():
     # 20 matrices of 512 x 512 stay in one grouped launch, and 20 groups of one
     # wave per pair (32 workgroups each) exceed the resident workgroups, so
