@@ -239,6 +239,16 @@ def plain_single(torch, jsvd, gen, cpu):
     out["batch_48"] = {"gpu_ms": batch_ms, "matrices_per_s": 48 / (batch_ms * 1e-3),
                        "mean_sweeps": float(bufs[4].double().mean().item())}
     plan.close()
+    # 64 tall 1024 x 64 N(0,1) matrices: one workgroup per matrix, rows
+    # streamed (taller than the register kernels).
+    tall = [np.random.default_rng(gen.BASE_SEED + 100 + k).standard_normal((1024, 64)) for k in range(64)]
+    flat, ms, ns = gen.pack(tall)
+    plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    bufs = device_buffers(plan, flat, torch)
+    wall, tall_ms = time_plan(plan, bufs, 3, 1, torch)
+    out["tall_batch_64x1024x64"] = {"gpu_ms": tall_ms, "matrices_per_s": 64 / (tall_ms * 1e-3),
+                                    "mean_sweeps": float(bufs[4].double().mean().item())}
+    plan.close()
     if cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle

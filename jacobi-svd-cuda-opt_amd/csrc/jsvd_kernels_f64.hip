@@ -545,8 +545,8 @@ namespace jsvd
 
         // One column pair of any height by one wave, rows streamed in chunks
         // of 256 (4 per lane): the dots accumulate rows lane + 64 j in
-        // increasing j, the same FMA chain as wave_pair_f64, so both give the
-        // same result bitwise; the update re-reads the columns.
+        // increasing j, the same FMA chain as wave_pair_f64 (equal dots, so
+        // equal rotation decisions); the update re-reads the columns.
         __device__ __forceinline__ int wave_pair_stream_f64(double *__restrict__ A, double *__restrict__ V,
                                                             int64_t m, int n, int p, int q, int lane, double eps)
         {
