@@ -442,15 +442,17 @@ namespace
     }
 
     // A launch of `count` matrices whose widest has `max_half` column pairs
-    // per round runs one workgroup per matrix when it has more matrices than
-    // max(4, max_half / 12): alone, a sixteen-wave workgroup takes half / 16
-    // pairs per wave and round, while a group's time grows with the number
-    // of groups that share the L2 write-backs of their barriers.  Measured on
-    // MI355X, the two forms cross at about 7 matrices of 160 x 150, 10 of
-    // 256 x 256 and 40 of 512 x 512.
+    // per round runs one workgroup per matrix when it has more than
+    // 4 + max_half^2 / 1600 matrices of at most 512 columns: alone, a
+    // sixteen-wave workgroup takes half / 16 pairs per wave and round, while
+    // a group's time grows with the number of groups that share the L2
+    // write-backs of their barriers.  Measured on MI355X, the two forms cross
+    // at about 7 matrices of 160 x 150, 10 of 256 x 256 and 40 of 512 x 512.
+    // Wider matrices keep the barrier kernels whatever their count (one
+    // workgroup would take dozens of pairs per wave and round).
     bool wave_block_mode(int64_t count, int64_t max_half)
     {
-        return count > std::max<int64_t>(4, max_half / 12);
+        return max_half <= 256 && count > 4 + max_half * max_half / 1600;
     }
 
     // Every grouped-wave matrix of the plan: row-major A into its

@@ -123,7 +123,7 @@ def test_grouped_wave_batch_against_oracle_and_single_solves():
 
 
 def test_tall_batch_streamed_against_oracle():
-    # More than max(4, pairs / 12) plain-path matrices taller than 512 rows
+    # More than 4 + pairs^2 / 1600 plain-path matrices taller than 512 rows
     # run one workgroup per matrix with their rows streamed in 256-row
     # chunks (one launch); a single such matrix stays on the grid-barrier
     # kernel.  Uneven heights, odd and even widths.
@@ -138,16 +138,17 @@ def test_tall_batch_streamed_against_oracle():
 
 
 def test_grouped_wave_launch_with_narrow_groups():
-    # 20 matrices of 512 x 512 stay in one grouped launch, and 20 groups of one
-    # wave per pair (32 workgroups each) exceed the resident workgroups, so
-    # the groups get fewer workgroups and each wave takes several pairs per
-    # round.  Each matrix must equal its single solve bitwise; sigma is
+    # 40 matrices of 512 x 512 stay in one grouped launch (below the
+    # 4 + 256^2 / 1600 matrices of the one-workgroup-per-matrix form), and 40
+    # groups of one wave per pair (32 workgroups each) exceed the at most
+    # 1024 resident workgroups of eight waves, so the groups get fewer
+    # workgroups and each wave takes several pairs per round.  Each matrix must equal its single solve bitwise; sigma is
     # checked against LAPACK (the oracle needs seconds per matrix here).
     # eps = 1e-14 (above the rounding floor of 512-row dots, about 2.5e-15):
     # at 1e-13, 512 columns each within eps of orthogonal can sum to
     # ||U^T U - I||_F above the 9e-12 bound.
-    _note("grouped wave launch, 20 x 512x512")
-    mats = [gen.gaussian_square(512, seed=gen.BASE_SEED + k) for k in range(20)]
+    _note("grouped wave launch, 40 x 512x512")
+    mats = [gen.gaussian_square(512, seed=gen.BASE_SEED + k) for k in range(40)]
     got = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30)
     for k, (a, g) in enumerate(zip(mats, got)):
         u, s, v, sw = g
@@ -157,7 +158,7 @@ def test_grouped_wave_launch_with_narrow_groups():
         assert metrics.orthogonality(u, s > 1e-14) <= tol, f"U orthogonality {k}"
         assert metrics.orthogonality(v) <= tol, f"V orthogonality {k}"
         assert metrics.residual(a, u, s, v) <= tol, f"residual {k}"
-    for k in (0, 19):
+    for k in (0, 39):
         single = jsvd.svd(mats[k], epsilon=1e-14, max_sweeps=30)
         for x, y in zip(single[:3], got[k][:3]):
             np.testing.assert_array_equal(x, y)
