@@ -299,6 +299,7 @@ struct jsvd_plan
         int first = 0, count = 0;
         int members = 1; // workgroups per group; 0 = one workgroup per matrix
         int groups = 1;
+        int waves = 8;   // waves per workgroup of the grouped form
     };
     std::vector<int64_t> wave;                 // batch index per entry
     std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry
@@ -443,16 +444,17 @@ namespace
 
     // A launch of `count` matrices whose widest has `max_half` column pairs
     // per round runs one workgroup per matrix when it has more than
-    // 4 + max_half^2 / 1600 matrices of at most 512 columns: alone, a
+    // 4 + max_half^2 / 400 matrices of at most 512 columns: alone, a
     // sixteen-wave workgroup takes half / 16 pairs per wave and round, while
     // a group's time grows with the number of groups that share the L2
-    // write-backs of their barriers.  Measured on MI355X, the two forms cross
-    // at about 7 matrices of 160 x 150, 10 of 256 x 256 and 40 of 512 x 512.
-    // Wider matrices keep the barrier kernels whatever their count (one
-    // workgroup would take dozens of pairs per wave and round).
+    // write-backs of their barriers.  Measured on MI355X (grouped form with
+    // sixteen-wave workgroups), the two forms cross at about 15 matrices of
+    // 160 x 150 and 48 of 256 x 256, and the grouped form still wins at 96
+    // of 512 x 512.  Wider matrices keep the barrier kernels whatever their
+    // count (one workgroup would take dozens of pairs per wave and round).
     bool wave_block_mode(int64_t count, int64_t max_half)
     {
-        return max_half <= 256 && count > 4 + max_half * max_half / 1600;
+        return max_half <= 256 && count > 4 + max_half * max_half / 400;
     }
 
     // Every grouped-wave matrix of the plan: row-major A into its
@@ -485,7 +487,7 @@ namespace
             if (l.members == 0)
                 JSVD_HIP(jsvd::launch_wave_block_sweeps_f64(l.m_max, l.count, args, stream));
             else
-                JSVD_HIP(jsvd::launch_wave_group_sweeps_f64(l.m_max, l.groups, args, stream));
+                JSVD_HIP(jsvd::launch_wave_group_sweeps_f64(l.m_max, l.waves, l.groups, args, stream));
         }
         for (int64_t e = 0; e < entries; ++e)
         {
@@ -559,11 +561,12 @@ namespace
                 p.wave_launches.push_back(l);
                 continue;
             }
+            l.waves = l.count <= 2 ? 8 : 16;
             int occupancy = 0;
             JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max), 512, 0));
+                &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max, l.waves), 64 * l.waves, 0));
             const int64_t resident = int64_t(std::max(1, occupancy)) * cus;
-            const int64_t need = (max_half + 7) / 8;
+            const int64_t need = (max_half + l.waves - 1) / l.waves;
             const int64_t members = std::max<int64_t>(1, std::min(need, resident / l.count));
             l.members = static_cast<int>(std::min(members, resident));
             l.groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(l.count, resident / l.members)));

@@ -118,7 +118,7 @@ namespace jsvd
                                             int32_t *sweeps_out, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
     // Matrices too large for LDS with m up to wave_sweeps_max_rows(): one
-    // wave per column pair (workgroups of eight waves), solved by groups of
+    // wave per column pair (workgroups of 8 or 16 waves), solved by groups of
     // `members` workgroups, each group with its own barrier, so one launch
     // runs `groups` independent solves at once.
     struct WaveMatrix
@@ -143,9 +143,11 @@ namespace jsvd
     };
     int wave_sweeps_max_rows();
     // Kernel instance for matrices of up to m rows (all matrices of one
-    // launch share it: ceil(m / 64) row registers per lane).
-    const void *wave_group_sweeps_f64_symbol(int64_t m);
-    hipError_t launch_wave_group_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream);
+    // launch share it: ceil(m / 64) row registers per lane) and workgroups
+    // of `waves` (8 or 16) waves.
+    const void *wave_group_sweeps_f64_symbol(int64_t m, int waves);
+    hipError_t launch_wave_group_sweeps_f64(int64_t m, int waves, int groups, const WaveGroupArgs &args,
+                                            hipStream_t stream);
     // The same solves, one workgroup of sixteen waves per matrix (no
     // cross-workgroup barrier; `members` and `bars` are unused): the
     // throughput form for many matrices.  Matrices of more than

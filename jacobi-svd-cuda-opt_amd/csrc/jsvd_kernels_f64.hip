@@ -445,21 +445,22 @@ namespace jsvd
         // the groups of one launch run independent solves side by side.  One
         // wave per column pair: lane l holds rows l + 64 j (j < RW) of both
         // columns in registers, the three dots are reduced across the wave
-        // (DPP and lane shuffles), so a round needs no workgroup barrier;
-        // eight waves per workgroup (an eighth of the barrier arrivals of one
-        // workgroup per pair; four or sixteen waves measured slower).  A wave
-        // takes pairs k, k + 8 members, ... of each round, so the arithmetic
-        // of a pair, and with it the result, does not depend on the group
-        // size.  Rotation parameters from the refined hardware estimates.
+        // (DPP and lane shuffles), so a round needs no workgroup barrier.
+        // WAVES waves per workgroup: 8 for one or two matrices (a lone solve
+        // is barrier-bound and eight-wave groups measured fastest), 16 for
+        // more (half the workgroups per group: fewer barrier arrivals and L2
+        // write-backs where many groups share the GPU).  A wave takes pairs
+        // k, k + WAVES members, ... of each round, so the arithmetic of a
+        // pair, and with it the result, does not depend on the group shape.
         // Same pairs and order as the reference circle.
-        template <int RW>
-        __global__ __launch_bounds__(512) void wave_group_sweeps_f64(WaveGroupArgs g)
+        template <int RW, int WAVES>
+        __global__ __launch_bounds__(64 * WAVES) void wave_group_sweeps_f64(WaveGroupArgs g)
         {
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
             const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
             const int groups = gridDim.x / g.members;
             unsigned int *bar = g.bars + 2 * group;
-            const int stride = g.members * 8;
+            const int stride = g.members * WAVES;
             for (int idx = group; idx < g.count; idx += groups)
             {
                 const WaveMatrix w = g.mats[idx];
@@ -475,7 +476,7 @@ namespace jsvd
                     int any = 0;
                     for (int r = 0; r < rounds; ++r)
                     {
-                        for (int k = member * 8 + wave; k < half; k += stride)
+                        for (int k = member * WAVES + wave; k < half; k += stride)
                         {
                             const int p = dev::column_at(k, r, n, even);
                             const int q = dev::column_at(even - 1 - k, r, n, even);
@@ -787,21 +788,27 @@ namespace jsvd
         return 64 * kWaveRowsMax;
     }
 
-    const void *wave_group_sweeps_f64_symbol(int64_t m)
+    template <int WAVES>
+    const void *wave_group_symbol(int rw)
     {
-        const int rw = static_cast<int>((m + 63) / 64);
         switch (rw)
         {
-        case 1: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<1>);
-        case 2: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<2>);
-        case 3: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<3>);
-        case 4: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<4>);
-        case 5: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<5>);
-        case 6: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<6>);
-        case 7: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<7>);
-        case 8: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<8>);
+        case 1: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<1, WAVES>);
+        case 2: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<2, WAVES>);
+        case 3: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<3, WAVES>);
+        case 4: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<4, WAVES>);
+        case 5: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<5, WAVES>);
+        case 6: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<6, WAVES>);
+        case 7: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<7, WAVES>);
+        case 8: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<8, WAVES>);
         default: return nullptr;
         }
+    }
+
+    const void *wave_group_sweeps_f64_symbol(int64_t m, int waves)
+    {
+        const int rw = static_cast<int>((m + 63) / 64);
+        return waves == 16 ? wave_group_symbol<16>(rw) : waves == 8 ? wave_group_symbol<8>(rw) : nullptr;
     }
 
     hipError_t launch_wave_block_sweeps_f64(int64_t m, int grid, const WaveGroupArgs &args, hipStream_t stream)
@@ -834,30 +841,17 @@ namespace jsvd
         return hipGetLastError();
     }
 
-    hipError_t launch_wave_group_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream)
+    hipError_t launch_wave_group_sweeps_f64(int64_t m, int waves, int groups, const WaveGroupArgs &args,
+                                            hipStream_t stream)
     {
         if (groups < 1 || args.members < 1 || args.count < 1)
             return hipErrorInvalidValue;
-        const int rw = static_cast<int>((m + 63) / 64);
+        const void *fn = wave_group_sweeps_f64_symbol(m, waves);
+        if (!fn)
+            return hipErrorInvalidValue;
         const unsigned int grid = static_cast<unsigned int>(groups) * static_cast<unsigned int>(args.members);
-#define JSVD_WAVE_CASE(RW)                                                                                         \
-    case RW:                                                                                                       \
-        wave_group_sweeps_f64<RW><<<grid, 512, 0, stream>>>(args);                                                 \
-        break;
-        switch (rw)
-        {
-            JSVD_WAVE_CASE(1)
-            JSVD_WAVE_CASE(2)
-            JSVD_WAVE_CASE(3)
-            JSVD_WAVE_CASE(4)
-            JSVD_WAVE_CASE(5)
-            JSVD_WAVE_CASE(6)
-            JSVD_WAVE_CASE(7)
-            JSVD_WAVE_CASE(8)
-        default: return hipErrorInvalidValue;
-        }
-#undef JSVD_WAVE_CASE
-        return hipGetLastError();
+        void *params[] = {const_cast<WaveGroupArgs *>(&args)};
+        return hipLaunchKernel(fn, dim3(grid), dim3(64 * waves), params, 0, stream);
     }
 
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,

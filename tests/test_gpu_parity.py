@@ -123,13 +123,14 @@ def test_grouped_wave_batch_against_oracle_and_single_solves():
 
 
 def test_tall_batch_streamed_against_oracle():
-    # More than 4 + pairs^2 / 1600 plain-path matrices taller than 512 rows
+    # More than 4 + pairs^2 / 400 plain-path matrices taller than 512 rows
     # run one workgroup per matrix with their rows streamed in 256-row
     # chunks (one launch); a single such matrix stays on the grid-barrier
     # kernel.  Uneven heights, odd and even widths.
     _note("tall batch, streamed rows")
     rng = np.random.default_rng(611)
-    shapes = [(1024, 64)] * 8 + [(700, 90), (513, 77), (2000, 40), (600, 129), (1500, 100), (777, 31)]
+    # 22 matrices, widest 129 columns (65 pairs): above the 14.6 of the rule.
+    shapes = [(1024, 64)] * 16 + [(700, 90), (513, 77), (2000, 40), (600, 129), (1500, 100), (777, 31)]
     mats = [rng.standard_normal(sh) for sh in shapes]
     got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
     ref = oracle.solve_batch(mats, 1e-13, 30)
@@ -139,10 +140,11 @@ def test_tall_batch_streamed_against_oracle():
 
 def test_grouped_wave_launch_with_narrow_groups():
     # 40 matrices of 512 x 512 stay in one grouped launch (below the
-    # 4 + 256^2 / 1600 matrices of the one-workgroup-per-matrix form), and 40
-    # groups of one wave per pair (32 workgroups each) exceed the at most
-    # 1024 resident workgroups of eight waves, so the groups get fewer
-    # workgroups and each wave takes several pairs per round.  Each matrix must equal its single solve bitwise; sigma is
+    # 4 + 256^2 / 400 matrices of the one-workgroup-per-matrix form), and 40
+    # groups of one wave per pair (16 workgroups of sixteen waves each)
+    # exceed the at most 512 resident workgroups of sixteen waves, so the
+    # groups get fewer workgroups and each wave takes several pairs per
+    # round.  Each matrix must equal its single solve bitwise; sigma is
     # checked against LAPACK (the oracle needs seconds per matrix here).
     # eps = 1e-14 (above the rounding floor of 512-row dots, about 2.5e-15):
     # at 1e-13, 512 columns each within eps of orthogonal can sum to
