@@ -106,7 +106,8 @@ typedef struct jsvd_plan_info
 {
     int64_t count;             /* matrices in the plan */
     int64_t lds_matrices;      /* solved by the LDS-resident work-list kernel */
-    int64_t large_matrices;    /* too large for LDS: grid-barrier kernel or blocked Jacobi */
+    int64_t large_matrices;    /* too large for LDS: wave-per-pair kernels (grouped barriers or one
+                                  workgroup per matrix), the grid-barrier kernel, or blocked Jacobi */
     int64_t a_elements;        /* packed A / U elements */
     int64_t v_elements;        /* packed V elements */
     int64_t sigma_elements;    /* packed sigma elements */
