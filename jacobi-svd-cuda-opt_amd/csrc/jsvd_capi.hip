@@ -444,17 +444,31 @@ namespace
 
     // A launch of `count` matrices whose widest has `max_half` column pairs
     // per round runs one workgroup per matrix when it has more than
-    // 4 + max_half^2 / 400 matrices of at most 512 columns: alone, a
+    // 4 + max_half^2 / 200 matrices of at most 512 columns: alone, a
     // sixteen-wave workgroup takes half / 16 pairs per wave and round, while
-    // a group's time grows with the number of groups that share the L2
-    // write-backs of their barriers.  Measured on MI355X (grouped form with
-    // sixteen-wave workgroups), the two forms cross at about 15 matrices of
-    // 160 x 150 and 48 of 256 x 256, and the grouped form still wins at 96
-    // of 512 x 512.  Wider matrices keep the barrier kernels whatever their
-    // count (one workgroup would take dozens of pairs per wave and round).
+    // the grouped form's rounds slow down as more groups share the GPU.
+    // Measured on MI355X (grouped form with sixteen-wave workgroups and flag
+    // barriers), the two forms cross at about 30 matrices of 160 x 150 and 90
+    // of 256 x 256, and tie at 96 of 512 x 512.  Wider matrices keep the
+    // barrier kernels whatever their count (one workgroup would take dozens
+    // of pairs per wave and round).
     bool wave_block_mode(int64_t count, int64_t max_half)
     {
-        return max_half <= 256 && count > 4 + max_half * max_half / 400;
+        return max_half <= 256 && count > 4 + max_half * max_half / 200;
+    }
+
+    // Plain-path matrices taller than the register kernels have no grouped
+    // form: the alternative is one matrix at a time on the grid-barrier
+    // kernel (about 7 us per round).  A workgroup of the streamed kernel
+    // takes ceil(half / 16) pairs per wave and round at about
+    // 2 ceil(m / 256) + 3 us per pair (measured: 23 us per round at
+    // 1024 x 64, 78 us at 600 x 300), so the streamed launch wins once the
+    // matrices outnumber that ratio.
+    bool tall_block_mode(int64_t count, int64_t max_half, int64_t m_max)
+    {
+        const int64_t per_wave = (max_half + 15) / 16;
+        const int64_t pair_us = 2 * ((m_max + 255) / 256) + 3;
+        return max_half <= 256 && 7 * count > per_wave * pair_us;
     }
 
     // Every grouped-wave matrix of the plan: row-major A into its
@@ -487,7 +501,12 @@ namespace
             if (l.members == 0)
                 JSVD_HIP(jsvd::launch_wave_block_sweeps_f64(l.m_max, l.count, args, stream));
             else
+            {
+                // The group barriers' flag words start every launch at zero.
+                JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.wv_bars.ptr), 0,
+                                               int64_t(l.groups) * l.members, stream));
                 JSVD_HIP(jsvd::launch_wave_group_sweeps_f64(l.m_max, l.waves, l.groups, args, stream));
+            }
         }
         for (int64_t e = 0; e < entries; ++e)
         {
@@ -540,7 +559,7 @@ namespace
             a_total += p.m[k] * p.n[k];
             v_total += p.n[k] * p.n[k];
         }
-        int max_groups = 1;
+        int64_t max_flags = 1;
         for (int64_t e = 0; e < entries;)
         {
             jsvd_plan::WaveLaunch l;
@@ -568,16 +587,16 @@ namespace
             const int64_t resident = int64_t(std::max(1, occupancy)) * cus;
             const int64_t need = (max_half + l.waves - 1) / l.waves;
             const int64_t members = std::max<int64_t>(1, std::min(need, resident / l.count));
-            l.members = static_cast<int>(std::min(members, resident));
+            // At most 64 members: one polling lane per member in the barrier.
+            l.members = static_cast<int>(std::min<int64_t>(std::min(members, resident), 64));
             l.groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(l.count, resident / l.members)));
-            max_groups = std::max(max_groups, l.groups);
+            max_flags = std::max<int64_t>(max_flags, int64_t(l.groups) * l.members);
             p.wave_launches.push_back(l);
         }
         JSVD_CHECK(p.wv_a.alloc(a_total));
         JSVD_CHECK(p.wv_v.alloc(v_total));
         JSVD_CHECK(p.wv_flags.alloc(entries * p.cfg.max_sweeps));
-        JSVD_CHECK(p.wv_bars.alloc(2 * max_groups));
-        JSVD_HIP(hipMemset(p.wv_bars.ptr, 0, sizeof(unsigned int) * 2 * max_groups));
+        JSVD_CHECK(p.wv_bars.alloc(max_flags));
         std::vector<jsvd::WaveMatrix> mats(entries);
         for (int64_t e = 0; e < entries; ++e)
         {
@@ -789,10 +808,13 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     // Plain-path matrices of more than wave_sweeps_max_rows() rows: many of
     // them run one workgroup per matrix (rows streamed) beside the other
     // wave-path matrices; a few run one at a time on the grid-barrier kernel.
-    int64_t tall_half = 1;
+    int64_t tall_half = 1, tall_m = 0;
     for (int64_t k : tall)
+    {
         tall_half = std::max<int64_t>(tall_half, (n[k] + 1) / 2);
-    if (wave_block_mode(static_cast<int64_t>(tall.size()), tall_half))
+        tall_m = std::max<int64_t>(tall_m, m[k]);
+    }
+    if (tall_block_mode(static_cast<int64_t>(tall.size()), tall_half, tall_m))
         plan->wave.insert(plan->wave.end(), tall.begin(), tall.end());
     else
         for (int64_t k : tall)

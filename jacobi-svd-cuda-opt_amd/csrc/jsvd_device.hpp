@@ -269,4 +269,35 @@ namespace jsvd::dev
         }
         __syncthreads();
     }
+    // Barrier among `members` (at most 64) co-resident workgroups through one
+    // flag word each: workgroup `member` publishes its barrier count `epoch`
+    // with a release fence and a relaxed store, then wave 0 polls every
+    // member's word at once (one lane per member, relaxed loads) until all
+    // have reached `epoch`, and acquires.  Two trips to the point of agent
+    // coherence per barrier (the store, the loads that see the last store)
+    // instead of the arrival RMW, the generation store and the poll of the
+    // counter barrier.  Flags start at zero and only grow within a launch.
+    __device__ __forceinline__ void flag_barrier(unsigned int *flags, int members, int member, unsigned int epoch)
+    {
+        __syncthreads();
+        if (threadIdx.x < 64)
+        {
+            if (threadIdx.x == 0)
+            {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __hip_atomic_store(&flags[member], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const int l = threadIdx.x;
+            for (;;)
+            {
+                const unsigned int v =
+                    l < members ? __hip_atomic_load(&flags[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : epoch;
+                if (__all(v >= epoch))
+                    break;
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        __syncthreads();
+    }
 } // namespace jsvd::dev

@@ -138,7 +138,7 @@ namespace jsvd
         int members;            // workgroups per group
         double eps;
         int max_sweeps;
-        unsigned int *bars;     // two words per group (arrivals, generation)
+        unsigned int *bars;     // one flag word per workgroup (members per group), zero at launch
         int32_t *sweeps;        // optional
     };
     int wave_sweeps_max_rows();

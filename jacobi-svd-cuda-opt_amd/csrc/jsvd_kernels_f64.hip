@@ -459,7 +459,8 @@ namespace jsvd
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
             const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
             const int groups = gridDim.x / g.members;
-            unsigned int *bar = g.bars + 2 * group;
+            unsigned int *bar = g.bars + group * g.members;
+            unsigned int epoch = 0;
             const int stride = g.members * WAVES;
             for (int idx = group; idx < g.count; idx += groups)
             {
@@ -486,7 +487,7 @@ namespace jsvd
                         }
                         if (r == rounds - 1 && any && lane == 0)
                             __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        dev::grid_barrier(bar, g.members);
+                        dev::flag_barrier(bar, g.members, member, ++epoch);
                     }
                     executed = sweep + 1;
                     const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

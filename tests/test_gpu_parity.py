@@ -123,13 +123,14 @@ def test_grouped_wave_batch_against_oracle_and_single_solves():
 
 
 def test_tall_batch_streamed_against_oracle():
-    # More than 4 + pairs^2 / 400 plain-path matrices taller than 512 rows
+    # Enough plain-path matrices taller than 512 rows
     # run one workgroup per matrix with their rows streamed in 256-row
     # chunks (one launch); a single such matrix stays on the grid-barrier
     # kernel.  Uneven heights, odd and even widths.
     _note("tall batch, streamed rows")
     rng = np.random.default_rng(611)
-    # 22 matrices, widest 129 columns (65 pairs): above the 14.6 of the rule.
+    # 22 matrices, widest 129 columns (65 pairs), tallest 2000 rows: above the
+    # 5 x 19 / 7 = 13.6 matrices from which the streamed launch is chosen.
     shapes = [(1024, 64)] * 16 + [(700, 90), (513, 77), (2000, 40), (600, 129), (1500, 100), (777, 31)]
     mats = [rng.standard_normal(sh) for sh in shapes]
     got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
@@ -140,7 +141,7 @@ def test_tall_batch_streamed_against_oracle():
 
 def test_grouped_wave_launch_with_narrow_groups():
     # 40 matrices of 512 x 512 stay in one grouped launch (below the
-    # 4 + 256^2 / 400 matrices of the one-workgroup-per-matrix form), and 40
+    # 4 + 256^2 / 200 matrices of the one-workgroup-per-matrix form), and 40
     # groups of one wave per pair (16 workgroups of sixteen waves each)
     # exceed the at most 512 resident workgroups of sixteen waves, so the
     # groups get fewer workgroups and each wave takes several pairs per
