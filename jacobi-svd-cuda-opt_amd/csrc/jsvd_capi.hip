@@ -457,18 +457,15 @@ namespace
         return max_half <= 256 && count > 4 + max_half * max_half / 200;
     }
 
-    // Plain-path matrices taller than the register kernels have no grouped
-    // form: the alternative is one matrix at a time on the grid-barrier
-    // kernel (about 7 us per round).  A workgroup of the streamed kernel
-    // takes ceil(half / 16) pairs per wave and round at about
-    // 2 ceil(m / 256) + 3 us per pair (measured: 23 us per round at
-    // 1024 x 64, 78 us at 600 x 300), so the streamed launch wins once the
-    // matrices outnumber that ratio.
-    bool tall_block_mode(int64_t count, int64_t max_half, int64_t m_max)
+    // The same choice for a launch of matrices taller than the register
+    // kernels (rows streamed in both forms): the streamed per-matrix
+    // workgroup is slower per matrix, so it pays off only once the matrices
+    // outnumber 64 + max_half^2 / 100.  Measured on MI355X, the forms cross at
+    // about 90 matrices of 1024 x 64 and 70 of 2000 x 100; at 128 of 600 x 300
+    // the grouped form still wins (354 vs 458 ms).
+    bool tall_block_mode(int64_t count, int64_t max_half)
     {
-        const int64_t per_wave = (max_half + 15) / 16;
-        const int64_t pair_us = 2 * ((m_max + 255) / 256) + 3;
-        return max_half <= 256 && 7 * count > per_wave * pair_us;
+        return max_half <= 256 && count > 64 + max_half * max_half / 100;
     }
 
     // Every grouped-wave matrix of the plan: row-major A into its
@@ -573,7 +570,8 @@ namespace
                 ++e;
             }
             l.count = static_cast<int>(e - l.first);
-            if (l.m_max > jsvd::wave_sweeps_max_rows() || wave_block_mode(l.count, max_half))
+            const bool streamed = l.m_max > jsvd::wave_sweeps_max_rows();
+            if (streamed ? tall_block_mode(l.count, max_half) : wave_block_mode(l.count, max_half))
             {
                 l.members = 0;
                 l.groups = l.count;
@@ -712,7 +710,6 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     std::vector<std::pair<GroupKey, std::vector<jsvd::WorkItem>>> groups;
     bool any_mixed = false;
     static const int kSlots[] = {32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
-    std::vector<int64_t> tall;
     bool any_four_wave = false;
     for (int64_t k = 0; k < count && !any_four_wave; ++k)
         any_four_wave = tuned_class(m[k], n[k]) < 0 && jsvd::select_class_f64(m[k], n[k]) == 4;
@@ -783,8 +780,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         }
         else if (!uses_blocked(cfg, n[k]) && m[k] <= jsvd::wave_sweeps_max_rows())
             plan->wave.push_back(k);
-        else if (!uses_blocked(cfg, n[k]))
-            tall.push_back(k);
+        else if (!uses_blocked(cfg, n[k]) && n[k] <= jsvd::wave_sweeps_max_rows())
+            plan->wave.push_back(k);
         else
         {
             plan->large.push_back(k);
@@ -805,25 +802,6 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             big_n = std::max(big_n, n[k]);
         }
     }
-    // Plain-path matrices of more than wave_sweeps_max_rows() rows: many of
-    // them run one workgroup per matrix (rows streamed) beside the other
-    // wave-path matrices; a few run one at a time on the grid-barrier kernel.
-    int64_t tall_half = 1, tall_m = 0;
-    for (int64_t k : tall)
-    {
-        tall_half = std::max<int64_t>(tall_half, (n[k] + 1) / 2);
-        tall_m = std::max<int64_t>(tall_m, m[k]);
-    }
-    if (tall_block_mode(static_cast<int64_t>(tall.size()), tall_half, tall_m))
-        plan->wave.insert(plan->wave.end(), tall.begin(), tall.end());
-    else
-        for (int64_t k : tall)
-        {
-            plan->large.push_back(k);
-            big_mn = std::max(big_mn, m[k] * n[k]);
-            big_nn = std::max(big_nn, n[k] * n[k]);
-            big_n = std::max(big_n, n[k]);
-        }
     plan->a_elems = ao;
     plan->v_elems = vo;
     plan->s_elems = so;

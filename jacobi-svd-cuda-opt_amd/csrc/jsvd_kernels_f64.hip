@@ -438,113 +438,6 @@ namespace jsvd
             return 1;
         }
 
-        // Matrices too large for LDS with at most 64 * RW rows, solved by
-        // groups of `members` workgroups: group g takes matrices g, g +
-        // groups, ... of the list (sorted longest first by the host), and its
-        // workgroups meet at a barrier of their own after every round, so
-        // the groups of one launch run independent solves side by side.  One
-        // wave per column pair: lane l holds rows l + 64 j (j < RW) of both
-        // columns in registers, the three dots are reduced across the wave
-        // (DPP and lane shuffles), so a round needs no workgroup barrier.
-        // WAVES waves per workgroup: 8 for one or two matrices (a lone solve
-        // is barrier-bound and eight-wave groups measured fastest), 16 for
-        // more (half the workgroups per group: fewer barrier arrivals and L2
-        // write-backs where many groups share the GPU).  A wave takes pairs
-        // k, k + WAVES members, ... of each round, so the arithmetic of a
-        // pair, and with it the result, does not depend on the group shape.
-        // Same pairs and order as the reference circle.
-        template <int RW, int WAVES>
-        __global__ __launch_bounds__(64 * WAVES) void wave_group_sweeps_f64(WaveGroupArgs g)
-        {
-            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
-            const int groups = gridDim.x / g.members;
-            unsigned int *bar = g.bars + group * g.members;
-            unsigned int epoch = 0;
-            const int stride = g.members * WAVES;
-            for (int idx = group; idx < g.count; idx += groups)
-            {
-                const WaveMatrix w = g.mats[idx];
-                double *__restrict__ A = w.a;
-                double *__restrict__ V = w.v;
-                const int64_t m = w.m;
-                const int n = w.n;
-                const int even = n + (n & 1), half = even >> 1;
-                const int rounds = n < 2 ? 0 : even - 1;
-                int executed = 0;
-                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
-                {
-                    int any = 0;
-                    for (int r = 0; r < rounds; ++r)
-                    {
-                        for (int k = member * WAVES + wave; k < half; k += stride)
-                        {
-                            const int p = dev::column_at(k, r, n, even);
-                            const int q = dev::column_at(even - 1 - k, r, n, even);
-                            if (p < 0 || q < 0)
-                                continue;
-                            any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
-                        }
-                        if (r == rounds - 1 && any && lane == 0)
-                            __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        dev::flag_barrier(bar, g.members, member, ++epoch);
-                    }
-                    executed = sweep + 1;
-                    const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (f == 0)
-                        break;
-                }
-                if (member == 0 && threadIdx.x == 0 && g.sweeps)
-                    g.sweeps[w.index] = executed;
-            }
-        }
-
-        // The same solves with one workgroup of sixteen waves per matrix:
-        // workgroup w takes matrices w, w + grid, ... of the list.  A and V
-        // stay in global memory (L2 / Infinity Cache), but only this
-        // workgroup touches them, so a round ends with a workgroup barrier
-        // alone (no cross-CU write-back or invalidate).  Wave k takes pairs
-        // k, k + 16, ... of a round; the pair arithmetic is the grouped
-        // kernel's, so results equal it bitwise.  Bound by the CU's L2
-        // bandwidth: a round of a 256 x 256 matrix moves 2 MB through one CU.
-        template <int RW>
-        __global__ __launch_bounds__(1024) void wave_block_sweeps_f64(WaveGroupArgs g)
-        {
-            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            for (int idx = blockIdx.x; idx < g.count; idx += gridDim.x)
-            {
-                const WaveMatrix w = g.mats[idx];
-                double *__restrict__ A = w.a;
-                double *__restrict__ V = w.v;
-                const int64_t m = w.m;
-                const int n = w.n;
-                const int even = n + (n & 1), half = even >> 1;
-                const int rounds = n < 2 ? 0 : even - 1;
-                int executed = 0;
-                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
-                {
-                    int any = 0;
-                    for (int r = 0; r < rounds; ++r)
-                    {
-                        for (int k = wave; k < half; k += 16)
-                        {
-                            const int p = dev::column_at(k, r, n, even);
-                            const int q = dev::column_at(even - 1 - k, r, n, even);
-                            if (p < 0 || q < 0)
-                                continue;
-                            any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
-                        }
-                        __syncthreads();
-                    }
-                    executed = sweep + 1;
-                    if (!__syncthreads_or(any))
-                        break;
-                }
-                if (threadIdx.x == 0 && g.sweeps)
-                    g.sweeps[w.index] = executed;
-            }
-        }
-
         // One column pair of any height by one wave, rows streamed in chunks
         // of 256 (4 per lane): the dots accumulate rows lane + 64 j in
         // increasing j, the same FMA chain as wave_pair_f64 (equal dots, so
@@ -609,6 +502,117 @@ namespace jsvd
                 vq[i] = s * x + c * y;
             }
             return 1;
+        }
+
+        // Matrices too large for LDS with at most 64 * RW rows (RW = 0: any
+        // height, rows streamed by wave_pair_stream_f64), solved by
+        // groups of `members` workgroups: group g takes matrices g, g +
+        // groups, ... of the list (sorted longest first by the host), and its
+        // workgroups meet at a barrier of their own after every round, so
+        // the groups of one launch run independent solves side by side.  One
+        // wave per column pair: lane l holds rows l + 64 j (j < RW) of both
+        // columns in registers, the three dots are reduced across the wave
+        // (DPP and lane shuffles), so a round needs no workgroup barrier.
+        // WAVES waves per workgroup: 8 for one or two matrices (a lone solve
+        // is barrier-bound and eight-wave groups measured fastest), 16 for
+        // more (half the workgroups per group: fewer barrier arrivals and L2
+        // write-backs where many groups share the GPU).  A wave takes pairs
+        // k, k + WAVES members, ... of each round, so the arithmetic of a
+        // pair, and with it the result, does not depend on the group shape.
+        // Same pairs and order as the reference circle.
+        template <int RW, int WAVES>
+        __global__ __launch_bounds__(64 * WAVES) void wave_group_sweeps_f64(WaveGroupArgs g)
+        {
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
+            const int groups = gridDim.x / g.members;
+            unsigned int *bar = g.bars + group * g.members;
+            unsigned int epoch = 0;
+            const int stride = g.members * WAVES;
+            for (int idx = group; idx < g.count; idx += groups)
+            {
+                const WaveMatrix w = g.mats[idx];
+                double *__restrict__ A = w.a;
+                double *__restrict__ V = w.v;
+                const int64_t m = w.m;
+                const int n = w.n;
+                const int even = n + (n & 1), half = even >> 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                int executed = 0;
+                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
+                {
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        for (int k = member * WAVES + wave; k < half; k += stride)
+                        {
+                            const int p = dev::column_at(k, r, n, even);
+                            const int q = dev::column_at(even - 1 - k, r, n, even);
+                            if (p < 0 || q < 0)
+                                continue;
+                            if constexpr (RW == 0)
+                                any |= wave_pair_stream_f64(A, V, m, n, p, q, lane, g.eps);
+                            else
+                                any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
+                        }
+                        if (r == rounds - 1 && any && lane == 0)
+                            __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        dev::flag_barrier(bar, g.members, member, ++epoch);
+                    }
+                    executed = sweep + 1;
+                    const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (f == 0)
+                        break;
+                }
+                if (member == 0 && threadIdx.x == 0 && g.sweeps)
+                    g.sweeps[w.index] = executed;
+            }
+        }
+
+        // The same solves with one workgroup of sixteen waves per matrix:
+        // workgroup w takes matrices w, w + grid, ... of the list.  A and V
+        // stay in global memory (L2 / Infinity Cache), but only this
+        // workgroup touches them, so a round ends with a workgroup barrier
+        // alone (no cross-CU write-back or invalidate).  Wave k takes pairs
+        // k, k + 16, ... of a round; the pair arithmetic is the grouped
+        // kernel's, so results equal it bitwise.  Bound by the CU's L2
+        // bandwidth: a round of a 256 x 256 matrix moves 2 MB through one CU.
+        template <int RW>
+        __global__ __launch_bounds__(1024) void wave_block_sweeps_f64(WaveGroupArgs g)
+        {
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            for (int idx = blockIdx.x; idx < g.count; idx += gridDim.x)
+            {
+                const WaveMatrix w = g.mats[idx];
+                double *__restrict__ A = w.a;
+                double *__restrict__ V = w.v;
+                const int64_t m = w.m;
+                const int n = w.n;
+                const int even = n + (n & 1), half = even >> 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                int executed = 0;
+                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
+                {
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        for (int k = wave; k < half; k += 16)
+                        {
+                            const int p = dev::column_at(k, r, n, even);
+                            const int q = dev::column_at(even - 1 - k, r, n, even);
+                            if (p < 0 || q < 0)
+                                continue;
+                            any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
+                        }
+                        __syncthreads();
+                    }
+                    executed = sweep + 1;
+                    if (!__syncthreads_or(any))
+                        break;
+                }
+                if (threadIdx.x == 0 && g.sweeps)
+                    g.sweeps[w.index] = executed;
+            }
         }
 
         // wave_block_sweeps_f64 for matrices of any height: one workgroup of
@@ -794,6 +798,7 @@ namespace jsvd
     {
         switch (rw)
         {
+        case 0: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<0, WAVES>);
         case 1: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<1, WAVES>);
         case 2: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<2, WAVES>);
         case 3: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<3, WAVES>);
@@ -808,7 +813,8 @@ namespace jsvd
 
     const void *wave_group_sweeps_f64_symbol(int64_t m, int waves)
     {
-        const int rw = static_cast<int>((m + 63) / 64);
+        // Taller than the register instances: rows streamed (RW = 0).
+        const int rw = m > 64 * kWaveRowsMax ? 0 : static_cast<int>((m + 63) / 64);
         return waves == 16 ? wave_group_symbol<16>(rw) : waves == 8 ? wave_group_symbol<8>(rw) : nullptr;
     }
 
