@@ -136,7 +136,13 @@ namespace
         s.pairs = s.nb / 2;
         s.lda = (m + 63) / 64 * 64;
         const int64_t panels = s.lda / 64;
-        const int64_t chunks = std::clamp<int64_t>((512 + s.pairs - 1) / s.pairs, 1, panels);
+        // Row chunks per block pair for the partial Grams: about 512 Gram
+        // workgroups at block width 16 and 256 at 32 (more chunks keep more
+        // workgroups streaming, but the Gram solve sums every chunk; measured
+        // on 8192 x 2048: 11.7 / 12.5 ms per sweep at 512 / 256 for width 16,
+        // 9.5 / 9.3 for width 32).
+        const int64_t target = block == 32 ? 256 : 512;
+        const int64_t chunks = std::clamp<int64_t>((target + s.pairs - 1) / s.pairs, 1, panels);
         s.chunk_rows = ((panels + chunks - 1) / chunks) * 64;
         s.chunks = static_cast<int>((s.lda + s.chunk_rows - 1) / s.chunk_rows);
         return s;
