@@ -2,8 +2,10 @@
 // counterpart: the reference is FP64 only).
 //
 // Per matrix, four work-list launches on one stream:
-//  1. mixed_sweeps_f32: FP32 sweeps with A and V resident in LDS.  Dots and
-//     column updates run in FP32 with FP32 accumulation; the rotation
+//  1. mixed_sweeps_compact_f32: FP32 sweeps with A and V in the compact
+//     LDS layout of the FP64 class kernel (top columns held in registers,
+//     half + 2 bottom slots in LDS: two 128 x 128 matrices per CU).  Dots
+//     and column updates run in FP32 with FP32 accumulation; the rotation
 //     parameters (zeta, t, c, s) and the off-diagonal test are computed in
 //     FP64 from the FP32 dots.  The FP32 phase stops at the FP32 rounding
 //     floor, tol32 = max(eps, 8 sqrt(m) 2^-24).  V32 is written out as FP64.
@@ -58,31 +60,42 @@ namespace jsvd
         constexpr F32Spec kF32Classes[] = {
             {64, 4, 4, 2},     // n <= 32, m <= 64
             {256, 8, 4, 2},    // n <= 64, m <= 128
-            {1024, 16, 2, 2},  // n <= 128, m <= 128
+            {512, 8, 4, 4},    // n <= 128, m <= 128
             {512, 16, 4, 4},   // n <= 256 (two passes over 128), m <= 256
         };
         constexpr int kF32Count = sizeof(kF32Classes) / sizeof(kF32Classes[0]);
+
+        // Pairs per lane group and round of an FP32 class: its widest matrix
+        // (4 G vmax4 columns, so half = 2 G vmax4 pairs) over its BS / G
+        // groups, one or two.
+        __host__ __device__ constexpr int f32_passes(int block, int group, int vmax4)
+        {
+            return (2 * group * vmax4 + block / group - 1) / (block / group) < 2 ? 1 : 2;
+        }
 
         __host__ __device__ __forceinline__ int padded4(int rows, int G)
         {
             return ((rows + 4 * G - 1) / (4 * G)) * (4 * G);
         }
 
+        // One column pair in FP32 with its top column held in registers (xp
+        // for A, yp for V, updated in place) and its bottom column in LDS (aq,
+        // vq): FP32 dots (FP32 accumulation) and updates; the off-diagonal
+        // test and the rotation parameters (zeta, t, c, s) in FP64 from the
+        // FP32 dots.
         template <int G, int RMAX4, int VMAX4>
-        __device__ __forceinline__ int pair_f32(float4 *__restrict__ ap, float4 *__restrict__ aq,
-                                                float4 *__restrict__ vp, float4 *__restrict__ vq, int rows4,
-                                                int vrows4, int lane, double tol)
+        __device__ __forceinline__ int pair_f32_held(float4 (&xp)[RMAX4], float4 (&yp)[VMAX4],
+                                                     float4 *__restrict__ aq, float4 *__restrict__ vq, int rows4,
+                                                     int vrows4, int lane, double tol)
         {
-            float4 xp[RMAX4], xq[RMAX4];
+            float4 xq[RMAX4];
             float pp = 0.0f, qq = 0.0f, pq = 0.0f;
 #pragma unroll
             for (int j = 0; j < RMAX4; ++j)
             {
                 if (j < rows4)
                 {
-                    const int i = lane + j * G;
-                    xp[j] = ap[i];
-                    xq[j] = aq[i];
+                    xq[j] = aq[lane + j * G];
                     pp += xp[j].x * xp[j].x + xp[j].y * xp[j].y + xp[j].z * xp[j].z + xp[j].w * xp[j].w;
                     qq += xq[j].x * xq[j].x + xq[j].y * xq[j].y + xq[j].z * xq[j].z + xq[j].w * xq[j].w;
                     pq += xp[j].x * xq[j].x + xp[j].y * xq[j].y + xp[j].z * xq[j].z + xp[j].w * xq[j].w;
@@ -91,26 +104,19 @@ namespace jsvd
             const double dpp = group_sum_f32<G>(pp);
             const double dqq = group_sum_f32<G>(qq);
             const double dpq = group_sum_f32<G>(pq);
-            // FP64 off-diagonal test and rotation parameters.
             const double rhs = tol * sqrt(fmax(dpp, 0.0) * fmax(dqq, 0.0));
             const double apq = fabs(dpq);
             if (!(apq > rhs && apq > 1.0e-300))
                 return 0;
-            float4 yp[VMAX4], yq[VMAX4];
+            float4 yq[VMAX4];
 #pragma unroll
             for (int j = 0; j < VMAX4; ++j)
             {
                 if (j < vrows4)
-                {
-                    const int i = lane + j * G;
-                    yp[j] = vp[i];
-                    yq[j] = vq[i];
-                }
+                    yq[j] = vq[lane + j * G];
             }
             const double zeta = (dqq - dpp) / (2.0 * dpq);
             const double root = sqrt(1.0 + zeta * zeta);
-            // One division: 1 / (|zeta| + root) equals both branches of the
-            // sign-split form bit for bit (also for zeta = -0, which takes +).
             const double tmag = 1.0 / (fabs(zeta) + root);
             const double t = (zeta >= 0.0) ? tmag : -tmag;
             const double cd = 1.0 / sqrt(1.0 + t * t);
@@ -123,7 +129,6 @@ namespace jsvd
                 {
                     if (j < rows4)
                     {
-                        const int i = lane + j * G;
                         float4 np, nq;
                         np.x = c * xp[j].x - s * xq[j].x;
                         np.y = c * xp[j].y - s * xq[j].y;
@@ -133,8 +138,8 @@ namespace jsvd
                         nq.y = s * xp[j].y + c * xq[j].y;
                         nq.z = s * xp[j].z + c * xq[j].z;
                         nq.w = s * xp[j].w + c * xq[j].w;
-                        ap[i] = np;
-                        aq[i] = nq;
+                        xp[j] = np;
+                        aq[lane + j * G] = nq;
                     }
                 }
 #pragma unroll
@@ -142,7 +147,6 @@ namespace jsvd
                 {
                     if (j < vrows4)
                     {
-                        const int i = lane + j * G;
                         float4 np, nq;
                         np.x = c * yp[j].x - s * yq[j].x;
                         np.y = c * yp[j].y - s * yq[j].y;
@@ -152,24 +156,33 @@ namespace jsvd
                         nq.y = s * yp[j].y + c * yq[j].y;
                         nq.z = s * yp[j].z + c * yq[j].z;
                         nq.w = s * yp[j].w + c * yq[j].w;
-                        vp[i] = np;
-                        vq[i] = nq;
+                        yp[j] = np;
+                        vq[lane + j * G] = nq;
                     }
                 }
             }
             return 1;
         }
 
+        // FP32 sweeps in the compact layout of the FP64 class kernel: each
+        // lane group holds the top column (A and V) of its circle position in
+        // registers and follows it along the ring top[1] -> ... -> top[half-1]
+        // -> bottom[half-1] -> ... -> bottom[0] -> top[1]; LDS holds only the
+        // bottom row, half + 2 slots of A and of V, the bottom column at
+        // position P in global round g living in slot (g + P) mod (half + 2).
+        // Same pairs in the same order as the reference circle, and half the
+        // LDS of holding all columns (two 128 x 128 matrices per CU).
         template <int BS, int G, int RMAX4, int VMAX4>
-        __global__ __launch_bounds__(BS) void mixed_sweeps_f32(MixedArgs args)
+        __global__ __launch_bounds__(BS) void mixed_sweeps_compact_f32(MixedArgs args)
         {
-            // float4 keeps the dynamic LDS block 16-byte aligned for the
-            // row-quad accesses.
             extern __shared__ float4 lds_quads[];
             float *lds32 = reinterpret_cast<float *>(lds_quads);
             __shared__ int s_item;
             constexpr int GROUPS = BS / G;
-            const int tid = threadIdx.x;
+            // One pass (every class up to 128 x 128) holds RMAX4 + VMAX4
+            // quads of top columns in registers.
+            constexpr int PASSES = f32_passes(BS, G, VMAX4);
+            const int tid = threadIdx.x, gid = tid / G, lane = tid & (G - 1);
             for (;;)
             {
                 if (tid == 0)
@@ -181,56 +194,165 @@ namespace jsvd
                 const WorkItem w = args.items[item];
                 const int m = w.m, n = w.n;
                 const int lda = padded4(m, G), ldv = padded4(n, G);
-                float *A = lds32;
-                float *V = lds32 + lda * n;
+                const int even = n + (n & 1), half = even >> 1, ring = even - 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                const int ns = half + 2;
+                float *As = lds32;          // ns slots of lda rows
+                float *Vs = As + ns * lda;  // ns slots of ldv rows
+                float4 *A4 = reinterpret_cast<float4 *>(As);
+                float4 *V4 = reinterpret_cast<float4 *>(Vs);
+                const int lda4 = lda / 4, ldv4 = ldv / 4;
+                const int rows4 = lda / (4 * G), vrows4 = ldv / (4 * G);
                 const double *ag = args.a + w.a_off;
-                for (int idx = tid; idx < lda * n; idx += BS)
+                // Slots 0 .. half-1 <- columns col(0 .. half-1) of A and of the
+                // identity (zeros for the dummy column and padding).
+                auto fill = [&](auto col) {
+                    for (int idx = tid; idx < half * lda; idx += BS)
+                    {
+                        const int sl = idx / lda, i = idx - sl * lda, j = col(sl);
+                        As[idx] = (i < m && j < n) ? static_cast<float>(ag[i * n + j]) : 0.0f;
+                    }
+                    for (int idx = tid; idx < half * ldv; idx += BS)
+                    {
+                        const int sl = idx / ldv, i = idx - sl * ldv, j = col(sl);
+                        Vs[idx] = (i == j && j < n) ? 1.0f : 0.0f;
+                    }
+                };
+                int pos[PASSES], ct[PASSES], cb[PASSES], fresh[PASSES];
+                float4 ta[PASSES][RMAX4];
+                float4 tv[PASSES][VMAX4];
+                auto load_top = [&](int i, int sl) {
+#pragma unroll
+                    for (int j = 0; j < RMAX4; ++j)
+                        if (j < rows4)
+                            ta[i][j] = A4[sl * lda4 + lane + j * G];
+#pragma unroll
+                    for (int j = 0; j < VMAX4; ++j)
+                        if (j < vrows4)
+                            tv[i][j] = V4[sl * ldv4 + lane + j * G];
+                };
+                // The top row (position k holds column k) is staged through
+                // slots 0 .. half-1 into registers, then the bottom row
+                // (position P holds column even-1-P) fills slots 0 .. half-1.
+                fill([](int sl) { return sl; });
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < PASSES; ++i)
                 {
-                    const int j = idx / lda, i = idx - j * lda;
-                    A[idx] = i < m ? static_cast<float>(ag[i * n + j]) : 0.0f;
-                }
-                for (int idx = tid; idx < ldv * n; idx += BS)
-                {
-                    const int j = idx / ldv, i = idx - j * ldv;
-                    V[idx] = (i == j) ? 1.0f : 0.0f;
+                    const int k = gid + i * GROUPS;
+                    pos[i] = k;
+                    ct[i] = k;
+                    cb[i] = even - 1 - k;
+                    fresh[i] = 0;
+                    if (k < half && k < n)
+                        load_top(i, k);
                 }
                 __syncthreads();
+                fill([even](int sl) { return even - 1 - sl; });
+                __syncthreads();
                 const double tol = fmax(args.eps, 8.0 * sqrt(static_cast<double>(m)) * 5.9604644775390625e-08);
-                const int even = n + (n & 1);
-                const int half = even >> 1;
-                const int rounds = n < 2 ? 0 : even - 1;
-                const int gid = tid / G, lane = tid & (G - 1);
-                const int rows4 = lda / (4 * G), vrows4 = ldv / (4 * G);
-                float4 *A4 = reinterpret_cast<float4 *>(A);
-                float4 *V4 = reinterpret_cast<float4 *>(V);
-                const int lda4 = lda / 4, ldv4 = ldv / 4;
+                int z = even - 1; // bottom[0]'s column this round
+                int gs = 0;       // global round mod ns
                 int executed = 0;
                 for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
                 {
                     int any = 0;
                     for (int r = 0; r < rounds; ++r)
                     {
-                        for (int k = gid; k < half; k += GROUPS)
+                        const int z1 = z - 1 < 1 ? ring : z - 1;
+                        const int z2 = z - 2 < 1 ? z - 2 + ring : z - 2;
+                        const int ls = gs == 0 ? ns - 1 : gs - 1;     // fresh top loads
+                        const int ws = gs < 2 ? gs + ns - 2 : gs - 2; // wrap stores
+#pragma unroll
+                        for (int i = 0; i < PASSES; ++i)
                         {
-                            const int p = dev::column_at(k, r, n, even);
-                            const int q = dev::column_at(even - 1 - k, r, n, even);
-                            if (p < 0 || q < 0)
-                                continue;
-                            any |= pair_f32<G, RMAX4, VMAX4>(A4 + p * lda4, A4 + q * lda4, V4 + p * ldv4,
-                                                             V4 + q * ldv4, rows4, vrows4, lane, tol);
+                            const int k = gid + i * GROUPS;
+                            if (k < half)
+                            {
+                                const int p = ct[i], q = cb[i];
+                                if (fresh[i] & (p < n))
+                                    load_top(i, ls);
+                                int bsl = gs + pos[i];
+                                bsl = half == 1 ? 0 : (bsl >= ns ? bsl - ns : bsl);
+                                int rot = 0;
+                                if ((p < n) & (q < n))
+                                    rot = pair_f32_held<G, RMAX4, VMAX4>(ta[i], tv[i], A4 + bsl * lda4,
+                                                                         V4 + bsl * ldv4, rows4, vrows4, lane, tol);
+                                any |= rot;
+                                // Leaving position half-1: the top column becomes
+                                // bottom[half-1] in slot ws; the group restarts at
+                                // position 1 with the previous bottom[0].
+                                const int wrap = (k > 0) & (pos[i] == half - 1);
+                                if (wrap & (p < n))
+                                {
+#pragma unroll
+                                    for (int j = 0; j < RMAX4; ++j)
+                                        if (j < rows4)
+                                            A4[ws * lda4 + lane + j * G] = ta[i][j];
+#pragma unroll
+                                    for (int j = 0; j < VMAX4; ++j)
+                                        if (j < vrows4)
+                                            V4[ws * ldv4 + lane + j * G] = tv[i][j];
+                                }
+                                const int q1 = q - 1 < 1 ? ring : q - 1;
+                                const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
+                                cb[i] = k == 0 ? q1 : (wrap ? z2 : q2);
+                                ct[i] = wrap ? z : p;
+                                pos[i] = wrap ? 1 : (k > 0 ? pos[i] + 1 : 0);
+                                fresh[i] = wrap;
+                            }
                         }
+                        z = z1;
+                        gs = gs + 1 == ns ? 0 : gs + 1;
                         __syncthreads();
                     }
                     executed = sweep + 1;
                     if (!__syncthreads_or(any))
                         break;
                 }
-                // V32 -> packed column-major FP64 scratch.
+                // V32 -> packed column-major FP64 scratch: every group writes
+                // its held top column (loading it first if it is still in its
+                // slot) and its bottom column from the slot of its position.
                 double *qg = args.q + w.v_off;
-                for (int idx = tid; idx < n * n; idx += BS)
+                const int ls = gs == 0 ? ns - 1 : gs - 1;
+#pragma unroll
+                for (int i = 0; i < PASSES; ++i)
                 {
-                    const int j = idx / n, i = idx - j * n;
-                    qg[idx] = static_cast<double>(V[j * ldv + i]);
+                    const int k = gid + i * GROUPS;
+                    if (k < half)
+                    {
+                        if (fresh[i] && ct[i] < n)
+                            load_top(i, ls);
+                        int bsl = gs + pos[i];
+                        bsl = half == 1 ? 0 : (bsl >= ns ? bsl - ns : bsl);
+                        if (ct[i] < n)
+                        {
+#pragma unroll
+                            for (int j = 0; j < VMAX4; ++j)
+                            {
+                                const int r0 = 4 * (lane + j * G);
+                                if (j < vrows4)
+                                {
+                                    const float4 x = tv[i][j];
+                                    double *col = qg + (int64_t)ct[i] * n;
+                                    if (r0 < n)
+                                        col[r0] = x.x;
+                                    if (r0 + 1 < n)
+                                        col[r0 + 1] = x.y;
+                                    if (r0 + 2 < n)
+                                        col[r0 + 2] = x.z;
+                                    if (r0 + 3 < n)
+                                        col[r0 + 3] = x.w;
+                                }
+                            }
+                        }
+                        if (cb[i] < n)
+                        {
+                            double *col = qg + (int64_t)cb[i] * n;
+                            for (int row = lane; row < n; row += G)
+                                col[row] = static_cast<double>(Vs[bsl * ldv + row]);
+                        }
+                    }
                 }
                 if (tid == 0 && args.sweeps)
                     args.sweeps[w.index] = executed;
@@ -385,13 +507,13 @@ namespace jsvd
         const void *f32_symbol_t()
         {
             constexpr F32Spec c = kF32Classes[I];
-            return reinterpret_cast<const void *>(&mixed_sweeps_f32<c.block, c.group, c.rmax4, c.vmax4>);
+            return reinterpret_cast<const void *>(&mixed_sweeps_compact_f32<c.block, c.group, c.rmax4, c.vmax4>);
         }
         template <int I>
         void f32_launch_t(int grid, int lds_bytes, const MixedArgs &args, hipStream_t stream)
         {
             constexpr F32Spec c = kF32Classes[I];
-            mixed_sweeps_f32<c.block, c.group, c.rmax4, c.vmax4><<<grid, c.block, lds_bytes, stream>>>(args);
+            mixed_sweeps_compact_f32<c.block, c.group, c.rmax4, c.vmax4><<<grid, c.block, lds_bytes, stream>>>(args);
         }
         using F32Launch = void (*)(int, int, const MixedArgs &, hipStream_t);
         const F32Launch kF32Launch[] = {f32_launch_t<0>, f32_launch_t<1>, f32_launch_t<2>, f32_launch_t<3>};
@@ -404,8 +526,10 @@ namespace jsvd
 
     int64_t f32_class_lds_bytes(int cls, int64_t m, int64_t n)
     {
+        // half + 2 slots of A and of V (compact layout).
         const int g = kF32Classes[cls].group;
-        return (padded4(static_cast<int>(m), g) * n + padded4(static_cast<int>(n), g) * n) * 4;
+        const int64_t slots = (n + 1) / 2 + 2;
+        return (padded4(static_cast<int>(m), g) + padded4(static_cast<int>(n), g)) * slots * 4;
     }
 
     bool f32_class_fits(int cls, int64_t m, int64_t n)
@@ -415,7 +539,7 @@ namespace jsvd
         const F32Spec &c = kF32Classes[cls];
         const int64_t half = (n + 1) / 2;
         const int64_t groups = c.block / c.group;
-        const int64_t passes = c.block == 64 ? 1 : 2;
+        const int64_t passes = f32_passes(c.block, c.group, c.vmax4);
         if (half > passes * groups)
             return false;
         if (padded4(static_cast<int>(m), c.group) / (4 * c.group) > c.rmax4 ||
