@@ -370,14 +370,16 @@ namespace jsvd
         }
 
         // Modified Gram-Schmidt of the n columns of Q (column-major, n x n) in
-        // FP64, in LDS.  Column j is normalised, then projected out of every
-        // later column by lane groups of 16.
+        // FP64, in LDS, with deferred normalisation: step j projects the unit
+        // vector q_j / ||q_j|| out of every later column (lane groups of 16,
+        // one column each at a time), and the group that updates column j + 1
+        // also computes its norm for the next step, so a step needs a single
+        // workgroup barrier; every column is scaled to unit length at the end.
         template <int BS>
         __global__ __launch_bounds__(BS) void orthonormalize_f64(MixedArgs args)
         {
             extern __shared__ double ldsq[];
             __shared__ int s_item;
-            __shared__ double red[BS / 64];
             constexpr int G = 16, GROUPS = BS / G;
             const int tid = threadIdx.x, lane = tid & (G - 1), gid = tid / G;
             for (;;)
@@ -391,43 +393,50 @@ namespace jsvd
                 const WorkItem w = args.items[item];
                 const int n = w.n;
                 double *Q = ldsq;
+                double *inv = ldsq + n * n; // 1 / ||q_j|| of the projected q_j
                 double *qg = args.q + w.v_off;
                 for (int idx = tid; idx < n * n; idx += BS)
                     Q[idx] = qg[idx];
                 __syncthreads();
+                if (gid == 0)
+                {
+                    double d = 0.0;
+                    for (int i = lane; i < n; i += G)
+                        d += Q[i] * Q[i];
+                    d = dev::group_sum_dpp<G>(d);
+                    if (lane == 0)
+                        inv[0] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+                }
+                __syncthreads();
                 for (int j = 0; j < n; ++j)
                 {
-                    double *qj = Q + j * n;
-                    double part = 0.0;
-                    for (int i = tid; i < n; i += BS)
-                        part += qj[i] * qj[i];
-                    part = dev::wave_sum(part);
-                    if ((tid & 63) == 0)
-                        red[tid >> 6] = part;
-                    __syncthreads();
-                    double nrm2 = red[0];
-#pragma unroll
-                    for (int wv = 1; wv < BS / 64; ++wv)
-                        nrm2 += red[wv];
-                    const double inv = nrm2 > 0.0 ? 1.0 / sqrt(nrm2) : 0.0;
-                    __syncthreads();
-                    for (int i = tid; i < n; i += BS)
-                        qj[i] *= inv;
-                    __syncthreads();
+                    const double *qj = Q + j * n;
+                    const double ij2 = inv[j] * inv[j];
                     for (int k = j + 1 + gid; k < n; k += GROUPS)
                     {
                         double *qk = Q + k * n;
                         double d = 0.0;
                         for (int i = lane; i < n; i += G)
                             d += qj[i] * qk[i];
-                        d = dev::group_sum_dpp<G>(d);
+                        d = dev::group_sum_dpp<G>(d) * ij2;
+                        double nk = 0.0;
                         for (int i = lane; i < n; i += G)
-                            qk[i] -= d * qj[i];
+                        {
+                            const double x = qk[i] - d * qj[i];
+                            qk[i] = x;
+                            nk += x * x;
+                        }
+                        if (k == j + 1)
+                        {
+                            nk = dev::group_sum_dpp<G>(nk);
+                            if (lane == 0)
+                                inv[k] = nk > 0.0 ? 1.0 / sqrt(nk) : 0.0;
+                        }
                     }
                     __syncthreads();
                 }
                 for (int idx = tid; idx < n * n; idx += BS)
-                    qg[idx] = Q[idx];
+                    qg[idx] = Q[idx] * inv[idx / n];
                 __syncthreads();
             }
             if (tid == 0)
@@ -521,7 +530,7 @@ namespace jsvd
         static_assert(sizeof(kF32Launch) / sizeof(kF32Launch[0]) == kF32Count);
     } // namespace
 
-    constexpr int kOrthoBlock = 256;
+    constexpr int kOrthoBlock = 512;
     constexpr int kGemmBlock = 256;
 
     int64_t f32_class_lds_bytes(int cls, int64_t m, int64_t n)
@@ -576,7 +585,7 @@ namespace jsvd
 
     int64_t orthonormalize_lds_bytes(int64_t n)
     {
-        return n * n * 8;
+        return (n * n + n) * 8;
     }
 
     const void *orthonormalize_symbol()
