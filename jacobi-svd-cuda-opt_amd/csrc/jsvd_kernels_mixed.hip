@@ -104,9 +104,7 @@ namespace jsvd
             const double dpp = group_sum_f32<G>(pp);
             const double dqq = group_sum_f32<G>(qq);
             const double dpq = group_sum_f32<G>(pq);
-            const double rhs = tol * sqrt(fmax(dpp, 0.0) * fmax(dqq, 0.0));
-            const double apq = fabs(dpq);
-            if (!(apq > rhs && apq > 1.0e-300))
+            if (!dev::rotation_needed(dpp, dqq, dpq, tol))
                 return 0;
             float4 yq[VMAX4];
 #pragma unroll
@@ -115,12 +113,11 @@ namespace jsvd
                 if (j < vrows4)
                     yq[j] = vq[lane + j * G];
             }
-            const double zeta = (dqq - dpp) / (2.0 * dpq);
-            const double root = sqrt(1.0 + zeta * zeta);
-            const double tmag = 1.0 / (fabs(zeta) + root);
-            const double t = (zeta >= 0.0) ? tmag : -tmag;
-            const double cd = 1.0 / sqrt(1.0 + t * t);
-            const double sd = t * cd;
+            // FP64 parameters from the refined hardware estimates (the FP64
+            // kernels' evaluation: about half the dependent operations of
+            // IEEE division and square root).
+            double cd, sd;
+            dev::rotation_from_tangent(dev::rotation_tangent(dpp, dqq, dpq), cd, sd);
             const float c = static_cast<float>(cd), s = static_cast<float>(sd);
             if (s != 0.0f)
             {
