@@ -448,6 +448,13 @@ namespace
         return finish_large(p, k, m, n, m, n, d_u, d_sigma, d_v, stream);
     }
 
+    // Plain-path matrices too large for LDS take the wave kernels up to this
+    // many columns: a group of at most 64 sixteen-wave workgroups (one flag
+    // word each) holds one wave per pair of 2048 columns.  Measured per sweep
+    // against one workgroup per pair with a grid barrier: 8192 x 2048 247 vs
+    // 374 ms, 2048 x 1024 26 vs 68 ms, 1100 x 520 6.3 vs 11.4 ms.
+    constexpr int64_t kWaveMaxColumns = 2048;
+
     // A launch of `count` matrices whose widest has `max_half` column pairs
     // per round runs one workgroup per matrix when it has more than
     // 4 + max_half^2 / 200 matrices of at most 512 columns: alone, a
@@ -584,7 +591,9 @@ namespace
                 p.wave_launches.push_back(l);
                 continue;
             }
-            l.waves = l.count <= 2 ? 8 : 16;
+            // Eight-wave workgroups for one or two matrices whose group fits the
+            // 64 flag words of a barrier; sixteen otherwise.
+            l.waves = (l.count <= 2 && (max_half + 7) / 8 <= 64) ? 8 : 16;
             int occupancy = 0;
             JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max, l.waves), 64 * l.waves, 0));
@@ -786,7 +795,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         }
         else if (!uses_blocked(cfg, n[k]) && m[k] <= jsvd::wave_sweeps_max_rows())
             plan->wave.push_back(k);
-        else if (!uses_blocked(cfg, n[k]) && n[k] <= jsvd::wave_sweeps_max_rows())
+        else if (!uses_blocked(cfg, n[k]) && n[k] <= kWaveMaxColumns)
             plan->wave.push_back(k);
         else
         {

@@ -83,11 +83,10 @@ def test_lds_path_against_oracle(shape):
                                            ((5000, 40), 256), ((1024, 129), 1024), ((1100, 520), 256),
                                            ((600, 513), 1024)])
 def test_persistent_path_against_oracle(shape, threads):
-    # These shapes exceed the LDS budget, so they run the barrier kernels:
-    # the grouped wave kernel for up to 512 columns (rows in registers up to
-    # 512 rows: 256 x 256, 300 x 15x; streamed beyond: 5000 x 40, 1024 x 129),
-    # one workgroup per pair with a grid barrier for wider matrices
-    # (1100 x 520, 600 x 513).
+    # These shapes exceed the LDS budget, so they run the grouped wave kernel
+    # (rows in registers up to 512 rows: 256 x 256, 300 x 15x; streamed
+    # beyond: 5000 x 40, 1024 x 129, 1100 x 520, 600 x 513 with sixteen-wave
+    # workgroups).
     # For the two wide Gaussian inputs the final column order is chaotic
     # under rounding: a one-ulp perturbation of the 1100 x 520 input moves
     # about 480 of its 520 sigma positions in the oracle itself (sorted sigma
@@ -186,6 +185,23 @@ def test_grouped_wave_launch_with_narrow_groups():
         for x, y in zip(single[:3], got[k][:3]):
             np.testing.assert_array_equal(x, y)
         assert single[3] == got[k][3]
+
+
+def test_widest_plain_path_on_grid_barrier_kernel():
+    # More than 2048 columns: one workgroup per pair with a grid barrier
+    # (persistent_sweeps_f64).  The oracle needs minutes here, so sigma is
+    # checked against LAPACK and the invariants with the bounds of the
+    # 8192 x 2048 blocked test (thousands of rotations touch every column).
+    _note("grid-barrier kernel, 2100 x 2049")
+    rng = np.random.default_rng(2049)
+    a = rng.standard_normal((2100, 2049))
+    u, s, v, sw = jsvd.svd(a, epsilon=1e-14, max_sweeps=30)
+    ref = np.linalg.svd(a, compute_uv=False)
+    assert np.max(np.abs(np.sort(s)[::-1] - ref)) <= 1e-11 * ref[0]
+    assert metrics.orthogonality(u, s > 1e-10 * ref[0]) <= 1e-9
+    assert metrics.orthogonality(v) <= 1e-9
+    assert metrics.residual(a, u, s, v) <= 1e-11
+    assert sw < 30
 
 
 def test_every_shape_class_against_oracle():
