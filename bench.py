@@ -183,8 +183,9 @@ def blocked_flops(m, n, b):
 
 def blocked_tall(torch, jsvd, gen, metrics, cpu):
     """Single 8192 x 2048 matrix, sigma_k = 10^(-10k/2047), Haar factors, blocked
-    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps).  With cpu,
-    the oracle's plain sweep is timed beside it (one sweep, all host cores)."""
+    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps), and two
+    sweeps of the plain column-pair path.  With cpu, the oracle's plain sweep
+    is timed beside it (one sweep, all host cores)."""
     a, sig = gen.tall_geometric()
     m, n = a.shape
     out = {"workload": "8192x2048, sigma_k=10^(-10k/2047), Haar factors", "epsilon": EPS, "max_sweeps": 80}
@@ -216,12 +217,24 @@ def blocked_tall(torch, jsvd, gen, metrics, cpu):
             "mfma_frac": fl / per_sweep_s / 1e12 / FP64_PEAK_TFLOPS,
             "residual": res, "sigma_err_normwise": norm_err, "sigma_err_componentwise": comp_err}
         plan.close()
+    # Plain column-pair order on the GPU (grouped wave kernel, rows streamed):
+    # two sweeps timed, algorithmic traffic 16 n (n-1) (m+n) bytes per sweep.
+    plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=2)
+    bufs = device_buffers(plan, a.ravel(), torch)
+    wall, kernel_ms = time_plan(plan, bufs, 1, 1, torch)
+    sweeps = int(bufs[4].item())
+    per_sweep_s = kernel_ms * 1e-3 / sweeps
+    byt = 16.0 * n * (n - 1) * (m + n)
+    out["plain_gpu"] = {"sweeps_timed": sweeps, "ms_per_sweep": per_sweep_s * 1e3, "bytes_per_sweep": byt,
+                        "hbm_GBs_per_sweep": byt / per_sweep_s / 1e9,
+                        "hbm_frac": byt / per_sweep_s / 1e9 / HBM_PEAK_GBS}
+    plan.close()
     return out
 
 
 def plain_single(torch, jsvd, gen, cpu):
     """The reference's CPU-runnable case: one 256 x 256 N(0,1) matrix, plain
-    column-pair Jacobi (grid-barrier kernel), with the oracle beside it."""
+    column-pair Jacobi (grouped wave kernel), with the oracle beside it."""
     a = gen.gaussian_square(256)
     plan = jsvd.BatchPlan([256], [256], epsilon=EPS, max_sweeps=MAX_SWEEPS)
     bufs = device_buffers(plan, a.ravel(), torch)
