@@ -449,10 +449,11 @@ namespace
     }
 
     // Plain-path matrices too large for LDS take the wave kernels up to this
-    // many columns: a group of at most 64 sixteen-wave workgroups (one flag
-    // word each) holds one wave per pair of 2048 columns.  Measured per sweep
-    // against one workgroup per pair with a grid barrier: 8192 x 2048 247 vs
-    // 374 ms, 2048 x 1024 26 vs 68 ms, 1100 x 520 6.3 vs 11.4 ms.
+    // many columns: a group of at most 64 sixteen-wave or 256 four-wave
+    // workgroups (one flag word each) holds one wave per pair of 2048
+    // columns.  Measured per sweep against one workgroup per pair with a grid
+    // barrier: 8192 x 2048 175 vs 374 ms, 2048 x 1024 26 vs 68 ms, 1100 x 520
+    // 6.3 vs 11.4 ms.
     constexpr int64_t kWaveMaxColumns = 2048;
 
     // A launch of `count` matrices whose widest has `max_half` column pairs
@@ -591,17 +592,18 @@ namespace
                 p.wave_launches.push_back(l);
                 continue;
             }
-            // Eight-wave workgroups for one or two matrices whose group fits the
-            // 64 flag words of a barrier; sixteen otherwise.
-            l.waves = (l.count <= 2 && (max_half + 7) / 8 <= 64) ? 8 : 16;
+            // One or two matrices: eight-wave workgroups, or four-wave ones when
+            // eight-wave groups would exceed 64 workgroups (so a lone wide
+            // matrix spreads over up to 256 CUs); sixteen-wave groups for more.
+            l.waves = l.count > 2 ? 16 : ((max_half + 7) / 8 <= 64 ? 8 : 4);
             int occupancy = 0;
             JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max, l.waves), 64 * l.waves, 0));
             const int64_t resident = int64_t(std::max(1, occupancy)) * cus;
             const int64_t need = (max_half + l.waves - 1) / l.waves;
             const int64_t members = std::max<int64_t>(1, std::min(need, resident / l.count));
-            // At most 64 members: one polling lane per member in the barrier.
-            l.members = static_cast<int>(std::min<int64_t>(std::min(members, resident), 64));
+            // At most 256 members: four flag words per polling lane.
+            l.members = static_cast<int>(std::min<int64_t>(std::min(members, resident), 256));
             l.groups = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(l.count, resident / l.members)));
             max_flags = std::max<int64_t>(max_flags, int64_t(l.groups) * l.members);
             p.wave_launches.push_back(l);

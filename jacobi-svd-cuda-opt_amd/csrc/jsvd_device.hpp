@@ -269,11 +269,11 @@ namespace jsvd::dev
         }
         __syncthreads();
     }
-    // Barrier among `members` (at most 64) co-resident workgroups through one
+    // Barrier among `members` (at most 256) co-resident workgroups through one
     // flag word each: workgroup `member` publishes its barrier count `epoch`
     // with a release fence and a relaxed store, then wave 0 polls every
-    // member's word at once (one lane per member, relaxed loads) until all
-    // have reached `epoch`, and acquires.  Two trips to the point of agent
+    // member's word at once (lane l reads words l, l + 64, ...; relaxed
+    // loads) until all have reached `epoch`, and acquires.  Two trips to the point of agent
     // coherence per barrier (the store, the loads that see the last store)
     // instead of the arrival RMW, the generation store and the poll of the
     // counter barrier.  Flags start at zero and only grow within a launch.
@@ -290,9 +290,10 @@ namespace jsvd::dev
             const int l = threadIdx.x;
             for (;;)
             {
-                const unsigned int v =
-                    l < members ? __hip_atomic_load(&flags[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : epoch;
-                if (__all(v >= epoch))
+                bool ready = true;
+                for (int w = l; w < members; w += 64)
+                    ready &= __hip_atomic_load(&flags[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= epoch;
+                if (__all(ready))
                     break;
                 __builtin_amdgcn_s_sleep(1);
             }

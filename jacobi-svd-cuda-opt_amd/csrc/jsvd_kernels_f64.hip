@@ -514,9 +514,11 @@ namespace jsvd
         // columns in registers, the three dots are reduced across the wave
         // (DPP and lane shuffles), so a round needs no workgroup barrier.
         // WAVES waves per workgroup: 8 for one or two matrices (a lone solve
-        // is barrier-bound and eight-wave groups measured fastest), 16 for
-        // more (half the workgroups per group: fewer barrier arrivals and L2
-        // write-backs where many groups share the GPU).  A wave takes pairs
+        // is barrier-bound and eight-wave groups measured fastest; 4 when an
+        // eight-wave group would exceed 64 workgroups, so a lone wide matrix
+        // spreads over more CUs), 16 for more (half the workgroups per group:
+        // fewer barrier arrivals and L2 write-backs where many groups share
+        // the GPU).  A wave takes pairs
         // k, k + WAVES members, ... of each round, so the arithmetic of a
         // pair, and with it the result, does not depend on the group shape.
         // Same pairs and order as the reference circle.
@@ -815,7 +817,10 @@ namespace jsvd
     {
         // Taller than the register instances: rows streamed (RW = 0).
         const int rw = m > 64 * kWaveRowsMax ? 0 : static_cast<int>((m + 63) / 64);
-        return waves == 16 ? wave_group_symbol<16>(rw) : waves == 8 ? wave_group_symbol<8>(rw) : nullptr;
+        return waves == 16  ? wave_group_symbol<16>(rw)
+               : waves == 8 ? wave_group_symbol<8>(rw)
+               : waves == 4 ? wave_group_symbol<4>(rw)
+                            : nullptr;
     }
 
     hipError_t launch_wave_block_sweeps_f64(int64_t m, int grid, const WaveGroupArgs &args, hipStream_t stream)
