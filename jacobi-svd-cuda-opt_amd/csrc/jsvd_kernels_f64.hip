@@ -809,14 +809,27 @@ namespace jsvd
         case 6: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<6, WAVES>);
         case 7: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<7, WAVES>);
         case 8: return reinterpret_cast<const void *>(&wave_group_sweeps_f64<8, WAVES>);
+        case 16:
+            if constexpr (WAVES != 16)
+                return reinterpret_cast<const void *>(&wave_group_sweeps_f64<16, WAVES>);
+            else
+                return nullptr;
         default: return nullptr;
         }
     }
 
     const void *wave_group_sweeps_f64_symbol(int64_t m, int waves)
     {
-        // Taller than the register instances: rows streamed (RW = 0).
-        const int rw = m > 64 * kWaveRowsMax ? 0 : static_cast<int>((m + 63) / 64);
+        // Up to 512 rows: the exact row-register count.  513 .. 1024 rows: 16
+        // row registers per lane for the eight- and four-wave groups of one
+        // or two matrices (one load round per pair instead of streamed
+        // chunks: 1024 x 129 10.8 -> 7.9 ms, 1000 x 500 68.8 -> 48.8 ms); the
+        // sixteen-wave groups of batches stream (the registers would halve
+        // their occupancy: 64 x 1024x64 11.8 -> 16.9 ms).  Taller: streamed
+        // (RW = 0).
+        const int rw = m <= 64 * kWaveRowsMax ? static_cast<int>((m + 63) / 64)
+                       : (m <= 1024 && waves != 16) ? 16
+                                                      : 0;
         return waves == 16  ? wave_group_symbol<16>(rw)
                : waves == 8 ? wave_group_symbol<8>(rw)
                : waves == 4 ? wave_group_symbol<4>(rw)
