@@ -592,10 +592,14 @@ namespace
                 p.wave_launches.push_back(l);
                 continue;
             }
-            // One or two matrices: eight-wave workgroups, or four-wave ones when
-            // eight-wave groups would exceed 64 workgroups (so a lone wide
-            // matrix spreads over up to 256 CUs); sixteen-wave groups for more.
-            l.waves = l.count > 2 ? 16 : ((max_half + 7) / 8 <= 64 ? 8 : 4);
+            // One or two matrices: eight-wave workgroups, or four-wave ones (one
+            // wave per SIMD, twice the workgroups) for matrices of more than
+            // 512 rows and at most 512 columns, whose rounds are bound by each
+            // wave's long row loop, and when eight-wave groups would exceed 64
+            // workgroups (so a lone wide matrix spreads over up to 256 CUs);
+            // sixteen-wave groups for more matrices.
+            const bool four = (l.m_max > 512 && max_half <= 256) || (max_half + 7) / 8 > 64;
+            l.waves = l.count > 2 ? 16 : (four ? 4 : 8);
             int occupancy = 0;
             JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max, l.waves), 64 * l.waves, 0));
