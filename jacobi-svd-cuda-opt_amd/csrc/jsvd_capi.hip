@@ -239,6 +239,10 @@ namespace
 // Work-list counters per bucket: two per stage (FP32 sweeps,
 // orthonormalisation, B = A Q, FP64 sweeps); plain buckets use the last two.
 constexpr int kCountersPerBucket = 8;
+// Mixed-precision buckets are split into up to kMixedSplitParts parts of at
+// least kMixedSplitMin matrices, each on a stream of its own.
+constexpr size_t kMixedSplitMin = 64;
+constexpr size_t kMixedSplitParts = 4;
 
 // A bucket is one persistent launch of the LDS-resident work-list kernel:
 // matrices of one shape class whose LDS needs allow the same number of
@@ -830,6 +834,33 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     // Longest-processing-time-first inside each bucket (a sweep costs about
     // n^2 (m + n)), and the buckets with the most work are launched first.
     auto cost = [](const jsvd::WorkItem &x) { return double(x.n) * x.n * (x.m + x.n); };
+    // A mixed-precision bucket is four dependent launches (FP32 sweeps,
+    // Gram-Schmidt, B = AQ, FP64 refinement), and each ends in a tail where
+    // few workgroups still run.  A large one is split into parts (items of
+    // its longest-first order dealt out in turn) that the stream assignment
+    // below puts on streams of their own: one part's stages then fill the
+    // others' tails.  Each matrix's arithmetic does not depend on its
+    // bucket.
+    {
+        const size_t initial = groups.size();
+        for (size_t i = 0; i < initial; ++i)
+        {
+            const size_t parts = std::min(kMixedSplitParts, groups[i].second.size() / kMixedSplitMin);
+            if (groups[i].first.second < 0 || parts < 2)
+                continue;
+            std::vector<jsvd::WorkItem> items;
+            items.swap(groups[i].second);
+            std::stable_sort(items.begin(), items.end(),
+                             [&](const jsvd::WorkItem &x, const jsvd::WorkItem &y) { return cost(x) > cost(y); });
+            std::vector<std::vector<jsvd::WorkItem>> split(parts);
+            for (size_t j = 0; j < items.size(); ++j)
+                split[j % parts].push_back(items[j]);
+            const GroupKey key = groups[i].first;
+            groups[i].second.swap(split[0]);
+            for (size_t q = 1; q < parts; ++q)
+                groups.push_back({key, std::move(split[q])});
+        }
+    }
     for (auto &g : groups)
     {
         std::stable_sort(g.second.begin(), g.second.end(),

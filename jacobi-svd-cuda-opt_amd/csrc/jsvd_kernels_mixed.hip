@@ -1,7 +1,8 @@
 // jsvd_kernels_mixed.hip -- mixed-precision one-sided Jacobi (no reference
 // counterpart: the reference is FP64 only).
 //
-// Per matrix, four work-list launches on one stream:
+// Four work-list launches on one stream per part of a batch (the plan splits
+// a large batch into up to four parts, each on a stream of its own):
 //  1. mixed_sweeps_compact_f32: FP32 sweeps with A and V in the compact
 //     LDS layout of the FP64 class kernel (top columns held in registers,
 //     half + 2 bottom slots in LDS: two 128 x 128 matrices per CU).  Dots
