@@ -78,6 +78,20 @@ namespace jsvd::dev
         s = t * c;
     }
 
+    // The rotated pair x' = c x - s y, y' = s x + c y (the reference's update,
+    // src/domain/jacobi_rotation_kernels.cuh:334-352) with its roundings
+    // fixed: one product, then one fused multiply-add.  Written out so that
+    // every kernel that shares these helpers rounds alike, whatever the
+    // compiler would contract in each loop.
+    __device__ __forceinline__ double rot_first(double c, double s, double x, double y)
+    {
+        return fma(c, x, -(s * y));
+    }
+    __device__ __forceinline__ double rot_second(double c, double s, double x, double y)
+    {
+        return fma(s, x, c * y);
+    }
+
     // Rotation of the pair after the threshold test: Rutishauser's tangent
     // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (qq - pp) / (2 pq)
     // (src/domain/jacobi_rotation_kernels.cuh:313-322), evaluated as

@@ -421,8 +421,8 @@ namespace jsvd
                 const int64_t i = lane + 64 * j;
                 if (i < m)
                 {
-                    ap[i] = c * xp[j] - s * xq[j];
-                    aq[i] = s * xp[j] + c * xq[j];
+                    ap[i] = dev::rot_first(c, s, xp[j], xq[j]);
+                    aq[i] = dev::rot_second(c, s, xp[j], xq[j]);
                 }
             }
 #pragma unroll
@@ -431,8 +431,8 @@ namespace jsvd
                 const int i = lane + 64 * j;
                 if (i < n)
                 {
-                    vp[i] = c * yp[j] - s * yq[j];
-                    vq[i] = s * yp[j] + c * yq[j];
+                    vp[i] = dev::rot_first(c, s, yp[j], yq[j]);
+                    vq[i] = dev::rot_second(c, s, yp[j], yq[j]);
                 }
             }
             return 1;
@@ -441,7 +441,8 @@ namespace jsvd
         // One column pair of any height by one wave, rows streamed in chunks
         // of 256 (4 per lane): the dots accumulate rows lane + 64 j in
         // increasing j, the same FMA chain as wave_pair_f64 (equal dots, so
-        // equal rotation decisions); the update re-reads the columns.
+        // equal rotation decisions); the update re-reads the columns and
+        // rounds like wave_pair_f64's (dev::rot_first / rot_second).
         __device__ __forceinline__ int wave_pair_stream_f64(double *__restrict__ A, double *__restrict__ V,
                                                             int64_t m, int n, int p, int q, int lane, double eps)
         {
@@ -488,8 +489,8 @@ namespace jsvd
                     const int64_t i = r0 + lane + 64 * j;
                     if (i < m)
                     {
-                        ap[i] = c * xp[j] - s * xq[j];
-                        aq[i] = s * xp[j] + c * xq[j];
+                        ap[i] = dev::rot_first(c, s, xp[j], xq[j]);
+                        aq[i] = dev::rot_second(c, s, xp[j], xq[j]);
                     }
                 }
             }
@@ -498,8 +499,8 @@ namespace jsvd
             for (int i = lane; i < n; i += 64)
             {
                 const double x = vp[i], y = vq[i];
-                vp[i] = c * x - s * y;
-                vq[i] = s * x + c * y;
+                vp[i] = dev::rot_first(c, s, x, y);
+                vq[i] = dev::rot_second(c, s, x, y);
             }
             return 1;
         }

@@ -137,7 +137,9 @@ def test_tall_batch_streamed_against_oracle():
     # columns) share one grouped launch; 100 of 700 x 24 exceed the
     # 64 + 12^2 / 100 matrices from which one workgroup per matrix is chosen.
     # Both forms use the same pair arithmetic, so each of the 100 equals its
-    # single (grouped) solve bitwise.
+    # single (grouped) solve bitwise.  A single solve of at most 1024 rows
+    # holds its rows in registers instead of streaming them; it rounds like
+    # the streamed batch, so those singles equal their batch entries too.
     _note("tall batch, streamed rows")
     rng = np.random.default_rng(611)
     shapes = [(1024, 64)] * 16 + [(700, 90), (513, 77), (2000, 40), (600, 129), (1500, 100), (777, 31)]
@@ -146,6 +148,11 @@ def test_tall_batch_streamed_against_oracle():
     ref = oracle.solve_batch(mats, 1e-13, 30)
     for k, (a, g, r) in enumerate(zip(mats, got, ref)):
         _assert_parity(a, g, r, 1e-13, f"tall{k} {a.shape}")
+    for k in (0, 16, 17, 19):
+        single = jsvd.svd(mats[k], epsilon=1e-13, max_sweeps=30)
+        for x, y in zip(single[:3], got[k][:3]):
+            np.testing.assert_array_equal(x, y, err_msg=f"single vs batch {mats[k].shape}")
+        assert single[3] == got[k][3]
     mats = [rng.standard_normal((700, 24)) for _ in range(100)]
     got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
     ref = oracle.solve_batch(mats, 1e-13, 30)
