@@ -601,9 +601,13 @@ namespace
             // 512 rows and at most 512 columns, whose rounds are bound by each
             // wave's long row loop, and when eight-wave groups would exceed 64
             // workgroups (so a lone wide matrix spreads over up to 256 CUs);
-            // sixteen-wave groups for more matrices.
+            // sixteen-wave groups for more matrices, or eight-wave ones for
+            // matrices of more than 512 rows when a group of eight-wave
+            // workgroups per matrix still fits one workgroup per CU (twice the
+            // CUs at work on the long row loops).
             const bool four = (l.m_max > 512 && max_half <= 256) || (max_half + 7) / 8 > 64;
-            l.waves = l.count > 2 ? 16 : (four ? 4 : 8);
+            const bool spread = l.m_max > 512 && int64_t(l.count) * ((max_half + 7) / 8) <= cus;
+            l.waves = l.count > 2 ? (spread ? 8 : 16) : (four ? 4 : 8);
             int occupancy = 0;
             JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
                 &occupancy, jsvd::wave_group_sweeps_f64_symbol(l.m_max, l.waves), 64 * l.waves, 0));

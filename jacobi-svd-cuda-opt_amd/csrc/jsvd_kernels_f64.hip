@@ -514,12 +514,16 @@ namespace jsvd
         // wave per column pair: lane l holds rows l + 64 j (j < RW) of both
         // columns in registers, the three dots are reduced across the wave
         // (DPP and lane shuffles), so a round needs no workgroup barrier.
-        // WAVES waves per workgroup: 8 for one or two matrices (a lone solve
-        // is barrier-bound and eight-wave groups measured fastest; 4 when an
-        // eight-wave group would exceed 64 workgroups, so a lone wide matrix
-        // spreads over more CUs), 16 for more (half the workgroups per group:
-        // fewer barrier arrivals and L2 write-backs where many groups share
-        // the GPU).  A wave takes pairs
+        // WAVES waves per workgroup (chosen by the host): 8 for one or two
+        // matrices (a lone solve is barrier-bound and eight-wave groups
+        // measured fastest), 4 for one or two matrices of more than 512 rows
+        // and at most 512 columns (one wave per SIMD shortens the long row
+        // loop) or when an eight-wave group would exceed 64 workgroups (a
+        // lone wide matrix spreads over more CUs); 16 for more matrices (half
+        // the workgroups per group: fewer barrier arrivals and L2 write-backs
+        // where many groups share the GPU), 8 for more matrices of more than
+        // 512 rows while their eight-wave groups fit one workgroup per CU.
+        // A wave takes pairs
         // k, k + WAVES members, ... of each round, so the arithmetic of a
         // pair, and with it the result, does not depend on the group shape.
         // Same pairs and order as the reference circle.
