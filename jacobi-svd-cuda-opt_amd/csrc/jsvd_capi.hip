@@ -598,14 +598,16 @@ namespace
             }
             // One or two matrices: eight-wave workgroups, or four-wave ones (one
             // wave per SIMD, twice the workgroups) for matrices of more than
-            // 512 rows and at most 512 columns, whose rounds are bound by each
-            // wave's long row loop, and when eight-wave groups would exceed 64
+            // 512 rows and at most 512 columns, or more than 256 rows and at
+            // most 256 columns, whose rounds are bound by each wave's row
+            // loop, and when eight-wave groups would exceed 64
             // workgroups (so a lone wide matrix spreads over up to 256 CUs);
             // sixteen-wave groups for more matrices, or eight-wave ones for
             // matrices of more than 512 rows when a group of eight-wave
             // workgroups per matrix still fits one workgroup per CU (twice the
             // CUs at work on the long row loops).
-            const bool four = (l.m_max > 512 && max_half <= 256) || (max_half + 7) / 8 > 64;
+            const bool four = (l.m_max > 512 && max_half <= 256) || (l.m_max > 256 && max_half <= 128) ||
+                              (max_half + 7) / 8 > 64;
             const bool spread = l.m_max > 512 && int64_t(l.count) * ((max_half + 7) / 8) <= cus;
             l.waves = l.count > 2 ? (spread ? 8 : 16) : (four ? 4 : 8);
             int occupancy = 0;
