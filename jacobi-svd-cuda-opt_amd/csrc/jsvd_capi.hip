@@ -34,13 +34,27 @@ namespace
         return buf;
     }
 
+    // The runtime keeps the last failing call's code until hipGetLastError
+    // reads it, and the launch wrappers report hipGetLastError after each
+    // launch: an earlier failure (the caller's, or an allocation this library
+    // already reported) would be taken for a launch error.  Entry points that
+    // launch therefore start by discarding it, and a reported failure is
+    // consumed as it is reported.
+    void discard_stale_error()
+    {
+        (void)hipGetLastError();
+    }
+
 #define JSVD_HIP(expr)                                                                                         \
     do                                                                                                         \
     {                                                                                                          \
         const hipError_t jsvd_e_ = (expr);                                                                     \
         if (jsvd_e_ != hipSuccess)                                                                             \
+        {                                                                                                      \
+            discard_stale_error();                                                                             \
             return fail(jsvd_e_ == hipErrorOutOfMemory ? JSVD_ENOMEM : JSVD_EDEVICE,                           \
                         hip_message(jsvd_e_, #expr, __FILE__, __LINE__));                                      \
+        }                                                                                                      \
     } while (0)
 
 #define JSVD_CHECK(expr)                                                                                       \
@@ -52,10 +66,14 @@ namespace
     } while (0)
 
     // Validation with the reference's messages
-    // (src/domain/jacobi_svd.cu:197-223 and :36-54 of that file).
+    // (src/domain/jacobi_svd.cu:197-223 and :36-54 of that file).  The
+    // epsilon test is the reference's `epsilon <= 0.0` (jacobi_svd.cu:211), so
+    // a NaN epsilon is accepted as there: every threshold comparison with it
+    // is false, no pair rotates, the solve stops after one sweep with sigma =
+    // the column norms, U = 0 and V = I.
     jsvd_status validate_config(const jsvd_config &c)
     {
-        if (!(c.epsilon > 0.0))
+        if (c.epsilon <= 0.0)
             return fail(JSVD_EINVAL, "epsilon must be positive.");
         if (c.max_sweeps <= 0)
             return fail(JSVD_EINVAL, "max_sweeps must be positive.");
@@ -705,6 +723,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                              jsvd_plan **out)
 {
     g_last_error.clear();
+    discard_stale_error();
     if (!out)
         return fail(JSVD_EINVAL, "plan output pointer is null.");
     *out = nullptr;
@@ -909,7 +928,11 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     for (auto &b : plan->buckets)
     {
         const void *fn = b.cls >= 0 ? jsvd::lds_class_f64_symbol(b.cls) : jsvd::lds_batch_f64_symbol(b.block);
-        JSVD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, b.lds_bytes));
+        // The attribute is a per-kernel upper bound shared by every plan: it
+        // is set to the budget, never to
+        // one bucket's need, so a later plan cannot lower it under an
+        // earlier one.  The occupancy query uses the bucket's own size.
+        JSVD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(jsvd::kMaxLdsBytes)));
         int occupancy = 0;
         JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, fn, b.block, b.lds_bytes));
         if (occupancy <= 0)
@@ -1047,6 +1070,7 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
                                   int32_t *d_sweeps, void *stream_ptr)
 {
     g_last_error.clear();
+    discard_stale_error();
     if (!p)
         return fail(JSVD_EINVAL, "plan is null.");
     if (p->count == 0)
@@ -1178,6 +1202,7 @@ namespace
                               T *ga, void *stream_ptr)
     {
         g_last_error.clear();
+        discard_stale_error();
         if (!p)
             return fail(JSVD_EINVAL, "plan is null.");
         if (p->count == 0)
@@ -1573,3 +1598,4 @@ int32_t jsvd_class_count(void)
 }
 
 } // extern "C"
+

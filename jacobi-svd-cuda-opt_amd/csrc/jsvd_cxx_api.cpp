@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <cstdint>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -27,15 +28,20 @@ namespace jacobi::svd
             throw CudaError(msg.c_str());
         }
 
-        void check_hip(hipError_t e, const char *expr)
+        // Same message shape as the reference's JACOBI_CUDA_CHECK
+        // (src/domain/cuda_check.cuh:18-29): "<runtime> call failed: <expr> @
+        // <file>:<line>, code=<n>, message=<text>".
+        void check_hip(hipError_t e, const char *expr, const char *file, int line)
         {
             if (e == hipSuccess)
                 return;
-            char buf[512];
-            std::snprintf(buf, sizeof(buf), "HIP call failed: %s, code=%d, message=%s", expr, static_cast<int>(e),
-                          hipGetErrorString(e));
+            (void)hipGetLastError(); // consumed here, so later launches do not report it again
+            char buf[768];
+            std::snprintf(buf, sizeof(buf), "HIP call failed: %s @ %s:%d, code=%d, message=%s", expr, file, line,
+                          static_cast<int>(e), hipGetErrorString(e));
             throw CudaError(buf);
         }
+#define JSVD_CXX_HIP(expr) check_hip((expr), #expr, __FILE__, __LINE__)
 
         jsvd_config to_c(const JacobiSvdConfig &c)
         {
@@ -127,18 +133,22 @@ namespace jacobi::svd
         return *this;
     }
 
+    // As the reference (src/domain/device_matrix.cu:48-63): zero dimensions
+    // are rejected before the old allocation is released.
     void DeviceMatrix::reset(std::size_t rows, std::size_t columns)
     {
+        if (rows == 0 || columns == 0)
+            throw std::invalid_argument("DeviceMatrix dimensions must be positive.");
+        if (columns > SIZE_MAX / sizeof(double) / rows)
+            throw std::overflow_error("DeviceMatrix size exceeds size_t.");
         if (data_)
         {
-            check_hip(hipFree(data_), "hipFree(data_)");
+            JSVD_CXX_HIP(hipFree(data_));
             data_ = nullptr;
         }
         rows_ = rows;
         columns_ = columns;
-        if (rows * columns > 0)
-            check_hip(hipMalloc(reinterpret_cast<void **>(&data_), rows * columns * sizeof(double)),
-                      "hipMalloc(&data_, bytes)");
+        JSVD_CXX_HIP(hipMalloc(reinterpret_cast<void **>(&data_), bytes()));
     }
 
     std::size_t DeviceMatrix::rows() const noexcept
@@ -169,18 +179,16 @@ namespace jacobi::svd
     void DeviceMatrix::copy_from_host(std::span<const double> host_values)
     {
         if (host_values.size() != size())
-            throw std::invalid_argument("Host buffer size does not match device matrix size.");
+            throw std::invalid_argument("Host data size does not match device matrix shape.");
         if (size() > 0)
-            check_hip(hipMemcpy(data_, host_values.data(), bytes(), hipMemcpyHostToDevice),
-                      "hipMemcpy(data_, host, bytes, H2D)");
+            JSVD_CXX_HIP(hipMemcpy(data_, host_values.data(), bytes(), hipMemcpyHostToDevice));
     }
 
     std::vector<double> DeviceMatrix::copy_to_host() const
     {
         std::vector<double> out(size(), 0.0);
         if (size() > 0)
-            check_hip(hipMemcpy(out.data(), data_, bytes(), hipMemcpyDeviceToHost),
-                      "hipMemcpy(host, data_, bytes, D2H)");
+            JSVD_CXX_HIP(hipMemcpy(out.data(), data_, bytes(), hipMemcpyDeviceToHost));
         return out;
     }
 } // namespace jacobi::svd

@@ -71,10 +71,14 @@ namespace jacobi::svd::io
     struct MatReader::Impl
     {
         std::ifstream in;
+        std::uint64_t size = 0; // file size, so a record's payload is checked before it is allocated
         explicit Impl(const std::filesystem::path &p) : in(p, std::ios::binary)
         {
             if (!in)
                 throw std::runtime_error("Failed to open mat matrix file for reading.");
+            in.seekg(0, std::ios::end);
+            size = static_cast<std::uint64_t>(in.tellg());
+            in.seekg(0, std::ios::beg);
         }
     };
 
@@ -122,7 +126,14 @@ namespace jacobi::svd::io
         if (rows > std::numeric_limits<std::size_t>::max() || cols > std::numeric_limits<std::size_t>::max())
             throw std::overflow_error("Matrix dimensions exceed size_t.");
         const std::size_t count = checked_mul(static_cast<std::size_t>(rows), static_cast<std::size_t>(cols));
-        checked_mul(count, sizeof(double));
+        const std::size_t payload = checked_mul(count, sizeof(double));
+        // The payload must be in the file before anything is allocated for it
+        // (as src/infrastructure/mat_file.cu:123-129 checks the remaining
+        // bytes first): a corrupt header then fails with the truncation
+        // error instead of a huge allocation.
+        const std::uint64_t here = static_cast<std::uint64_t>(in.tellg());
+        if (here > reader.impl_->size || payload > reader.impl_->size - here)
+            throw std::runtime_error("Truncated *.mat payload.");
         matrix.rows = static_cast<std::size_t>(rows);
         matrix.columns = static_cast<std::size_t>(cols);
         matrix.values.resize(count);

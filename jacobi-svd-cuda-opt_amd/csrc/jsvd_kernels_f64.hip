@@ -518,8 +518,9 @@ namespace jsvd
         // matrices (a lone solve is barrier-bound and eight-wave groups
         // measured fastest), 4 for one or two matrices of more than 512 rows
         // and at most 512 columns, or more than 256 rows and at most 256
-        // columns (one wave per SIMD shortens the row loop) or when an eight-wave group would exceed 64 workgroups (a
-        // lone wide matrix spreads over more CUs); 16 for more matrices (half
+        // columns (one wave per SIMD shortens the row loop) or when an
+        // eight-wave group would exceed 64 workgroups (a lone wide matrix
+        // spreads over more CUs); 16 for more matrices (half
         // the workgroups per group: fewer barrier arrivals and L2 write-backs
         // where many groups share the GPU), 8 for more matrices of more than
         // 512 rows while their eight-wave groups fit one workgroup per CU.

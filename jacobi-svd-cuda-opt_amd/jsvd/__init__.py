@@ -122,9 +122,18 @@ class BatchPlan:
         dt = u.dtype
         if dt not in (torch.float64, torch.float32):
             raise ValueError("backward supports float64 and float32")
-        for t in (u, s, v, gu, gs, gv, ga):
+        if ga is None:
+            raise ValueError("backward needs an output tensor ga")
+        # The kernel reads and writes the packed sizes: a short tensor would
+        # be overrun on the device, so sizes are checked here as in execute().
+        sizes = ((u, self.a_elems), (s, self.s_elems), (v, self.v_elems), (gu, self.a_elems),
+                 (gs, self.s_elems), (gv, self.v_elems), (ga, self.a_elems))
+        for t, nel in sizes:
             if t is not None and not (t.is_cuda and t.dtype == dt and t.is_contiguous()):
                 raise ValueError("backward needs contiguous CUDA tensors of one dtype")
+            if t is not None and t.numel() < nel:
+                raise ValueError("backward needs tensors of the packed sizes (U, gU, gA: sum m*n; sigma, gsigma: "
+                                 "sum n; V, gV: sum n*n)")
         ptr = lambda t: t.data_ptr() if t is not None else None
         st = stream if stream is not None else torch.cuda.current_stream()
         fn = load().jsvd_plan_backward_f64 if dt == torch.float64 else load().jsvd_plan_backward_f32

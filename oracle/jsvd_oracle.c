@@ -103,7 +103,7 @@ int32_t jsvd_oracle_validate(uint64_t rows, uint64_t columns, uint64_t input_siz
         msg = "One-sided Jacobi SVD requires rows >= columns in this implementation.";
     else if (input_size != rows * columns)
         msg = "Input buffer size mismatch.";
-    else if (!(epsilon > 0.0))
+    else if (epsilon <= 0.0) /* jacobi_svd.cu:211: a NaN epsilon passes, as there */
         msg = "epsilon must be positive.";
     else if (max_sweeps <= 0)
         msg = "max_sweeps must be positive.";

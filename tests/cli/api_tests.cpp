@@ -145,6 +145,36 @@ int run_checks()
             threw = true;
         }
         check(threw, "DeviceMatrix size check");
+        // The reference's messages (src/domain/device_matrix.cu:50-52, 99-101).
+        check(invalid([&] { other.copy_from_host(std::vector<double>(5)); },
+                      "Host data size does not match device matrix shape."),
+              "copy_from_host message");
+        check(invalid([&] { other.reset(0, 4); }, "DeviceMatrix dimensions must be positive."), "reset(0, n)");
+        check(invalid([&] { DeviceMatrix z(3, 0); }, "DeviceMatrix dimensions must be positive."), "ctor (m, 0)");
+        // A rejected reset leaves the old allocation in place.
+        check(other.rows() == 3 && other.copy_to_host() == h, "reset(0, n) keeps the matrix");
+    }
+
+    // A device failure surfaces as CudaError with the reference's message
+    // shape (src/domain/cuda_check.cuh:26-28): 2^20 x 2^20 doubles = 8 TiB
+    // is far beyond one GPU's HBM, so hipMalloc reports out of memory.
+    {
+        std::string what;
+        try
+        {
+            DeviceMatrix huge(std::size_t(1) << 20, std::size_t(1) << 20);
+        }
+        catch (const CudaError &e)
+        {
+            what = e.what();
+        }
+        check(what.rfind("HIP call failed: hipMalloc(", 0) == 0 && what.find(" @ ") != std::string::npos &&
+                  what.find(", code=") != std::string::npos && what.find(", message=") != std::string::npos,
+              "DeviceMatrix out-of-memory -> CudaError: '" + what + "'");
+        // The device is still usable afterwards.
+        DeviceMatrix after(2, 2);
+        after.copy_from_host(std::vector<double>{1, 2, 3, 4});
+        check(after.copy_to_host() == std::vector<double>({1, 2, 3, 4}), "device usable after OOM");
     }
 
     const LayoutTransposeAutoTuneReport rep = auto_tune_layout_transpose_threshold();

@@ -100,6 +100,16 @@ int main(int argc, char **argv)
     jacobi::svd::io::write_mat_file(mat, ms);
     fs::resize_file(mat, 16 + 8 * 3);
     check(throws([&] { (void)jacobi::svd::io::read_mat_file(mat); }, "Truncated *.mat payload."), "truncated payload");
+    // A header that claims 2^40 x 2^20 doubles (8 EiB) in a 40-byte file is
+    // a truncation, found before anything is allocated for the payload.
+    {
+        const unsigned char header[16] = {0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 16, 0, 0};
+        std::ofstream out(mat, std::ios::binary | std::ios::trunc);
+        out.write(reinterpret_cast<const char *>(header), 16);
+        out.write("0123456789abcdef01234567", 24);
+    }
+    check(throws([&] { (void)jacobi::svd::io::read_mat_file(mat); }, "Truncated *.mat payload."),
+          "oversized header");
     // Text errors.
     std::ofstream(txt) << "1 2\n3\n";
     check(throws([&] { (void)jacobi::svd::io::read_txt_file(txt); }, "Inconsistent column count"), "ragged rows");

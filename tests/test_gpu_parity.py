@@ -83,10 +83,13 @@ def test_lds_path_against_oracle(shape):
                                            ((5000, 40), 256), ((1024, 129), 1024), ((1100, 520), 256),
                                            ((600, 513), 1024)])
 def test_persistent_path_against_oracle(shape, threads):
-    # These shapes exceed the LDS budget, so they run the grouped wave kernel
-    # (rows in registers up to 512 rows: 256 x 256, 300 x 15x; streamed
-    # beyond: 5000 x 40, 1024 x 129, 1100 x 520, 600 x 513 with sixteen-wave
-    # workgroups).
+    # These shapes exceed the LDS budget, so each runs alone in the grouped
+    # wave kernel (wave_group_sweeps_f64): 256 x 256 with eight-wave
+    # workgroups and rows in registers; 300 x 150 and 300 x 151 with
+    # four-wave ones (more than 256 rows, at most 256 columns); 1024 x 129
+    # four-wave, 16 rows per lane in registers; 5000 x 40 four-wave with rows
+    # streamed; 600 x 513 eight-wave with registers and 1100 x 520 eight-wave
+    # streamed (more than 256 column pairs rule out four-wave groups).
     # For the two wide Gaussian inputs the final column order is chaotic
     # under rounding: a one-ulp perturbation of the 1100 x 520 input moves
     # about 480 of its 520 sigma positions in the oracle itself (sorted sigma
@@ -134,7 +137,7 @@ def test_grouped_wave_batch_against_oracle_and_single_solves():
 def test_tall_batch_streamed_against_oracle():
     # Plain-path matrices taller than 512 rows (and at most 512 columns) have
     # their rows streamed in 256-row chunks.  22 uneven ones (widest 129
-    # columns) share one grouped launch; 100 of 700 x 24 exceed the
+    # columns) share one grouped launch; 100 of 1500 x 24 exceed the
     # 64 + 12^2 / 100 matrices from which one workgroup per matrix is chosen.
     # Both forms use the same pair arithmetic, so each of the 100 equals its
     # single (grouped) solve bitwise.  A single solve of at most 1024 rows
@@ -153,7 +156,13 @@ def test_tall_batch_streamed_against_oracle():
         for x, y in zip(single[:3], got[k][:3]):
             np.testing.assert_array_equal(x, y, err_msg=f"single vs batch {mats[k].shape}")
         assert single[3] == got[k][3]
-    mats = [rng.standard_normal((700, 24)) for _ in range(100)]
+    # 1500 x 24 needs (36000 + 576 + 24) * 8 = 292,800 bytes, beyond the LDS
+    # budget, and 100 of them exceed the 64 + 12^2 / 100 matrices from which
+    # a streamed launch runs one workgroup per matrix (wave_block_stream_f64).
+    mats = [rng.standard_normal((1500, 24)) for _ in range(100)]
+    plan = jsvd.BatchPlan([1500] * 100, [24] * 100, epsilon=1e-13, max_sweeps=30)
+    assert plan.info["large_matrices"] == 100 and plan.info["lds_matrices"] == 0
+    plan.close()
     got = jsvd.svd_batched(mats, epsilon=1e-13, max_sweeps=30)
     ref = oracle.solve_batch(mats, 1e-13, 30)
     for k, (a, g, r) in enumerate(zip(mats, got, ref)):
@@ -274,6 +283,18 @@ def test_special_cases():
     tiny = np.vstack([np.diag([1.0, 5e-10]), np.zeros((1, 2))])
     u, s, v, sw = jsvd.svd(tiny, epsilon=1e-9)
     assert s[1] == 5e-10 and np.all(u[:, 1] == 0.0)
+
+    # A NaN epsilon passes the reference's `epsilon <= 0.0` check
+    # (src/domain/jacobi_svd.cu:211): nothing rotates, one sweep, sigma = the
+    # column norms, U = 0, V = I -- on the LDS path and a large-matrix path.
+    for shape in [(9, 5), (300, 150)]:
+        a = rng.standard_normal(shape)
+        u, s, v, sw = jsvd.svd(a, epsilon=float("nan"), max_sweeps=8)
+        uo, so, vo, swo = oracle.solve(a, float("nan"), 8)
+        assert sw == swo == 1, shape
+        np.testing.assert_array_equal(v, np.eye(shape[1]))
+        assert np.all(u == 0.0) and np.all(uo == 0.0)
+        np.testing.assert_allclose(s, so, rtol=1e-14)
 
 
 def test_repeat_runs_are_bitwise_identical():
@@ -407,3 +428,19 @@ def test_batch_plan_repeat_execution():
         for x, y in zip(outs[0], o):
             assert torch.equal(x, y)
     plan.close()
+
+
+def test_device_out_of_memory_is_reported():
+    # A plan whose workspace is far beyond HBM (a 2^24 x 2^14 matrix needs a
+    # 2 TiB column-major copy): hipMalloc fails, the C-ABI returns
+    # JSVD_ENOMEM with the message shape of the reference's CUDA check
+    # (src/domain/cuda_check.cuh:26-28), and the device stays usable.
+    _note("out-of-memory plan")
+    with pytest.raises(jsvd.JsvdError) as e:
+        jsvd.BatchPlan([1 << 24], [1 << 14])
+    assert e.value.status == 3  # JSVD_ENOMEM
+    msg = str(e.value)
+    assert msg.startswith("HIP call failed: ") and " @ " in msg and ", code=" in msg and ", message=" in msg
+    a = np.random.default_rng(3).standard_normal((40, 30))
+    u, s, v, sw = jsvd.svd(a, epsilon=1e-13, max_sweeps=30)
+    assert metrics.residual(a, u, s, v) <= 1e-13

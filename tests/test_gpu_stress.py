@@ -1,8 +1,9 @@
 """GPU tests on the stress mix (jsvd.gen.stress_mix): the reference
 generator's scenario kinds (dense uniform / normal / log-uniform,
 diagonal-dominant, low-rank, ill-conditioned, sparse, zero columns) through
-every solver path -- LDS work-list classes, the grid-barrier kernel and the
-blocked path -- against the oracle.
+every solver path -- LDS work-list classes, the grouped wave kernel (a lone
+300 x 150 runs four-wave workgroups with rows in registers) and the blocked
+path -- against the oracle.
 
 Bar: sigma sorted within 1e-12 of the oracle's largest singular value;
 residual ||A - U S V^T|| / ||A|| <= 1e-12; ||V^T V - I||_F <= 1e-11; U

@@ -216,6 +216,23 @@ def test_nan_column_quirk():
     assert s[0] == 1.0 and s[2] == 1.0
 
 
+def test_nan_epsilon_accepted_like_the_reference():
+    # src/domain/jacobi_svd.cu:211 rejects `epsilon <= 0.0`, which is false
+    # for NaN: the solve runs, every threshold comparison with NaN is false,
+    # so nothing rotates (one sweep, V = I), sigma = the column norms and the
+    # absolute-epsilon U rule (sigma > NaN is false) gives U = 0.
+    rng = np.random.default_rng(12)
+    a = rng.standard_normal((9, 5))
+    assert oracle.validate(9, 5, 45, float("nan"), 8) == (0, None)
+    u, s, v, sw = oracle.solve(a, float("nan"), 8)
+    assert sw == 1
+    np.testing.assert_array_equal(v, np.eye(5))
+    np.testing.assert_allclose(s, np.linalg.norm(a, axis=0), rtol=1e-15)
+    assert np.all(u == 0.0)
+    # -0.0 and negative values stay rejected.
+    assert oracle.validate(9, 5, 45, -0.0, 8)[0] == 1
+
+
 def test_max_sweeps_stops_silently():
     rng = np.random.default_rng(11)
     a = rng.standard_normal((30, 30))
