@@ -135,6 +135,23 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *plan, const double *d_a, double *d_
 jsvd_status jsvd_plan_get_info(const jsvd_plan *plan, jsvd_plan_info *info);
 void jsvd_plan_destroy(jsvd_plan *plan);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under stream capture, serving loops, harnesses that forbid hipMalloc): the
+ * solve of jsvd_plan_create + jsvd_plan_execute_f64 in one call that
+ * allocates nothing, on the device or the host.  Same packed layout, same
+ * per-matrix kernels and results (bitwise) as a plan of the same batch;
+ * asynchronous on `stream`, d_sweeps may be NULL.
+ * Limits: at most 32768 matrices, FP64 only (cfg->precision must be
+ * JSVD_PRECISION_FP64), and every matrix must fit the LDS-resident work-list
+ * kernel with A and V in LDS (every m <= 160 with n <= 128 does, and every
+ * m <= 256 with n <= 64); anything else returns JSVD_EINVAL with a message.  The work-lists live in
+ * a static 2.5 MiB device array per GPU, which keeps the last call's
+ * contents: calls that overlap on one GPU (two streams, two threads) race on
+ * it and must be serialised by the caller. */
+jsvd_status jsvd_solve_device_noalloc_f64(int64_t count, const int64_t *m, const int64_t *n,
+                                          const jsvd_config *cfg, const double *d_a, double *d_u, double *d_sigma,
+                                          double *d_v, int32_t *d_sweeps, void *stream);
+
 /* SVD adjoint (no reference counterpart: the reference has no derivative).
  * For each matrix of the plan, from the solve's U, sigma, V (any column
  * order, as returned) and upstream gradients gU, gsigma, gV (each may be

@@ -1,6 +1,7 @@
 // jsvd_capi.hip -- the C-ABI (include/jsvd_c.h): validation, device work-list
-// plans, workspace, launches and error reporting.  Host code only; the
-// kernels live in jsvd_kernels_*.hip.
+// plans, workspace, launches and error reporting.  The solver kernels live in
+// jsvd_kernels_*.hip; this file holds only the two small work-list kernels of
+// the no-allocation entry point, beside its static scratch array.
 #include "jsvd_c.h"
 #include "jsvd_kernels.hpp"
 
@@ -928,8 +929,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     for (auto &b : plan->buckets)
     {
         const void *fn = b.cls >= 0 ? jsvd::lds_class_f64_symbol(b.cls) : jsvd::lds_batch_f64_symbol(b.block);
-        // The attribute is a per-kernel upper bound shared by every plan: it
-        // is set to the budget, never to
+        // The attribute is a per-kernel upper bound shared by every plan (and
+        // the no-allocation entry point): it is set to the budget, never to
         // one bucket's need, so a later plan cannot lower it under an
         // earlier one.  The occupancy query uses the bucket's own size.
         JSVD_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(jsvd::kMaxLdsBytes)));
@@ -1599,3 +1600,259 @@ int32_t jsvd_class_count(void)
 
 } // extern "C"
 
+// ---------------------------------------------------------------------------
+// jsvd_solve_device_noalloc_f64: the device-resident batch of
+// jsvd_plan_create + jsvd_plan_execute_f64 for callers that cannot allocate
+// (code under stream capture, serving loops).  Its work-lists and counters
+// live in the static array below; nothing is allocated on the device or the
+// host, and every device step is a kernel on the caller's stream.
+// ---------------------------------------------------------------------------
+namespace
+{
+    // At most this many matrices per call; the scratch holds two work-lists of
+    // that length (the batch-order staging copy and the bucket-ordered list
+    // the solver reads) and the work-list counters.
+    constexpr int64_t kNoAllocMaxMatrices = 32768;
+    constexpr int kNoAllocBuckets = 16; // >= jsvd::class_count()
+    constexpr int kNoAllocWidths = 257; // column counts 0 .. 256 for the longest-first order
+    constexpr size_t kNoAllocListBytes = size_t(kNoAllocMaxMatrices) * sizeof(jsvd::WorkItem);
+    constexpr size_t kNoAllocCounterBytes = 2 * kNoAllocBuckets * sizeof(unsigned int);
+    constexpr size_t kNoAllocScratchBytes = 2 * kNoAllocListBytes + 256;
+    static_assert(kNoAllocCounterBytes <= 256, "counters fit the tail of the scratch");
+    // Work-list entries per upload launch: they travel in the kernel
+    // arguments (3848 bytes), so no host-to-device copy is needed.
+    constexpr int kNoAllocChunk = 96;
+} // namespace
+
+// 2 x 1.25 MiB + 256 B on every device the library runs on, held until the
+// process ends.  Host code never uses this name as an address: it is only the
+// symbol handed to hipGetSymbolAddress, whose per-device result is what the
+// kernels receive.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace
+{
+    struct NoAllocChunk
+    {
+        int32_t count;
+        int32_t base;
+        jsvd::WorkItem items[kNoAllocChunk];
+    };
+
+    __global__ __launch_bounds__(128) void noalloc_store_items(NoAllocChunk chunk, jsvd::WorkItem *staging)
+    {
+        const int i = static_cast<int>(threadIdx.x);
+        if (i < chunk.count)
+            staging[chunk.base + i] = chunk.items[i];
+    }
+
+    struct NoAllocOrder
+    {
+        int32_t count;
+        int32_t buckets;
+        int32_t base[kNoAllocBuckets]; // first list entry of each bucket
+    };
+
+    // One workgroup: counting sort of the staged entries into bucket order
+    // (the bucket is in the entry's pad field), widest matrices first inside
+    // a bucket (the longest-first order of the plan's work-lists).  Entries of
+    // equal width land in no fixed order, which changes no result: each
+    // matrix is solved whole by one workgroup whatever its position.
+    __global__ __launch_bounds__(1024) void noalloc_order(const jsvd::WorkItem *__restrict__ staging,
+                                                          jsvd::WorkItem *__restrict__ list, NoAllocOrder args)
+    {
+        __shared__ int cursor[kNoAllocBuckets * kNoAllocWidths];
+        const int tid = static_cast<int>(threadIdx.x);
+        for (int i = tid; i < args.buckets * kNoAllocWidths; i += 1024)
+            cursor[i] = 0;
+        __syncthreads();
+        auto bin = [](const jsvd::WorkItem &w) {
+            return w.pad * kNoAllocWidths + (kNoAllocWidths - 1 - min(w.n, kNoAllocWidths - 1));
+        };
+        for (int k = tid; k < args.count; k += 1024)
+            atomicAdd(&cursor[bin(staging[k])], 1);
+        __syncthreads();
+        if (tid < args.buckets)
+        {
+            int run = args.base[tid];
+            for (int w = 0; w < kNoAllocWidths; ++w)
+            {
+                const int c = cursor[tid * kNoAllocWidths + w];
+                cursor[tid * kNoAllocWidths + w] = run;
+                run += c;
+            }
+        }
+        __syncthreads();
+        for (int k = tid; k < args.count; k += 1024)
+        {
+            const jsvd::WorkItem w = staging[k];
+            list[atomicAdd(&cursor[bin(w)], 1)] = w;
+        }
+    }
+
+    // Per device: the scratch array's address (hipGetSymbolAddress, once,
+    // with that device current) and the CU count for the launch grids.
+    struct NoAllocDevice
+    {
+        bool ready = false;
+        unsigned char *ptr = nullptr;
+        int cus = 0;
+    };
+    std::mutex g_noalloc_mu;
+    NoAllocDevice g_noalloc[64];
+
+    jsvd_status noalloc_device(int dev, NoAllocDevice *out)
+    {
+        if (dev < 0 || dev >= 64)
+            return fail(JSVD_EINVAL, "jsvd_solve_device_noalloc_f64 supports devices 0..63.");
+        std::lock_guard<std::mutex> lock(g_noalloc_mu);
+        NoAllocDevice &d = g_noalloc[dev];
+        if (!d.ready)
+        {
+            void *p = nullptr;
+            JSVD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            int cus = 0;
+            JSVD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            for (int c = 0; c < jsvd::class_count(); ++c)
+                if (!jsvd::class_uses_vlog(c))
+                    JSVD_HIP(hipFuncSetAttribute(jsvd::lds_class_f64_symbol(c),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 int(jsvd::kMaxLdsBytes)));
+            d.ptr = static_cast<unsigned char *>(p);
+            d.cus = cus;
+            d.ready = true;
+        }
+        *out = d;
+        return JSVD_OK;
+    }
+} // namespace
+
+extern "C" jsvd_status jsvd_solve_device_noalloc_f64(int64_t count, const int64_t *m, const int64_t *n,
+                                                     const jsvd_config *cfg_in, const double *d_a, double *d_u,
+                                                     double *d_sigma, double *d_v, int32_t *d_sweeps,
+                                                     void *stream_ptr)
+{
+    g_last_error.clear();
+    discard_stale_error();
+    if (count < 0 || (count > 0 && (!m || !n)))
+        return fail(JSVD_EINVAL, "Input batch description is invalid.");
+    if (count > kNoAllocMaxMatrices)
+        return fail(JSVD_EINVAL, "jsvd_solve_device_noalloc_f64 takes at most 32768 matrices (the size of its "
+                                 "static work-list); larger batches need jsvd_plan_create.");
+    jsvd_config cfg;
+    jsvd_config_default(&cfg);
+    if (cfg_in)
+        cfg = *cfg_in;
+    JSVD_CHECK(validate_config(cfg));
+    if (cfg.precision != JSVD_PRECISION_FP64)
+        return fail(JSVD_EINVAL, "jsvd_solve_device_noalloc_f64 solves in FP64 only.");
+    if (count == 0)
+        return JSVD_OK;
+    if (!d_a || !d_u || !d_sigma || !d_v)
+        return fail(JSVD_EINVAL, "device buffer pointer is null.");
+
+    // Pass 1: shapes, the class of every matrix (the plan's choice, so each
+    // result equals the plan's bitwise), and per-class counts, work and LDS.
+    bool four_wave = false;
+    for (int64_t k = 0; k < count; ++k)
+    {
+        if (m[k] <= 0 || n[k] <= 0)
+            return fail(JSVD_EINVAL, "Input matrix shape must be non-zero.");
+        JSVD_CHECK(validate_shape(static_cast<uint64_t>(m[k]), static_cast<uint64_t>(n[k])));
+        four_wave = four_wave || (tuned_class(m[k], n[k]) < 0 && jsvd::select_class_f64(m[k], n[k]) == 4);
+    }
+    const int classes = jsvd::class_count();
+    int64_t per_class[kNoAllocBuckets] = {};
+    double work[kNoAllocBuckets] = {};
+    int lds[kNoAllocBuckets] = {};
+    for (int64_t k = 0; k < count; ++k)
+    {
+        const int cls = preferred_class(m[k], n[k], four_wave);
+        if (cls < 0 || jsvd::class_uses_vlog(cls))
+            return fail(JSVD_EINVAL, "jsvd_solve_device_noalloc_f64 takes matrices that fit the LDS-resident "
+                                     "work-list kernel with A and V in LDS (every m <= 160 with n <= 128 does, "
+                                     "and every m <= 256 with n <= 64); larger ones need jsvd_plan_create.");
+        per_class[cls] += 1;
+        work[cls] += double(n[k]) * n[k] * (m[k] + n[k]);
+        lds[cls] = std::max<int>(lds[cls], static_cast<int>(jsvd::class_lds_bytes(cls, m[k], n[k])));
+    }
+    // Buckets: one per class present, most work first (as the plan orders
+    // its launches); all run on the caller's stream.
+    int order[kNoAllocBuckets], buckets = 0, bucket_of[kNoAllocBuckets];
+    for (int c = 0; c < classes; ++c)
+        if (per_class[c] > 0)
+            order[buckets++] = c;
+    std::stable_sort(order, order + buckets, [&](int x, int y) { return work[x] > work[y]; });
+    NoAllocOrder ord{};
+    ord.count = static_cast<int32_t>(count);
+    ord.buckets = buckets;
+    for (int b = 0, first = 0; b < buckets; ++b)
+    {
+        bucket_of[order[b]] = b;
+        ord.base[b] = first;
+        first += static_cast<int>(per_class[order[b]]);
+    }
+
+    int dev = 0;
+    JSVD_HIP(hipGetDevice(&dev));
+    NoAllocDevice nd;
+    JSVD_CHECK(noalloc_device(dev, &nd));
+    auto *staging = reinterpret_cast<jsvd::WorkItem *>(nd.ptr);
+    auto *list = reinterpret_cast<jsvd::WorkItem *>(nd.ptr + kNoAllocListBytes);
+    auto *counters = reinterpret_cast<unsigned int *>(nd.ptr + 2 * kNoAllocListBytes);
+    hipStream_t stream = static_cast<hipStream_t>(stream_ptr);
+
+    // Pass 2: the entries in batch order (offsets of the packed layout),
+    // uploaded through kernel arguments, then ordered on the device.
+    NoAllocChunk chunk{};
+    int64_t ao = 0, vo = 0, so = 0;
+    for (int64_t k = 0; k < count; ++k)
+    {
+        jsvd::WorkItem &w = chunk.items[chunk.count++];
+        w.m = static_cast<int32_t>(m[k]);
+        w.n = static_cast<int32_t>(n[k]);
+        w.index = static_cast<int32_t>(k);
+        w.pad = bucket_of[preferred_class(m[k], n[k], four_wave)];
+        w.a_off = ao;
+        w.v_off = vo;
+        w.s_off = so;
+        ao += m[k] * n[k];
+        vo += n[k] * n[k];
+        so += n[k];
+        if (chunk.count == kNoAllocChunk || k + 1 == count)
+        {
+            hipLaunchKernelGGL(noalloc_store_items, dim3(1), dim3(128), 0, stream, chunk, staging);
+            JSVD_HIP(hipGetLastError());
+            chunk.base += chunk.count;
+            chunk.count = 0;
+        }
+    }
+    hipLaunchKernelGGL(noalloc_order, dim3(1), dim3(1024), 0, stream, staging, list, ord);
+    JSVD_HIP(hipGetLastError());
+    // The solver's counters start at zero (a fill kernel, not a memset).
+    JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(counters), 0, 2 * buckets, stream));
+    for (int b = 0; b < buckets; ++b)
+    {
+        const int cls = order[b];
+        int occ = 0;
+        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, jsvd::lds_class_f64_symbol(cls),
+                                                              jsvd::class_block(cls), lds[cls]));
+        if (occ <= 0)
+            return fail(JSVD_EDEVICE, "work-list kernel cannot be resident with the requested LDS size.");
+        jsvd::ClassArgs args{};
+        args.items = list + ord.base[b];
+        args.count = static_cast<int>(per_class[cls]);
+        args.a = d_a;
+        args.u = d_u;
+        args.sigma = d_sigma;
+        args.v = d_v;
+        args.sweeps = d_sweeps;
+        args.eps = cfg.epsilon;
+        args.max_sweeps = cfg.max_sweeps;
+        args.counters = counters + 2 * b;
+        args.priority = occ == 1 ? 1 : 0;
+        const int grid = static_cast<int>(std::min<int64_t>(int64_t(occ) * nd.cus, per_class[cls]));
+        JSVD_HIP(jsvd::launch_lds_class_f64(cls, grid, lds[cls], args, stream));
+    }
+    return JSVD_OK;
+}

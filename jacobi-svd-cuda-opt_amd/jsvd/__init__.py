@@ -19,7 +19,7 @@ from ._lib import (JsvdConfig, JsvdError, JsvdInvalidArgument, LAYOUT_AUTO, LAYO
 __all__ = [
     "JsvdConfig", "JsvdError", "JsvdInvalidArgument", "PRECISION_FP64", "PRECISION_MIXED", "LAYOUT_AUTO",
     "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets",
-    "autotune_batch", "tune_class",
+    "autotune_batch", "tune_class", "solve_device_noalloc",
 ]
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -149,6 +149,30 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def solve_device_noalloc(ms, ns, a, u, s, v, sweeps=None, cfg: JsvdConfig | None = None, stream=None,
+                         **overrides):
+    """``jsvd_solve_device_noalloc_f64``: the solve of a BatchPlan of these
+    shapes in one call that allocates nothing (static device work-lists; at
+    most 32768 matrices that fit the LDS-resident kernel, FP64).  Tensors as
+    in BatchPlan.execute; runs on the current torch stream."""
+    import torch
+
+    ms = np.ascontiguousarray(ms, dtype=np.int64)
+    ns = np.ascontiguousarray(ns, dtype=np.int64)
+    _, _, _, (a_elems, v_elems, s_elems) = pack_offsets(ms, ns)
+    for t, nel in ((a, a_elems), (u, a_elems), (s, s_elems), (v, v_elems)):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() >= nel):
+            raise ValueError("solve_device_noalloc needs contiguous float64 CUDA tensors of the packed sizes")
+    if sweeps is not None and not (sweeps.is_cuda and sweeps.dtype == torch.int32 and sweeps.numel() >= len(ms)):
+        raise ValueError("sweeps must be an int32 CUDA tensor with one slot per matrix")
+    c = _as_cfg(cfg, overrides)
+    st = stream if stream is not None else torch.cuda.current_stream()
+    _lib.check(load().jsvd_solve_device_noalloc_f64(
+        len(ms), ms.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), ns.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+        ctypes.byref(c), a.data_ptr(), u.data_ptr(), s.data_ptr(), v.data_ptr(),
+        sweeps.data_ptr() if sweeps is not None else None, st.cuda_stream))
 
 
 def tune_class(bucket: int, cls: int) -> int:

@@ -109,6 +109,8 @@ SIGNATURES = {
     "jsvd_plan_execute_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jsvd_plan_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(JsvdPlanInfo)]),
     "jsvd_plan_destroy": (None, [_vp]),
+    "jsvd_solve_device_noalloc_f64": (ctypes.c_int, [ctypes.c_int64, _i64p, _i64p, ctypes.POINTER(JsvdConfig),
+                                                     _vp, _vp, _vp, _vp, _vp, _vp]),
     "jsvd_plan_backward_f64": (ctypes.c_int, [_vp] * 9),
     "jsvd_plan_backward_f32": (ctypes.c_int, [_vp] * 9),
     "jsvd_autotune": (ctypes.c_int, [ctypes.POINTER(JsvdConfig), ctypes.POINTER(JsvdAutotuneReport)]),
