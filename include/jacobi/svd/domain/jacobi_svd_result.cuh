@@ -15,5 +15,8 @@ namespace jacobi::svd
         std::vector<double> sigma;
         std::vector<double> v;
         int sweeps = 0;
+        // Appended (the reference stops silently at max_sweeps): true when the
+        // last sweep rotated no pair, false when max_sweeps ran out first.
+        bool converged = true;
     };
 } // namespace jacobi::svd

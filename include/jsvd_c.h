@@ -44,6 +44,14 @@ enum
     JSVD_PRECISION_MIXED = 1  /* FP32 sweeps + FP64 rotation parameters + FP64 refinement */
 };
 
+/* Why a solve stopped, per matrix (no reference counterpart: the reference
+ * stops silently after max_sweeps, src/domain/jacobi_svd.cu:156-185). */
+enum
+{
+    JSVD_STOP_CONVERGED = 0,  /* the last sweep rotated no pair: every pair passed the threshold test */
+    JSVD_STOP_MAX_SWEEPS = 1  /* max_sweeps sweeps ran and the last one still rotated a pair */
+};
+
 /* Layout policy values (reference LayoutTransposeMode,
  * include/jacobi/svd/domain/layout_transpose.cuh:10-26). */
 enum
@@ -97,6 +105,13 @@ jsvd_status jsvd_solve_batched_f64(int64_t count, const double *const *a_rm, con
                                    const jsvd_config *cfg, double *const *u_rm, double *const *sigma,
                                    double *const *v_rm, int32_t *sweeps);
 
+/* jsvd_solve_batched_f64 plus the stop reason of every matrix
+ * (JSVD_STOP_*, one int32 per matrix; NULL = not wanted). */
+jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_rm, const uint64_t *m,
+                                          const uint64_t *n, const jsvd_config *cfg, double *const *u_rm,
+                                          double *const *sigma, double *const *v_rm, int32_t *sweeps,
+                                          int32_t *stop);
+
 /* ---- device-resident batches (no reference counterpart: the reference has no
  * batched or asynchronous entry point; this is the work-list launch the
  * batched pipeline above is built on) ---- */
@@ -131,6 +146,22 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
  * device once per sweep of each of them to read its convergence flag. */
 jsvd_status jsvd_plan_execute_f64(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma, double *d_v,
                                   int32_t *d_sweeps, void *stream);
+
+/* jsvd_plan_execute_f64 plus the stop reason of every matrix in device
+ * memory (JSVD_STOP_*, one int32 per matrix, batch order; may be NULL). */
+jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma,
+                                         double *d_v, int32_t *d_sweeps, int32_t *d_stop, void *stream);
+
+/* Error metrics of a plan's results (no reference counterpart: the reference
+ * reports sweeps only, src/interfaces/cli_support.cu:427-464).  From the
+ * device buffers of a jsvd_plan_execute_f64 call (d_a its input, d_u /
+ * d_sigma / d_v its outputs), writes three doubles per matrix to d_out:
+ *   ||U^T U - D||_F with D = diag(sigma_j > epsilon) (the U columns the U
+ *   rule keeps), ||V^T V - I||_F, and ||A - U diag(sigma) V^T||_F / ||A||_F.
+ * Asynchronous on `stream`; the first call on a plan allocates and uploads
+ * its tile list (synchronously), later calls allocate nothing. */
+jsvd_status jsvd_plan_metrics_f64(jsvd_plan *plan, const double *d_a, const double *d_u, const double *d_sigma,
+                                  const double *d_v, double *d_out, void *stream);
 
 jsvd_status jsvd_plan_get_info(const jsvd_plan *plan, jsvd_plan_info *info);
 void jsvd_plan_destroy(jsvd_plan *plan);

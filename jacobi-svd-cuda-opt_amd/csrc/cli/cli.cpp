@@ -170,6 +170,10 @@ namespace jacobi::svd::cli
               {OptionId::precision, "precision", '\0', true, "P", "Arithmetic: fp64|mixed (default fp64)."},
               {OptionId::block_width, "block-width", '\0', true, "B",
                "Blocked Jacobi for large matrices: 0 (off, default), 16 or 32."},
+              {OptionId::report_errors, "report-errors", '\0', false, "",
+               "Report per-testcase orthogonality of U and V and the relative residual."},
+              {OptionId::reference_sigma, "reference-sigma", '\0', true, "PATH",
+               "Stream of reference singular values (1 x n per testcase); implies --report-errors."},
           }
     {
     }
@@ -325,6 +329,11 @@ namespace jacobi::svd::cli
         case OptionId::version: r.action = ParseAction::version; return;
         case OptionId::precision: opt.precision = parse_precision(*v, o.long_name); return;
         case OptionId::block_width: opt.block_width = parse_block_width(*v, o.long_name); return;
+        case OptionId::report_errors: opt.report_errors = true; return;
+        case OptionId::reference_sigma:
+            opt.reference_sigma_path = std::filesystem::path(std::string(*v));
+            opt.report_errors = true;
+            return;
         }
     }
 
@@ -396,6 +405,8 @@ namespace jacobi::svd::cli
             throw CliArgumentError("Input and output paths must be different files.");
         if (std::filesystem::exists(o.output_path) && !o.force_overwrite)
             throw CliArgumentError("Output file already exists. Use --force to overwrite: " + o.output_path.string());
+        if (!o.reference_sigma_path.empty() && !std::filesystem::is_regular_file(o.reference_sigma_path))
+            throw CliArgumentError("Reference sigma file does not exist: " + o.reference_sigma_path.string());
     }
 
     pipeline::PipelineConfig make_pipeline_config(const CliOptions &o)
@@ -406,6 +417,8 @@ namespace jacobi::svd::cli
         c.input_format = o.input_format;
         c.output_format = o.output_format;
         c.max_queued_results = o.queue_capacity;
+        c.report_errors = o.report_errors;
+        c.reference_sigma_path = o.reference_sigma_path;
         c.kernel_config = JacobiSvdConfig{
             .epsilon = o.epsilon,
             .max_sweeps = o.max_sweeps,
@@ -464,7 +477,8 @@ namespace jacobi::svd::cli
                   << "queue-capacity  : " << o.queue_capacity << '\n'
                   << "force-overwrite : " << (o.force_overwrite ? "true" : "false") << '\n'
                   << "precision       : " << (o.precision == JacobiPrecision::mixed ? "mixed" : "fp64") << '\n'
-                  << "block-width     : " << o.block_width << '\n';
+                  << "block-width     : " << o.block_width << '\n'
+                  << "report-errors   : " << (o.report_errors ? "true" : "false") << '\n';
     }
 
     void print_text_report(const pipeline::PipelineReport &r, double ms)
@@ -479,7 +493,25 @@ namespace jacobi::svd::cli
                   << "layout-auto-tune: " << (r.layout_transpose_auto_tuned ? "true" : "false") << '\n'
                   << "layout-best-spd : " << std::fixed << std::setprecision(3)
                   << r.layout_transpose_estimated_best_speedup << '\n'
-                  << "elapsed-ms      : " << std::fixed << std::setprecision(3) << ms << '\n';
+                  << "elapsed-ms      : " << std::fixed << std::setprecision(3) << ms << '\n'
+                  << "unconverged     : " << r.unconverged_count << '\n';
+        if (r.errors_reported && !r.matrix_errors.empty())
+        {
+            double ou = 0.0, ov = 0.0, res = 0.0, se = -1.0;
+            for (const auto &e : r.matrix_errors)
+            {
+                ou = std::max(ou, e.orth_u);
+                ov = std::max(ov, e.orth_v);
+                res = std::max(res, e.residual);
+                se = std::max(se, e.sigma_err_norm);
+            }
+            std::cout << std::scientific << std::setprecision(3) << "max-orth-u      : " << ou << '\n'
+                      << "max-orth-v      : " << ov << '\n'
+                      << "max-residual    : " << res << '\n';
+            if (se >= 0.0)
+                std::cout << "max-sigma-err   : " << se << '\n';
+            std::cout << std::defaultfloat;
+        }
     }
 
     void print_json_report(const pipeline::PipelineReport &r, double ms)
@@ -495,8 +527,32 @@ namespace jacobi::svd::cli
                   << ",\n"
                   << "  \"layout_transpose_estimated_best_speedup\": " << std::fixed << std::setprecision(3)
                   << r.layout_transpose_estimated_best_speedup << ",\n"
-                  << "  \"elapsed_ms\": " << std::fixed << std::setprecision(3) << ms << '\n'
-                  << "}\n";
+                  << "  \"elapsed_ms\": " << std::fixed << std::setprecision(3) << ms << ",\n"
+                  << "  \"unconverged_count\": " << r.unconverged_count << ",\n"
+                  << "  \"matrix_sweeps\": [";
+        for (std::size_t k = 0; k < r.matrix_sweeps.size(); ++k)
+            std::cout << (k ? ", " : "") << r.matrix_sweeps[k];
+        std::cout << "],\n  \"matrix_stop\": [";
+        for (std::size_t k = 0; k < r.matrix_converged.size(); ++k)
+            std::cout << (k ? ", " : "") << (r.matrix_converged[k] ? "\"converged\"" : "\"max_sweeps\"");
+        std::cout << "]";
+        if (r.errors_reported)
+        {
+            // Full precision: these are error norms near the rounding level.
+            std::cout << ",\n  \"matrix_errors\": [" << std::setprecision(17) << std::scientific;
+            for (std::size_t k = 0; k < r.matrix_errors.size(); ++k)
+            {
+                const auto &e = r.matrix_errors[k];
+                std::cout << (k ? ",\n    " : "\n    ") << "{\"orth_u\": " << e.orth_u << ", \"orth_v\": " << e.orth_v
+                          << ", \"residual\": " << e.residual;
+                if (e.sigma_err_norm >= 0.0)
+                    std::cout << ", \"sigma_err_norm\": " << e.sigma_err_norm
+                              << ", \"sigma_err_comp\": " << e.sigma_err_comp;
+                std::cout << "}";
+            }
+            std::cout << "\n  ]" << std::defaultfloat;
+        }
+        std::cout << "\n}\n";
     }
 
     int run_main(int argc, char *argv[])

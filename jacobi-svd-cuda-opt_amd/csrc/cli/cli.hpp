@@ -55,6 +55,12 @@ namespace jacobi::svd::cli
         // and the column block width of large matrices (--block-width 0|16|32).
         JacobiPrecision precision = JacobiPrecision::fp64;
         int block_width = 0;
+        // No reference counterpart: per-testcase error metrics in the report
+        // (--report-errors), against reference singular values when
+        // --reference-sigma names a stream of 1 x n rows (implies
+        // --report-errors).
+        bool report_errors = false;
+        std::filesystem::path reference_sigma_path;
     };
 
     struct ParseResult final
@@ -68,7 +74,8 @@ namespace jacobi::svd::cli
         input, output, input_format, output_format, format, epsilon, max_sweeps, threads_per_block,
         layout_transpose_mode, layout_transpose_min_columns, layout_transpose_min_elements,
         layout_transpose_auto_tune, layout_transpose_benchmark_repetitions, layout_transpose_benchmark_sweeps,
-        queue_capacity, force, dry_run, print_config, json_report, quiet, help, version, precision, block_width
+        queue_capacity, force, dry_run, print_config, json_report, quiet, help, version, precision, block_width,
+        report_errors, reference_sigma
     };
 
     struct OptionDefinition final

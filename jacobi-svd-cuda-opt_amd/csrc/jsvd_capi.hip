@@ -345,6 +345,12 @@ struct jsvd_plan
     DevBuf<int32_t> bk_flags;
     int32_t *h_flag = nullptr;
     int64_t a_elems = 0, v_elems = 0, s_elems = 0;
+    // Error metrics (jsvd_plan_metrics_f64): tile list, per-matrix tile
+    // ranges and partial sums, built on the first metrics call.
+    DevBuf<jsvd::MetricTile> mt_tiles;
+    DevBuf<int64_t> mt_first;
+    DevBuf<double> mt_partial;
+    int64_t mt_count = -1;
 
     ~jsvd_plan()
     {
@@ -402,7 +408,7 @@ namespace
     // rotation (one 4-byte read-back per sweep, as the reference's sweep
     // loop does, src/domain/jacobi_svd.cu:156-185).
     jsvd_status plan_blocked(jsvd_plan &p, int64_t k, const double *d_a, double *d_u, double *d_sigma, double *d_v,
-                             int32_t *d_sweeps, hipStream_t stream)
+                             int32_t *d_sweeps, int32_t *d_stop, hipStream_t stream)
     {
         const int64_t m = p.m[k];
         const int n = static_cast<int>(p.n[k]);
@@ -427,7 +433,7 @@ namespace
         args.pair_rotated = p.bk_flags.ptr;
         args.sweep_flag = p.bk_flags.ptr + sh.pairs;
         const int rounds = sh.nb - 1;
-        int executed = 0;
+        int executed = 0, clean = 0;
         for (int sweep = 0; sweep < p.cfg.max_sweeps; ++sweep)
         {
             JSVD_HIP(jsvd::launch_fill_i32(args.sweep_flag, 0, 1, stream));
@@ -437,18 +443,23 @@ namespace
             JSVD_HIP(hipStreamSynchronize(stream));
             executed = sweep + 1;
             if (*p.h_flag == 0)
+            {
+                clean = 1;
                 break;
+            }
         }
         if (d_sweeps)
             JSVD_HIP(jsvd::launch_fill_i32(d_sweeps + k, executed, 1, stream));
+        if (d_stop)
+            JSVD_HIP(jsvd::launch_fill_i32(d_stop + k, clean ? 0 : 1, 1, stream));
         return finish_large(p, k, m, n, sh.lda, sh.nv, d_u, d_sigma, d_v, stream);
     }
 
     jsvd_status plan_large(jsvd_plan &p, int64_t k, const double *d_a, double *d_u, double *d_sigma, double *d_v,
-                           int32_t *d_sweeps, hipStream_t stream)
+                           int32_t *d_sweeps, int32_t *d_stop, hipStream_t stream)
     {
         if (uses_blocked(p.cfg, p.n[k]))
-            return plan_blocked(p, k, d_a, d_u, d_sigma, d_v, d_sweeps, stream);
+            return plan_blocked(p, k, d_a, d_u, d_sigma, d_v, d_sweeps, d_stop, stream);
         const int64_t m = p.m[k];
         const int n = static_cast<int>(p.n[k]);
         const double *a = d_a + p.a_off[k];
@@ -467,7 +478,7 @@ namespace
         const int grid = std::max(1, std::min(resident, half));
         JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
                                                     p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
-                                                    stream));
+                                                    d_stop ? d_stop + k : nullptr, stream));
         return finish_large(p, k, m, n, m, n, d_u, d_sigma, d_v, stream);
     }
 
@@ -509,7 +520,7 @@ namespace
     // column-major workspace, V <- I, the grouped sweep launches, then sigma,
     // U = A / sigma and V back to the caller's row-major layout.
     jsvd_status plan_wave(jsvd_plan &p, const double *d_a, double *d_u, double *d_sigma, double *d_v,
-                          int32_t *d_sweeps, hipStream_t stream)
+                          int32_t *d_sweeps, int32_t *d_stop, hipStream_t stream)
     {
         const int64_t entries = static_cast<int64_t>(p.wave.size());
         JSVD_HIP(jsvd::launch_fill_i32(p.wv_flags.ptr, 0, entries * p.cfg.max_sweeps, stream));
@@ -532,6 +543,7 @@ namespace
             args.max_sweeps = p.cfg.max_sweeps;
             args.bars = p.wv_bars.ptr;
             args.sweeps = d_sweeps;
+            args.stop = d_stop;
             if (l.members == 0)
                 JSVD_HIP(jsvd::launch_wave_block_sweeps_f64(l.m_max, l.count, args, stream));
             else
@@ -1070,6 +1082,12 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
 jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, double *d_sigma, double *d_v,
                                   int32_t *d_sweeps, void *stream_ptr)
 {
+    return jsvd_plan_execute_status_f64(p, d_a, d_u, d_sigma, d_v, d_sweeps, nullptr, stream_ptr);
+}
+
+jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double *d_u, double *d_sigma,
+                                         double *d_v, int32_t *d_sweeps, int32_t *d_stop, void *stream_ptr)
+{
     g_last_error.clear();
     discard_stale_error();
     if (!p)
@@ -1125,6 +1143,7 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
             args.sigma = d_sigma;
             args.v = d_v;
             args.sweeps = d_sweeps;
+            args.stop = d_stop;
             args.eps = p->cfg.epsilon;
             args.max_sweeps = p->cfg.max_sweeps;
             args.vlog = bk.vlog;
@@ -1144,8 +1163,8 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
         }
         else
             JSVD_HIP(jsvd::launch_lds_batch_f64(bk.block, bk.grid, bk.lds_bytes, items, static_cast<int>(bk.count),
-                                                d_a, d_u, d_sigma, d_v, d_sweeps, p->cfg.epsilon, p->cfg.max_sweeps,
-                                                ctr, st));
+                                                d_a, d_u, d_sigma, d_v, d_sweeps, d_stop, p->cfg.epsilon,
+                                                p->cfg.max_sweeps, ctr, st));
     }
     for (size_t i = 0; i < p->aux.size(); ++i)
     {
@@ -1161,12 +1180,60 @@ jsvd_status jsvd_plan_execute_f64(jsvd_plan *p, const double *d_a, double *d_u, 
         JSVD_HIP(hipEventRecord(p->big_in, stream));
         JSVD_HIP(hipStreamWaitEvent(bs, p->big_in, 0));
         if (!p->wave.empty())
-            JSVD_CHECK(plan_wave(*p, d_a, d_u, d_sigma, d_v, d_sweeps, bs));
+            JSVD_CHECK(plan_wave(*p, d_a, d_u, d_sigma, d_v, d_sweeps, d_stop, bs));
         for (int64_t k : p->large)
-            JSVD_CHECK(plan_large(*p, k, d_a, d_u, d_sigma, d_v, d_sweeps, bs));
+            JSVD_CHECK(plan_large(*p, k, d_a, d_u, d_sigma, d_v, d_sweeps, d_stop, bs));
         JSVD_HIP(hipEventRecord(p->big_out, bs));
         JSVD_HIP(hipStreamWaitEvent(stream, p->big_out, 0));
     }
+    return JSVD_OK;
+}
+
+jsvd_status jsvd_plan_metrics_f64(jsvd_plan *p, const double *d_a, const double *d_u, const double *d_sigma,
+                                  const double *d_v, double *d_out, void *stream_ptr)
+{
+    g_last_error.clear();
+    discard_stale_error();
+    if (!p)
+        return fail(JSVD_EINVAL, "plan is null.");
+    if (p->count == 0)
+        return JSVD_OK;
+    if (!d_a || !d_u || !d_sigma || !d_v || !d_out)
+        return fail(JSVD_EINVAL, "device buffer pointer is null.");
+    int dev = 0;
+    JSVD_HIP(hipGetDevice(&dev));
+    if (dev != p->device)
+        return fail(JSVD_EINVAL, "plan was created on another device.");
+    if (p->mt_count < 0)
+    {
+        // 32 x 32 output tiles per matrix: U^T U and V^T V (n x n each), then
+        // the m x n reconstruction.
+        std::vector<jsvd::MetricTile> tiles;
+        std::vector<int64_t> first(p->count + 1, 0);
+        for (int64_t k = 0; k < p->count; ++k)
+        {
+            first[k] = static_cast<int64_t>(tiles.size());
+            const int tn = static_cast<int>((p->n[k] + 31) / 32), tm = static_cast<int>((p->m[k] + 31) / 32);
+            for (int kind = 0; kind < 3; ++kind)
+                for (int ti = 0; ti < (kind == 2 ? tm : tn); ++ti)
+                    for (int tj = 0; tj < tn; ++tj)
+                        tiles.push_back({static_cast<int32_t>(k), kind, ti, tj});
+        }
+        first[p->count] = static_cast<int64_t>(tiles.size());
+        if (tiles.size() > size_t(0x7fffffff))
+            return fail(JSVD_EINVAL, "metrics: the batch needs more than 2^31 - 1 output tiles.");
+        JSVD_CHECK(p->mt_tiles.alloc(tiles.size()));
+        JSVD_CHECK(p->mt_first.alloc(first.size()));
+        JSVD_CHECK(p->mt_partial.alloc(2 * std::max<size_t>(tiles.size(), 1)));
+        if (!tiles.empty())
+            JSVD_HIP(hipMemcpy(p->mt_tiles.ptr, tiles.data(), sizeof(jsvd::MetricTile) * tiles.size(),
+                               hipMemcpyHostToDevice));
+        JSVD_HIP(hipMemcpy(p->mt_first.ptr, first.data(), sizeof(int64_t) * first.size(), hipMemcpyHostToDevice));
+        p->mt_count = static_cast<int64_t>(tiles.size());
+    }
+    JSVD_HIP(jsvd::launch_metrics_f64(p->mt_tiles.ptr, p->mt_count, p->mt_first.ptr, p->d_all.ptr,
+                                      static_cast<int>(p->count), d_a, d_u, d_sigma, d_v, p->cfg.epsilon,
+                                      p->mt_partial.ptr, d_out, static_cast<hipStream_t>(stream_ptr)));
     return JSVD_OK;
 }
 
@@ -1249,6 +1316,14 @@ jsvd_status jsvd_solve_batched_f64(int64_t count, const double *const *a_rm, con
                                    const jsvd_config *cfg, double *const *u_rm, double *const *sigma,
                                    double *const *v_rm, int32_t *sweeps)
 {
+    return jsvd_solve_batched_status_f64(count, a_rm, m, n, cfg, u_rm, sigma, v_rm, sweeps, nullptr);
+}
+
+jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_rm, const uint64_t *m,
+                                          const uint64_t *n, const jsvd_config *cfg, double *const *u_rm,
+                                          double *const *sigma, double *const *v_rm, int32_t *sweeps,
+                                          int32_t *stop)
+{
     g_last_error.clear();
     if (count < 0 || (count > 0 && (!a_rm || !m || !n || !u_rm || !sigma || !v_rm)))
         return fail(JSVD_EINVAL, "Input batch description is invalid.");
@@ -1282,12 +1357,13 @@ jsvd_status jsvd_solve_batched_f64(int64_t count, const double *const *a_rm, con
     JSVD_CHECK(ctx->u.reserve(plan->a_elems));
     JSVD_CHECK(ctx->s.reserve(plan->s_elems));
     JSVD_CHECK(ctx->v.reserve(plan->v_elems));
-    JSVD_CHECK(ctx->sweeps.reserve(count));
+    JSVD_CHECK(ctx->sweeps.reserve(2 * count));
     hipStream_t st = ctx->stream;
     for (int64_t k = 0; k < count; ++k)
         JSVD_HIP(hipMemcpyAsync(ctx->a.ptr + plan->a_off[k], a_rm[k], sizeof(double) * mm[k] * nn[k],
                                 hipMemcpyHostToDevice, st));
-    JSVD_CHECK(jsvd_plan_execute_f64(plan, ctx->a.ptr, ctx->u.ptr, ctx->s.ptr, ctx->v.ptr, ctx->sweeps.ptr, st));
+    JSVD_CHECK(jsvd_plan_execute_status_f64(plan, ctx->a.ptr, ctx->u.ptr, ctx->s.ptr, ctx->v.ptr, ctx->sweeps.ptr,
+                                            ctx->sweeps.ptr + count, st));
     for (int64_t k = 0; k < count; ++k)
     {
         JSVD_HIP(hipMemcpyAsync(u_rm[k], ctx->u.ptr + plan->a_off[k], sizeof(double) * mm[k] * nn[k],
@@ -1297,11 +1373,13 @@ jsvd_status jsvd_solve_batched_f64(int64_t count, const double *const *a_rm, con
         JSVD_HIP(hipMemcpyAsync(v_rm[k], ctx->v.ptr + plan->v_off[k], sizeof(double) * nn[k] * nn[k],
                                 hipMemcpyDeviceToHost, st));
     }
-    std::vector<int32_t> sw(count, 0);
-    JSVD_HIP(hipMemcpyAsync(sw.data(), ctx->sweeps.ptr, sizeof(int32_t) * count, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> sw(2 * count, 0);
+    JSVD_HIP(hipMemcpyAsync(sw.data(), ctx->sweeps.ptr, sizeof(int32_t) * 2 * count, hipMemcpyDeviceToHost, st));
     JSVD_HIP(hipStreamSynchronize(st));
     if (sweeps)
-        std::copy(sw.begin(), sw.end(), sweeps);
+        std::copy(sw.begin(), sw.begin() + count, sweeps);
+    if (stop)
+        std::copy(sw.begin() + count, sw.end(), stop);
     return JSVD_OK;
 }
 

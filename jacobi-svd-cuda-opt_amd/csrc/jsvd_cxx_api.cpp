@@ -82,9 +82,12 @@ namespace jacobi::svd
         result.sigma.assign(columns, 0.0);
         result.v.assign(columns * columns, 0.0);
         const jsvd_config cfg = to_c(config);
-        int32_t sweeps = 0;
-        rethrow(jsvd_solve_f64(host_input.data(), rows, columns, &cfg, result.u.data(), result.sigma.data(),
-                               result.v.data(), &sweeps));
+        int32_t sweeps = 0, stop = JSVD_STOP_CONVERGED;
+        const double *a = host_input.data();
+        double *u = result.u.data(), *s = result.sigma.data(), *v = result.v.data();
+        const uint64_t m64 = rows, n64 = columns;
+        rethrow(jsvd_solve_batched_status_f64(1, &a, &m64, &n64, &cfg, &u, &s, &v, &sweeps, &stop));
+        result.converged = stop == JSVD_STOP_CONVERGED;
         result.sweeps = sweeps;
         return result;
     }

@@ -35,9 +35,12 @@ namespace jsvd
     // kernels' static LDS and the 16-byte alignment of the dynamic block.
     constexpr int64_t kMaxLdsBytes = 160 * 1024 - 1024;
 
+    // stop (optional): per matrix, 0 when the last sweep rotated no pair
+    // (converged), 1 when max_sweeps sweeps ran and the last still rotated.
     hipError_t launch_lds_batch_f64(int block, int grid, int lds_bytes, const WorkItem *items, int count,
                                     const double *a, double *u, double *sigma, double *v, int32_t *sweeps,
-                                    double eps, int max_sweeps, unsigned int *counters, hipStream_t stream);
+                                    int32_t *stop, double eps, int max_sweeps, unsigned int *counters,
+                                    hipStream_t stream);
     const void *lds_batch_f64_symbol(int block);
 
     // Arguments of one launch of the shape-class work-list kernel.
@@ -52,6 +55,7 @@ namespace jsvd
         double *sigma;
         double *v;
         int32_t *sweeps;
+        int32_t *stop;        // optional: 0 converged, 1 stopped at max_sweeps (slot w.index)
         int add_sweeps;       // 1: add this solve's sweeps to the slot instead of storing them
         double eps;
         int max_sweeps;
@@ -112,10 +116,28 @@ namespace jsvd
                                    const float *v, const float *gu, const float *gs, const float *gv, float *ga,
                                    hipStream_t stream);
 
+    // Error metrics of computed SVDs (packed batch, row-major operands): per
+    // matrix ||U^T U - D||_F (D = diag(sigma > eps)), ||V^T V - I||_F and
+    // ||A - U diag(sigma) V^T||_F / ||A||_F, three doubles per matrix in
+    // `out`.  The work is cut into 32 x 32 output tiles (kind 0: U^T U,
+    // 1: V^T V, 2: the reconstruction), listed per matrix in that order;
+    // first[k] .. first[k+1]-1 are matrix k's tiles and `partial` holds two
+    // doubles per tile.
+    struct MetricTile
+    {
+        int32_t matrix; // index into the batch-order work-list
+        int32_t kind;
+        int32_t ti, tj; // tile row and column
+    };
+    hipError_t launch_metrics_f64(const MetricTile *tiles, int64_t tile_count, const int64_t *first,
+                                  const WorkItem *mats, int count, const double *a, const double *u,
+                                  const double *sigma, const double *v, double eps, double *partial, double *out,
+                                  hipStream_t stream);
+
     // Persistent plain-FP64 sweep kernel over one column-major matrix.
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
-                                            int32_t *sweeps_out, hipStream_t stream);
+                                            int32_t *sweeps_out, int32_t *stop_out, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
     // Matrices too large for LDS with m up to wave_sweeps_max_rows(): one
     // wave per column pair (workgroups of 8 or 16 waves), solved by groups of
@@ -140,6 +162,7 @@ namespace jsvd
         int max_sweeps;
         unsigned int *bars;     // one flag word per workgroup (members per group), zero at launch
         int32_t *sweeps;        // optional
+        int32_t *stop;          // optional: 0 converged, 1 stopped at max_sweeps
     };
     int wave_sweeps_max_rows();
     // Kernel instance for matrices of up to m rows (all matrices of one

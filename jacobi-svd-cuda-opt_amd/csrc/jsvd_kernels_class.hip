@@ -241,7 +241,7 @@ namespace jsvd
         // applied to A bit for bit.  Returns the number of sweeps.
         template <int BS, int G, int R, int VR>
         __device__ __forceinline__ int sweeps_split_f64(int n, int lda, int ldv, double *A, double *V, double *T,
-                                                        const ClassArgs &args)
+                                                        const ClassArgs &args, int &clean_out)
         {
             constexpr int HALF_BS = BS / 2;
             const int tid = threadIdx.x;
@@ -328,7 +328,7 @@ namespace jsvd
                 dirty = (dirty & (fresh ^ 1)) | rot;
                 advance(p, q);
             };
-            int executed = 0, gr = 0;
+            int executed = 0, gr = 0, clean = 0;
             for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
             {
                 int any = 0;
@@ -357,7 +357,10 @@ namespace jsvd
                 }
                 executed = sweep + 1;
                 if (!__syncthreads_or(any))
+                {
+                    clean = 1;
                     break;
+                }
             }
             // The V waves' last round, then every held top column to LDS.
             if (role == 1 && k < half && gr > 0)
@@ -371,6 +374,7 @@ namespace jsvd
                     store_v(V2 + __umul24(ct, ldv2));
             }
             __syncthreads();
+            clean_out = clean;
             return executed;
         }
 
@@ -456,7 +460,7 @@ namespace jsvd
             int z = even - 1; // bottom[0]'s column this round
             int gs = 0;       // global round mod ns
             double dummy_t;
-            int executed = 0;
+            int executed = 0, clean = 0;
             for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
             {
                 int any = 0;
@@ -509,7 +513,10 @@ namespace jsvd
                 }
                 executed = sweep + 1;
                 if (!__syncthreads_or(any))
+                {
+                    clean = 1;
                     break;
+                }
             }
 
             // Epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
@@ -628,6 +635,8 @@ namespace jsvd
                 else
                     args.sweeps[w.index] = executed;
             }
+            if (tid == 0 && args.stop)
+                args.stop[w.index] = clean ? 0 : 1;
         }
 
         // Inlined into the kernel: as a call, the kernel-argument struct it
@@ -703,11 +712,11 @@ namespace jsvd
             // top column turns into bottom[half-1]) stores that column and,
             // at position 1, loads the new top (the previous bottom[0]).
             // Column indices >= n are the dummy column of an odd n.
-            int executed = 0;
+            int executed = 0, clean = 0;
             if constexpr (SPLIT)
             {
                 static_assert(!VLOG && PASSES == 1, "split roles carry V in LDS, one pair per group");
-                executed = sweeps_split_f64<BS, G, R, VR>(n, lda, ldv, A, V, S + n, args);
+                executed = sweeps_split_f64<BS, G, R, VR>(n, lda, ldv, A, V, S + n, args, clean);
             }
             else
             {
@@ -783,7 +792,10 @@ namespace jsvd
                     }
                     executed = sweep + 1;
                     if (!__syncthreads_or(any))
+                    {
+                        clean = 1;
                         break;
+                    }
                 }
                 // Held top columns back to LDS (those loaded and changed since).
 #pragma unroll
@@ -910,6 +922,8 @@ namespace jsvd
                 else
                     args.sweeps[w.index] = executed;
             }
+            if (tid == 0 && args.stop)
+                args.stop[w.index] = clean ? 0 : 1;
         }
 
         // Runs solve_class_f64 instantiated for the matrix's exact numbers of

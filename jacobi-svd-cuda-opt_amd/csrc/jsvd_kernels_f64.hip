@@ -123,7 +123,8 @@ namespace jsvd
         template <int BS, int RMAX>
         __device__ void solve_lds_f64(const WorkItem &w, double *lds, const double *__restrict__ a,
                                       double *__restrict__ u, double *__restrict__ sigma, double *__restrict__ v,
-                                      int32_t *__restrict__ sweeps, double eps, int max_sweeps)
+                                      int32_t *__restrict__ sweeps, int32_t *__restrict__ stop, double eps,
+                                      int max_sweeps)
         {
             const int m = w.m, n = w.n, tid = threadIdx.x;
             double *A = lds;                         // column-major, leading dimension m
@@ -149,7 +150,7 @@ namespace jsvd
             const int groups = BS / G;
             const int gid = tid / G, lane = tid & (G - 1);
 
-            int executed = 0;
+            int executed = 0, clean = 0;
             for (int sweep = 0; sweep < max_sweeps; ++sweep)
             {
                 int any = 0;
@@ -167,7 +168,10 @@ namespace jsvd
                 }
                 executed = sweep + 1;
                 if (!__syncthreads_or(any))
+                {
+                    clean = 1;
                     break;
+                }
             }
 
             // U / Sigma epilogue (reference src/domain/jacobi_rotation_kernels.cuh:370-413):
@@ -204,6 +208,8 @@ namespace jsvd
             }
             if (tid == 0 && sweeps)
                 sweeps[w.index] = executed;
+            if (tid == 0 && stop)
+                stop[w.index] = clean ? 0 : 1;
         }
 
         // counters[0]: next work-list entry; counters[1]: workgroups finished.
@@ -213,7 +219,8 @@ namespace jsvd
         __global__ __launch_bounds__(BS) void lds_batch_f64(const WorkItem *__restrict__ items, int count,
                                                             const double *__restrict__ a, double *__restrict__ u,
                                                             double *__restrict__ sigma, double *__restrict__ v,
-                                                            int32_t *__restrict__ sweeps, double eps, int max_sweeps,
+                                                            int32_t *__restrict__ sweeps, int32_t *__restrict__ stop,
+                                                            double eps, int max_sweeps,
                                                             unsigned int *__restrict__ counters)
         {
             extern __shared__ double lds[];
@@ -227,7 +234,7 @@ namespace jsvd
                 if (item >= count)
                     break;
                 const WorkItem w = items[item];
-                solve_lds_f64<BS, RMAX>(w, lds, a, u, sigma, v, sweeps, eps, max_sweeps);
+                solve_lds_f64<BS, RMAX>(w, lds, a, u, sigma, v, sweeps, stop, eps, max_sweeps);
                 __syncthreads();
             }
             if (threadIdx.x == 0)
@@ -250,7 +257,8 @@ namespace jsvd
                                                                     int64_t m, int n, double eps, int max_sweeps,
                                                                     int32_t *__restrict__ flags,
                                                                     unsigned int *__restrict__ bar,
-                                                                    int32_t *__restrict__ sweeps_out)
+                                                                    int32_t *__restrict__ sweeps_out,
+                                                                    int32_t *__restrict__ stop_out)
         {
             constexpr int WAVES = BS / 64;
             __shared__ double red[3][WAVES];
@@ -259,7 +267,7 @@ namespace jsvd
             const int half = even >> 1;
             const int rounds = n < 2 ? 0 : even - 1;
             const bool in_regs = m <= (int64_t)BS * RMAX;
-            int executed = 0;
+            int executed = 0, clean = 0;
             for (int sweep = 0; sweep < max_sweeps; ++sweep)
             {
                 int any = 0;
@@ -367,10 +375,15 @@ namespace jsvd
                 executed = sweep + 1;
                 const int f = __hip_atomic_load(&flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (f == 0)
+                {
+                    clean = 1;
                     break;
+                }
             }
             if (blockIdx.x == 0 && tid == 0 && sweeps_out)
                 *sweeps_out = executed;
+            if (blockIdx.x == 0 && tid == 0 && stop_out)
+                *stop_out = clean ? 0 : 1;
         }
 
         // One column pair of a column-major m x n matrix (m <= 64 RW, n <= m)
@@ -546,7 +559,7 @@ namespace jsvd
                 const int n = w.n;
                 const int even = n + (n & 1), half = even >> 1;
                 const int rounds = n < 2 ? 0 : even - 1;
-                int executed = 0;
+                int executed = 0, clean = 0;
                 for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
                 {
                     int any = 0;
@@ -570,10 +583,15 @@ namespace jsvd
                     executed = sweep + 1;
                     const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (f == 0)
+                    {
+                        clean = 1;
                         break;
+                    }
                 }
                 if (member == 0 && threadIdx.x == 0 && g.sweeps)
                     g.sweeps[w.index] = executed;
+                if (member == 0 && threadIdx.x == 0 && g.stop)
+                    g.stop[w.index] = clean ? 0 : 1;
             }
         }
 
@@ -598,7 +616,7 @@ namespace jsvd
                 const int n = w.n;
                 const int even = n + (n & 1), half = even >> 1;
                 const int rounds = n < 2 ? 0 : even - 1;
-                int executed = 0;
+                int executed = 0, clean = 0;
                 for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
                 {
                     int any = 0;
@@ -616,10 +634,15 @@ namespace jsvd
                     }
                     executed = sweep + 1;
                     if (!__syncthreads_or(any))
+                    {
+                        clean = 1;
                         break;
+                    }
                 }
                 if (threadIdx.x == 0 && g.sweeps)
                     g.sweeps[w.index] = executed;
+                if (threadIdx.x == 0 && g.stop)
+                    g.stop[w.index] = clean ? 0 : 1;
             }
         }
 
@@ -635,7 +658,7 @@ namespace jsvd
                 const int n = w.n;
                 const int even = n + (n & 1), half = even >> 1;
                 const int rounds = n < 2 ? 0 : even - 1;
-                int executed = 0;
+                int executed = 0, clean = 0;
                 for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
                 {
                     int any = 0;
@@ -653,10 +676,15 @@ namespace jsvd
                     }
                     executed = sweep + 1;
                     if (!__syncthreads_or(any))
+                    {
+                        clean = 1;
                         break;
+                    }
                 }
                 if (threadIdx.x == 0 && g.sweeps)
                     g.sweeps[w.index] = executed;
+                if (threadIdx.x == 0 && g.stop)
+                    g.stop[w.index] = clean ? 0 : 1;
             }
         }
 
@@ -756,26 +784,21 @@ namespace jsvd
 
     hipError_t launch_lds_batch_f64(int block, int grid, int lds_bytes, const WorkItem *items, int count,
                                     const double *a, double *u, double *sigma, double *v, int32_t *sweeps,
-                                    double eps, int max_sweeps, unsigned int *counters, hipStream_t stream)
+                                    int32_t *stop, double eps, int max_sweeps, unsigned int *counters,
+                                    hipStream_t stream)
     {
         switch (block)
         {
-        case 64:
-            lds_batch_f64<64, kLdsRmax><<<grid, 64, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
-                                                                         max_sweeps, counters);
-            break;
-        case 128:
-            lds_batch_f64<128, kLdsRmax><<<grid, 128, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
-                                                                           max_sweeps, counters);
-            break;
-        case 256:
-            lds_batch_f64<256, kLdsRmax><<<grid, 256, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
-                                                                           max_sweeps, counters);
-            break;
-        case 512:
-            lds_batch_f64<512, kLdsRmax><<<grid, 512, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, eps,
-                                                                           max_sweeps, counters);
-            break;
+#define JSVD_LDS_BATCH_CASE(BS)                                                                                     \
+    case BS:                                                                                                       \
+        lds_batch_f64<BS, kLdsRmax><<<grid, BS, lds_bytes, stream>>>(items, count, a, u, sigma, v, sweeps, stop, eps, \
+                                                                     max_sweeps, counters);                       \
+        break;
+            JSVD_LDS_BATCH_CASE(64)
+            JSVD_LDS_BATCH_CASE(128)
+            JSVD_LDS_BATCH_CASE(256)
+            JSVD_LDS_BATCH_CASE(512)
+#undef JSVD_LDS_BATCH_CASE
         default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
@@ -887,22 +910,22 @@ namespace jsvd
 
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
-                                            int32_t *sweeps_out, hipStream_t stream)
+                                            int32_t *sweeps_out, int32_t *stop_out, hipStream_t stream)
     {
         switch (block)
         {
         case 256:
             persistent_sweeps_f64<256, kPersistentRmax><<<grid, 256, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps,
-                                                                                flags, barrier, sweeps_out);
+                                                                                flags, barrier, sweeps_out, stop_out);
             break;
         case 512:
             persistent_sweeps_f64<512, kPersistentRmax><<<grid, 512, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps,
-                                                                                flags, barrier, sweeps_out);
+                                                                                flags, barrier, sweeps_out, stop_out);
             break;
         case 1024:
             persistent_sweeps_f64<1024, kPersistentRmax><<<grid, 1024, 0, stream>>>(a_cm, v_cm, m, n, eps,
                                                                                   max_sweeps, flags, barrier,
-                                                                                  sweeps_out);
+                                                                                  sweeps_out, stop_out);
             break;
         default: return hipErrorInvalidValue;
         }

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cmath>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -146,9 +147,10 @@ namespace jacobi::svd::pipeline
             std::vector<int64_t> m, n;
             std::size_t a_elems = 0, v_elems = 0, s_elems = 0;
             bool capturable = false;
-            Pinned<double> h_in, h_u, h_s, h_v;
-            Pinned<int32_t> h_sw;
-            Device<double> d_a, d_u, d_s, d_v;
+            bool metrics = false; // error metrics requested (PipelineConfig::report_errors)
+            Pinned<double> h_in, h_u, h_s, h_v, h_met;
+            Pinned<int32_t> h_sw;  // sweeps, then the stop reasons
+            Device<double> d_a, d_u, d_s, d_v, d_met;
             Device<int32_t> d_sw;
             bool busy = false;
 
@@ -181,16 +183,24 @@ namespace jacobi::svd::pipeline
             {
                 hip_check(hipMemcpyAsync(d_a.ptr, h_in.ptr, a_elems * sizeof(double), hipMemcpyHostToDevice, stream),
                           "hipMemcpyAsync(H2D A)");
-                rethrow(jsvd_plan_execute_f64(plan, d_a.ptr, d_u.ptr, d_s.ptr, d_v.ptr, d_sw.ptr, stream));
+                const std::size_t count = m.size();
+                rethrow(jsvd_plan_execute_status_f64(plan, d_a.ptr, d_u.ptr, d_s.ptr, d_v.ptr, d_sw.ptr,
+                                                     d_sw.ptr + count, stream));
+                if (metrics)
+                    rethrow(jsvd_plan_metrics_f64(plan, d_a.ptr, d_u.ptr, d_s.ptr, d_v.ptr, d_met.ptr, stream));
                 hip_check(hipMemcpyAsync(h_u.ptr, d_u.ptr, a_elems * sizeof(double), hipMemcpyDeviceToHost, stream),
                           "hipMemcpyAsync(D2H U)");
                 hip_check(hipMemcpyAsync(h_s.ptr, d_s.ptr, s_elems * sizeof(double), hipMemcpyDeviceToHost, stream),
                           "hipMemcpyAsync(D2H sigma)");
                 hip_check(hipMemcpyAsync(h_v.ptr, d_v.ptr, v_elems * sizeof(double), hipMemcpyDeviceToHost, stream),
                           "hipMemcpyAsync(D2H V)");
-                hip_check(hipMemcpyAsync(h_sw.ptr, d_sw.ptr, m.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                hip_check(hipMemcpyAsync(h_sw.ptr, d_sw.ptr, 2 * count * sizeof(int32_t), hipMemcpyDeviceToHost,
                                          stream),
                           "hipMemcpyAsync(D2H sweeps)");
+                if (metrics)
+                    hip_check(hipMemcpyAsync(h_met.ptr, d_met.ptr, 3 * count * sizeof(double), hipMemcpyDeviceToHost,
+                                             stream),
+                              "hipMemcpyAsync(D2H metrics)");
             }
 
             void submit(const std::vector<io::Matrix> &batch, std::size_t first_index, const jsvd_config &cfg)
@@ -219,20 +229,27 @@ namespace jacobi::svd::pipeline
                     a_elems = static_cast<std::size_t>(info.a_elements);
                     v_elems = static_cast<std::size_t>(info.v_elements);
                     s_elems = static_cast<std::size_t>(info.sigma_elements);
-                    // Cooperative launches of the large-matrix path stay out of graphs.
-                    capturable = info.large_matrices == 0;
+                    // The large-matrix paths stay out of graphs (the blocked one
+                    // reads its sweep flag on the host), and so does a metrics
+                    // run (its first call on a plan uploads the tile list).
+                    capturable = info.large_matrices == 0 && !metrics;
                 }
                 bool moved = false;
                 moved |= h_in.reserve(a_elems);
                 moved |= h_u.reserve(a_elems);
                 moved |= h_s.reserve(s_elems);
                 moved |= h_v.reserve(v_elems);
-                moved |= h_sw.reserve(m.size());
+                moved |= h_sw.reserve(2 * m.size());
                 moved |= d_a.reserve(a_elems);
                 moved |= d_u.reserve(a_elems);
                 moved |= d_s.reserve(s_elems);
                 moved |= d_v.reserve(v_elems);
-                moved |= d_sw.reserve(m.size());
+                moved |= d_sw.reserve(2 * m.size());
+                if (metrics)
+                {
+                    moved |= h_met.reserve(3 * m.size());
+                    moved |= d_met.reserve(3 * m.size());
+                }
                 if (moved && graph)
                 {
                     (void)hipGraphExecDestroy(graph);
@@ -274,11 +291,18 @@ namespace jacobi::svd::pipeline
                 busy = true;
             }
 
-            std::size_t drain(const std::function<void(const io::Matrix &)> &write)
+            // Writes the batch's results in input order and appends its
+            // per-testcase sweeps, stop reasons and (when requested) error
+            // metrics to the report.  `reference` reads the next testcase's
+            // reference sigma (1 x n), or is empty.
+            void drain(const std::function<void(const io::Matrix &)> &write, PipelineReport &report,
+                       const std::function<bool(io::Matrix &)> &reference)
             {
                 hip_check(hipEventSynchronize(done), "hipEventSynchronize");
-                std::size_t sweeps = 0, a = 0, v = 0, s = 0;
-                for (std::size_t k = 0; k < m.size(); ++k)
+                const std::size_t count = m.size();
+                std::size_t a = 0, v = 0, s = 0;
+                io::Matrix ref;
+                for (std::size_t k = 0; k < count; ++k)
                 {
                     const std::size_t rows = static_cast<std::size_t>(m[k]), cols = static_cast<std::size_t>(n[k]);
                     io::Matrix out;
@@ -289,16 +313,50 @@ namespace jacobi::svd::pipeline
                     out.rows = 1;
                     out.values.assign(h_s.ptr + s, h_s.ptr + s + cols);
                     write(out);
+                    if (metrics)
+                    {
+                        PipelineReport::MatrixErrors e;
+                        e.orth_u = h_met.ptr[3 * k];
+                        e.orth_v = h_met.ptr[3 * k + 1];
+                        e.residual = h_met.ptr[3 * k + 2];
+                        if (reference)
+                        {
+                            const std::size_t index = report.matrix_sweeps.size();
+                            if (!reference(ref))
+                                throw std::invalid_argument("Reference sigma stream ends before testcase[" +
+                                                            std::to_string(index) + "].");
+                            if (ref.values.size() != cols)
+                                throw std::invalid_argument("Reference sigma of testcase[" + std::to_string(index) +
+                                                            "] does not have " + std::to_string(cols) + " values.");
+                            std::vector<double> got(out.values), want(ref.values);
+                            std::sort(got.begin(), got.end(), std::greater<double>());
+                            std::sort(want.begin(), want.end(), std::greater<double>());
+                            double worst = 0.0, comp = 0.0;
+                            for (std::size_t j = 0; j < cols; ++j)
+                            {
+                                const double d = std::abs(got[j] - want[j]);
+                                worst = std::max(worst, d);
+                                if (want[j] > 0.0)
+                                    comp = std::max(comp, d / want[j]);
+                            }
+                            e.sigma_err_norm = want[0] > 0.0 ? worst / want[0] : worst;
+                            e.sigma_err_comp = comp;
+                        }
+                        report.matrix_errors.push_back(e);
+                    }
                     out.rows = cols;
                     out.values.assign(h_v.ptr + v, h_v.ptr + v + cols * cols);
                     write(out);
                     a += rows * cols;
                     s += cols;
                     v += cols * cols;
-                    sweeps += static_cast<std::size_t>(h_sw.ptr[k]);
+                    report.total_sweeps += static_cast<std::size_t>(h_sw.ptr[k]);
+                    report.matrix_sweeps.push_back(h_sw.ptr[k]);
+                    const bool converged = h_sw.ptr[count + k] == JSVD_STOP_CONVERGED;
+                    report.matrix_converged.push_back(converged);
+                    report.unconverged_count += converged ? 0 : 1;
                 }
                 busy = false;
-                return sweeps;
             }
         };
     } // namespace
@@ -346,6 +404,34 @@ namespace jacobi::svd::pipeline
             txt_in = std::make_unique<io::TxtReader>(config_.input_path);
             read = [&](io::Matrix &x) { return io::TxtFilePolicy::read_next(*txt_in, x); };
         }
+        // Reference sigma for the error report: a matrix stream of 1 x n rows
+        // in the input's formats (extension or the input format).
+        std::function<bool(io::Matrix &)> reference;
+        std::unique_ptr<io::MatReader> ref_mat;
+        std::unique_ptr<io::TxtReader> ref_txt;
+        if (config_.report_errors && !config_.reference_sigma_path.empty())
+        {
+            MatrixFileFormat rf = in_fmt;
+            try
+            {
+                rf = resolve(MatrixFileFormat::auto_detect, config_.reference_sigma_path);
+            }
+            catch (const std::invalid_argument &)
+            {
+                // No .mat / .txt extension: the input's format.
+            }
+            if (rf == MatrixFileFormat::mat)
+            {
+                ref_mat = std::make_unique<io::MatReader>(config_.reference_sigma_path);
+                reference = [&](io::Matrix &x) { return io::MatFilePolicy::read_next(*ref_mat, x); };
+            }
+            else
+            {
+                ref_txt = std::make_unique<io::TxtReader>(config_.reference_sigma_path);
+                reference = [&](io::Matrix &x) { return io::TxtFilePolicy::read_next(*ref_txt, x); };
+            }
+        }
+        report.errors_reported = config_.report_errors;
         std::function<void(const io::Matrix &)> write;
         std::function<void()> flush;
         std::unique_ptr<io::MatWriter> mat_out;
@@ -372,10 +458,13 @@ namespace jacobi::svd::pipeline
             if (batch.empty())
                 return;
             if (slots.size() < window)
+            {
                 slots.push_back(std::make_unique<Slot>());
+                slots.back()->metrics = config_.report_errors;
+            }
             Slot &slot = *slots[next_slot];
             if (slot.busy)
-                report.total_sweeps += slot.drain(write);
+                slot.drain(write, report, reference);
             slot.submit(batch, submitted, cfg);
             submitted += batch.size();
             next_slot = (next_slot + 1) % window;
@@ -397,7 +486,7 @@ namespace jacobi::svd::pipeline
         {
             Slot &slot = *slots[(next_slot + i) % slots.size()];
             if (slot.busy)
-                report.total_sweeps += slot.drain(write);
+                slot.drain(write, report, reference);
         }
         flush();
         report.testcase_count = submitted;

@@ -14,10 +14,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (JsvdConfig, JsvdError, JsvdInvalidArgument, LAYOUT_AUTO, LAYOUT_OFF, LAYOUT_ON,
-                   PRECISION_FP64, PRECISION_MIXED, default_config, load)
+                   PRECISION_FP64, PRECISION_MIXED, STOP_CONVERGED, STOP_MAX_SWEEPS, default_config, load)
 
 __all__ = [
     "JsvdConfig", "JsvdError", "JsvdInvalidArgument", "PRECISION_FP64", "PRECISION_MIXED", "LAYOUT_AUTO",
+    "STOP_CONVERGED", "STOP_MAX_SWEEPS",
     "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets",
     "autotune_batch", "tune_class", "solve_device_noalloc",
 ]
@@ -50,8 +51,10 @@ def svd(a, cfg: JsvdConfig | None = None, **overrides):
     return u, s, v, sw.value
 
 
-def svd_batched(mats, cfg: JsvdConfig | None = None, **overrides):
-    """Many independent solves from host memory in one work-list launch."""
+def svd_batched(mats, cfg: JsvdConfig | None = None, with_stop: bool = False, **overrides):
+    """Many independent solves from host memory in one work-list launch.
+    Returns [(U, sigma, V, sweeps)], or with_stop=True [(U, sigma, V, sweeps,
+    stop)] with stop = STOP_CONVERGED or STOP_MAX_SWEEPS."""
     lib = load()
     mats = [np.ascontiguousarray(x, dtype=np.float64) for x in mats]
     count = len(mats)
@@ -61,8 +64,12 @@ def svd_batched(mats, cfg: JsvdConfig | None = None, **overrides):
     ms = (ctypes.c_uint64 * count)(*[x.shape[0] for x in mats])
     ns = (ctypes.c_uint64 * count)(*[x.shape[1] for x in mats])
     sw = (ctypes.c_int32 * count)()
-    _lib.check(lib.jsvd_solve_batched_f64(count, arr(mats), ms, ns, ctypes.byref(c), arr([o[0] for o in outs]),
-                                          arr([o[1] for o in outs]), arr([o[2] for o in outs]), sw))
+    stop = (ctypes.c_int32 * count)()
+    _lib.check(lib.jsvd_solve_batched_status_f64(count, arr(mats), ms, ns, ctypes.byref(c),
+                                                 arr([o[0] for o in outs]), arr([o[1] for o in outs]),
+                                                 arr([o[2] for o in outs]), sw, stop))
+    if with_stop:
+        return [(o[0], o[1], o[2], int(sw[k]), int(stop[k])) for k, o in enumerate(outs)]
     return [(o[0], o[1], o[2], int(sw[k])) for k, o in enumerate(outs)]
 
 
@@ -100,7 +107,9 @@ class BatchPlan:
         _lib.check(load().jsvd_plan_get_info(self._handle, ctypes.byref(info)))
         return {k: getattr(info, k) for k, _ in info._fields_}
 
-    def execute(self, a, u, s, v, sweeps=None, stream=None):
+    def execute(self, a, u, s, v, sweeps=None, stream=None, stop=None):
+        """Solve on device tensors; ``stop`` (optional int32 CUDA tensor, one
+        slot per matrix) receives STOP_CONVERGED or STOP_MAX_SWEEPS."""
         import torch
 
         for t, nel in ((a, self.a_elems), (u, self.a_elems), (s, self.s_elems), (v, self.v_elems)):
@@ -109,10 +118,31 @@ class BatchPlan:
         if sweeps is not None and not (sweeps.is_cuda and sweeps.dtype == torch.int32
                                        and sweeps.numel() >= len(self.ms)):
             raise ValueError("sweeps must be an int32 CUDA tensor with one slot per matrix")
+        if stop is not None and not (stop.is_cuda and stop.dtype == torch.int32 and stop.numel() >= len(self.ms)):
+            raise ValueError("stop must be an int32 CUDA tensor with one slot per matrix")
         st = stream if stream is not None else torch.cuda.current_stream()
-        _lib.check(load().jsvd_plan_execute_f64(self._handle, a.data_ptr(), u.data_ptr(), s.data_ptr(),
-                                                v.data_ptr(), sweeps.data_ptr() if sweeps is not None else None,
-                                                st.cuda_stream))
+        ptr = lambda t: t.data_ptr() if t is not None else None
+        _lib.check(load().jsvd_plan_execute_status_f64(self._handle, a.data_ptr(), u.data_ptr(), s.data_ptr(),
+                                                       v.data_ptr(), ptr(sweeps), ptr(stop), st.cuda_stream))
+
+    def metrics(self, a, u, s, v, out=None, stream=None):
+        """Per-matrix error metrics on the device (``jsvd_plan_metrics_f64``):
+        a float64 CUDA tensor of shape (count, 3) holding ||U^T U - D||_F
+        (D = diag(sigma > epsilon)), ||V^T V - I||_F and the relative
+        residual ||A - U S V^T||_F / ||A||_F of every matrix."""
+        import torch
+
+        for t, nel in ((a, self.a_elems), (u, self.a_elems), (s, self.s_elems), (v, self.v_elems)):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() >= nel):
+                raise ValueError("BatchPlan.metrics needs contiguous float64 CUDA tensors of the packed sizes")
+        if out is None:
+            out = torch.empty((len(self.ms), 3), dtype=torch.float64, device=a.device)
+        elif not (out.is_cuda and out.dtype == torch.float64 and out.is_contiguous() and out.numel() >= 3 * len(self.ms)):
+            raise ValueError("out must be a contiguous float64 CUDA tensor with three slots per matrix")
+        st = stream if stream is not None else torch.cuda.current_stream()
+        _lib.check(load().jsvd_plan_metrics_f64(self._handle, a.data_ptr(), u.data_ptr(), s.data_ptr(), v.data_ptr(),
+                                                out.data_ptr(), st.cuda_stream))
+        return out
 
     def backward(self, u, s, v, gu, gs, gv, ga, stream=None):
         """SVD adjoint on device tensors of the packed layout (float64 or float32,

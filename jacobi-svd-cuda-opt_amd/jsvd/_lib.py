@@ -16,6 +16,7 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "jsvd_c.h")
 
 JSVD_OK, JSVD_EINVAL, JSVD_EDEVICE, JSVD_ENOMEM = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+STOP_CONVERGED, STOP_MAX_SWEEPS = 0, 1
 LAYOUT_AUTO, LAYOUT_ON, LAYOUT_OFF = 0, 1, 2
 
 
@@ -104,10 +105,15 @@ SIGNATURES = {
     "jsvd_solve_batched_f64": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_dp), _u64p, _u64p,
                                               ctypes.POINTER(JsvdConfig), ctypes.POINTER(_dp),
                                               ctypes.POINTER(_dp), ctypes.POINTER(_dp), _i32p]),
+    "jsvd_solve_batched_status_f64": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_dp), _u64p, _u64p,
+                                                     ctypes.POINTER(JsvdConfig), ctypes.POINTER(_dp),
+                                                     ctypes.POINTER(_dp), ctypes.POINTER(_dp), _i32p, _i32p]),
+    "jsvd_plan_execute_status_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jsvd_plan_create": (ctypes.c_int, [ctypes.c_int64, _i64p, _i64p, ctypes.POINTER(JsvdConfig),
                                         ctypes.POINTER(_vp)]),
     "jsvd_plan_execute_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jsvd_plan_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(JsvdPlanInfo)]),
+    "jsvd_plan_metrics_f64": (ctypes.c_int, [_vp] * 7),
     "jsvd_plan_destroy": (None, [_vp]),
     "jsvd_solve_device_noalloc_f64": (ctypes.c_int, [ctypes.c_int64, _i64p, _i64p, ctypes.POINTER(JsvdConfig),
                                                      _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -177,6 +177,21 @@ int run_checks()
         check(after.copy_to_host() == std::vector<double>({1, 2, 3, 4}), "device usable after OOM");
     }
 
+    // Stop reason (appended JacobiSvdResult::converged): one sweep cannot
+    // orthogonalise a random 30 x 30; thirty can.
+    {
+        const std::vector<double> a = matrix(30, 30, 7);
+        JacobiSvdConfig cfg;
+        cfg.epsilon = 1e-13;
+        cfg.max_sweeps = 1;
+        const JacobiSvdResult r1 = one_sided_jacobi_svd(a, 30, 30, cfg);
+        check(!r1.converged && r1.sweeps == 1, "max_sweeps = 1 reported as not converged");
+        cfg.max_sweeps = 30;
+        const JacobiSvdResult r30 = one_sided_jacobi_svd(a, 30, 30, cfg);
+        check(r30.converged && r30.sweeps < 30, "converged solve reported as converged");
+        check(rd.converged, "diagonal input converged");
+    }
+
     const LayoutTransposeAutoTuneReport rep = auto_tune_layout_transpose_threshold();
     check(rep.executed && rep.sample_count > 0 && rep.estimated_best_speedup >= 1.0, "autotune report");
 

@@ -116,3 +116,40 @@ def test_cli_rejects_wide_matrix_in_stream(tmp_path):
     r = _run([CLI, src, dst])
     assert r.returncode == 1
     assert "Execution failed: One-sided Jacobi SVD requires rows >= columns" in r.stderr
+
+
+def test_cli_report_errors_stop_reason_and_reference_sigma(tmp_path):
+    # The JSON report carries per-testcase sweeps, stop reasons and (with
+    # --report-errors / --reference-sigma) the error norms; with
+    # --max-sweeps 1 every testcase is reported as stopped at max_sweeps.
+    rng = np.random.default_rng(99)
+    mats = [rng.standard_normal((int(m), int(n))) for n, m in [(8, 12), (30, 45), (64, 128), (150, 300)]]
+    inp, out = tmp_path / "in.mat", tmp_path / "out.mat"
+    jio.write_mat(inp, mats)
+    refs = oracle.solve_batch(mats, 1e-12, 40)
+    ref_path = tmp_path / "ref.mat"
+    jio.write_mat(ref_path, [r[1].reshape(1, -1) for r in refs])
+    r = _run([CLI, inp, out, "-e", "1e-12", "-s", "40", "--reference-sigma", ref_path, "--json-report", "-q"])
+    assert r.returncode == 0, r.stderr
+    rep = json.loads(r.stdout)
+    assert rep["unconverged_count"] == 0 and rep["matrix_stop"] == ["converged"] * 4
+    assert [abs(s - ref[3]) <= 1 for s, ref in zip(rep["matrix_sweeps"], refs)] == [True] * 4
+    assert rep["total_sweeps"] == sum(rep["matrix_sweeps"])
+    # The reported norms are those of the written factors (numpy on the
+    # output file), and small: orthogonality within n * eps (each converged
+    # pair is orthogonal to eps), residual near the rounding level.
+    for a, (u, s, v), e in zip(mats, _triples(out, jio.read_mat), rep["matrix_errors"]):
+        want = (metrics.orthogonality(u, s > 1e-12), metrics.orthogonality(v), metrics.residual(a, u, s, v))
+        for name, w in zip(("orth_u", "orth_v", "residual"), want):
+            assert abs(e[name] - w) <= 1e-13 + 1e-6 * w, (a.shape, name, e[name], w)
+        assert e["orth_u"] < a.shape[1] * 1e-12 and e["orth_v"] < a.shape[1] * 1e-12 and e["residual"] < 1e-13
+        assert e["sigma_err_norm"] < 1e-12
+    r = _run([CLI, inp, out, "-y", "-s", "1", "--json-report", "-q"])
+    assert r.returncode == 0, r.stderr
+    rep = json.loads(r.stdout)
+    assert rep["unconverged_count"] == 4 and rep["matrix_stop"] == ["max_sweeps"] * 4
+    assert rep["matrix_sweeps"] == [1] * 4 and "matrix_errors" not in rep
+    bad = tmp_path / "short.mat"
+    jio.write_mat(bad, [refs[0][1].reshape(1, -1)])
+    r = _run([CLI, inp, out, "-y", "--reference-sigma", bad, "--json-report", "-q"])
+    assert r.returncode == 1 and "Reference sigma" in r.stderr
