@@ -15,6 +15,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <tuple>
 #include <vector>
 
 namespace
@@ -256,8 +257,9 @@ namespace
 } // namespace
 
 // Work-list counters per bucket: two per stage (FP32 sweeps,
-// orthonormalisation, B = A Q, FP64 sweeps); plain buckets use the last two.
-constexpr int kCountersPerBucket = 8;
+// orthonormalisation, B = A Q, FP64 sweeps, then the QR preconditioner's two
+// kernels); plain buckets use the FP64 pair.
+constexpr int kCountersPerBucket = 12;
 // Mixed-precision buckets are split into up to kMixedSplitParts parts of at
 // least kMixedSplitMin matrices, each on a stream of its own.
 constexpr size_t kMixedSplitMin = 64;
@@ -275,7 +277,7 @@ struct JsvdBucket
     int64_t first = 0;      // first work-list entry of the bucket
     int64_t count = 0;
     double work = 0.0;      // sum of n^2 (m + n), for launch order
-    int64_t max_n = 0;
+    int64_t max_n = 0, max_m = 0;
     int stream = 0;         // 0 = the caller's stream, k = auxiliary stream k-1
     int per_cu = 0;         // resident workgroups per CU
     // Rotation-log classes: one tangent log per resident workgroup.
@@ -287,6 +289,11 @@ struct JsvdBucket
     int f32cls = -1;
     int lds32 = 0, grid32 = 0;
     int lds_ortho = 0, grid_ortho = 0, grid_gemm = 0;
+    // Mixed precision on X = R^T of a pivoted QR (the stages above and the
+    // FP64 refinement then run on n x n matrices): the QR and back-transform
+    // launch shapes.
+    bool precond = false;
+    int qr_variant = -1, lds_qr = 0, grid_qr = 0, lds_fin = 0, grid_fin = 0, grid_polar = 0;
 };
 
 struct jsvd_plan
@@ -302,6 +309,13 @@ struct jsvd_plan
     DevBuf<unsigned int> d_counters; // kCountersPerBucket per bucket
     std::vector<std::unique_ptr<DevBuf<double>>> vlogs;
     DevBuf<double> q_scratch;        // mixed precision: packed column-major Q
+    // QR-preconditioned mixed buckets: the square work-list entries of X
+    // (m = n, A/U offsets = V offsets), in the order of `items`, and the
+    // scratch of the preconditioner.
+    std::vector<jsvd::WorkItem> pre_items;
+    DevBuf<jsvd::WorkItem> d_pre_items;
+    DevBuf<double> pc_refl, pc_tau, pc_x, pc_b, pc_ux, pc_vx;
+    DevBuf<int32_t> pc_perm;
     // Every matrix in batch order (the adjoint kernel's work-list) and the
     // largest LDS need of the adjoint in FP64 / FP32.
     DevBuf<jsvd::WorkItem> d_all;
@@ -769,9 +783,10 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     JSVD_HIP(hipGetDeviceProperties(&prop, plan->device));
     const int generic_block = lds_kernel_block(cfg.threads_per_block);
     // (class, workgroups per CU by LDS, FP32 class or -1) -> bucket
-    using GroupKey = std::pair<std::pair<int, int>, int>;
+    // (class, workgroups per CU by LDS, FP32 class or -1, QR kernel shape + 1 or 0)
+    using GroupKey = std::tuple<int, int, int, int>;
     std::vector<std::pair<GroupKey, std::vector<jsvd::WorkItem>>> groups;
-    bool any_mixed = false;
+    bool any_mixed = false, any_precond = false;
     static const int kSlots[] = {32, 24, 16, 12, 8, 6, 4, 3, 2, 1};
     bool any_four_wave = false;
     for (int64_t k = 0; k < count && !any_four_wave; ++k)
@@ -804,8 +819,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         ao += m[k] * n[k];
         vo += n[k] * n[k];
         so += n[k];
-        const int cls = preferred_class(m[k], n[k], any_four_wave);
-        const int64_t need = cls >= 0 ? jsvd::class_lds_bytes(cls, m[k], n[k]) : jsvd::lds_bytes_f64(m[k], n[k]);
+        int cls = preferred_class(m[k], n[k], any_four_wave);
+        int64_t need = cls >= 0 ? jsvd::class_lds_bytes(cls, m[k], n[k]) : jsvd::lds_bytes_f64(m[k], n[k]);
         if (need <= jsvd::kMaxLdsBytes)
         {
             jsvd::WorkItem w{};
@@ -824,15 +839,38 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                     slots = sl;
                     break;
                 }
-            // Mixed precision needs an FP32 class for the sweeps, the class
-            // kernel for the refinement and Q (n x n FP64) in LDS; other
-            // matrices of a mixed plan take the plain FP64 path.
-            int f32cls = -1;
+            // Mixed precision prefers the QR preconditioner (pivoted QR,
+            // then the FP32 sweeps, orthonormalisation, B = X Q and the FP64
+            // refinement on X = R^T, n x n); without it the stages run on A.
+            // Either needs an FP32 class, a refinement class keeping V in
+            // LDS and Q (n x n FP64) in LDS; other matrices of a mixed plan
+            // take the plain FP64 path.
+            int f32cls = -1, pre = 0;
             if (cfg.precision == JSVD_PRECISION_MIXED && cls >= 0 &&
                 jsvd::orthonormalize_lds_bytes(n[k]) <= jsvd::kMaxLdsBytes)
-                f32cls = jsvd::select_f32_class(m[k], n[k]);
+            {
+                const int rc = preferred_class(n[k], n[k], any_four_wave);
+                const int fc = jsvd::select_f32_class(n[k], n[k]);
+                if (jsvd::precond_fits(m[k], n[k]) && rc >= 0 && !jsvd::class_uses_vlog(rc) && fc >= 0)
+                {
+                    pre = 1 + jsvd::qrcp_variant(m[k], n[k]); // the QR kernel shape joins the bucket key
+                    f32cls = fc;
+                    cls = rc;
+                    need = jsvd::class_lds_bytes(cls, n[k], n[k]);
+                    for (int sl : kSlots)
+                        if (need * sl <= jsvd::kMaxLdsBytes && (jsvd::class_block(cls) / 64) * sl <= 32 &&
+                            sl <= register_slots(cls, jsvd::class_block(cls)))
+                        {
+                            slots = sl;
+                            break;
+                        }
+                }
+                else
+                    f32cls = jsvd::select_f32_class(m[k], n[k]);
+            }
             any_mixed = any_mixed || f32cls >= 0;
-            const GroupKey key{{cls, slots}, f32cls};
+            any_precond = any_precond || pre;
+            const GroupKey key{cls, slots, f32cls, pre};
             auto it = std::find_if(groups.begin(), groups.end(), [&](auto &g) { return g.first == key; });
             if (it == groups.end())
             {
@@ -884,7 +922,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         for (size_t i = 0; i < initial; ++i)
         {
             const size_t parts = std::min(kMixedSplitParts, groups[i].second.size() / kMixedSplitMin);
-            if (groups[i].first.second < 0 || parts < 2)
+            if (std::get<2>(groups[i].first) < 0 || parts < 2)
                 continue;
             std::vector<jsvd::WorkItem> items;
             items.swap(groups[i].second);
@@ -904,21 +942,33 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         std::stable_sort(g.second.begin(), g.second.end(),
                          [&](const jsvd::WorkItem &x, const jsvd::WorkItem &y) { return cost(x) > cost(y); });
         JsvdBucket b;
-        b.cls = g.first.first.first;
-        b.f32cls = g.first.second;
+        b.cls = std::get<0>(g.first);
+        b.f32cls = std::get<2>(g.first);
+        b.precond = std::get<3>(g.first) != 0;
+        b.qr_variant = std::get<3>(g.first) - 1;
         b.block = b.cls >= 0 ? jsvd::class_block(b.cls) : generic_block;
         b.count = static_cast<int64_t>(g.second.size());
         for (const auto &w : g.second)
         {
-            const int64_t need = b.cls >= 0 ? jsvd::class_lds_bytes(b.cls, w.m, w.n) : jsvd::lds_bytes_f64(w.m, w.n);
+            // A preconditioned bucket's sweeps run on X = R^T (n x n).
+            const int64_t rows = b.precond ? w.n : w.m;
+            const int64_t need = b.cls >= 0 ? jsvd::class_lds_bytes(b.cls, rows, w.n) : jsvd::lds_bytes_f64(w.m, w.n);
             b.lds_bytes = std::max<int>(b.lds_bytes, static_cast<int>(need));
             b.work += cost(w);
             b.max_n = std::max<int64_t>(b.max_n, w.n);
             if (b.f32cls >= 0)
             {
-                b.lds32 = std::max<int>(b.lds32, static_cast<int>(jsvd::f32_class_lds_bytes(b.f32cls, w.m, w.n)));
+                b.lds32 = std::max<int>(b.lds32, static_cast<int>(jsvd::f32_class_lds_bytes(b.f32cls, rows, w.n)));
                 b.lds_ortho = std::max<int>(b.lds_ortho, static_cast<int>(jsvd::orthonormalize_lds_bytes(w.n)));
             }
+            b.max_m = std::max<int64_t>(b.max_m, w.m);
+        }
+        if (b.precond)
+        {
+            // Both preconditioner kernels size their rows for the bucket's
+            // tallest matrix.
+            b.lds_qr = static_cast<int>(jsvd::qrcp_lds_bytes(b.qr_variant));
+            b.lds_fin = static_cast<int>(jsvd::precond_finish_lds_bytes(b.max_m, b.max_n));
         }
         plan->buckets.push_back(b);
     }
@@ -956,8 +1006,12 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         if (b.f32cls >= 0)
         {
             auto resident_of = [&](const void *f, int block, int lds, int *grid) -> jsvd_status {
-                if (lds > 0)
-                    JSVD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+                // Above 48 KiB of dynamic LDS the kernel needs the attribute;
+                // it is set to the whole budget (the stage kernels' static LDS
+                // is a few words).
+                if (lds > 48 * 1024)
+                    JSVD_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 int(jsvd::kMaxLdsBytes)));
                 int occ = 0;
                 JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, block, lds));
                 if (occ <= 0)
@@ -970,6 +1024,14 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             JSVD_CHECK(resident_of(jsvd::orthonormalize_symbol(), jsvd::orthonormalize_block(), b.lds_ortho,
                                    &b.grid_ortho));
             JSVD_CHECK(resident_of(jsvd::gemm_aq_symbol(), jsvd::gemm_aq_block(), 0, &b.grid_gemm));
+            if (b.precond)
+            {
+                JSVD_CHECK(resident_of(jsvd::qrcp_symbol(b.qr_variant), jsvd::qrcp_block(b.qr_variant), b.lds_qr,
+                                       &b.grid_qr));
+                JSVD_CHECK(resident_of(jsvd::precond_finish_symbol(b.max_m), jsvd::precond_finish_block(), b.lds_fin,
+                                       &b.grid_fin));
+                JSVD_CHECK(resident_of(jsvd::polar_bq_symbol(b.max_n), 256, 0, &b.grid_polar));
+            }
         }
         if (b.cls >= 0 && jsvd::class_uses_vlog(b.cls))
         {
@@ -995,6 +1057,24 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         JSVD_HIP(hipMemset(plan->d_counters.ptr, 0, sizeof(unsigned int) * kCountersPerBucket * plan->buckets.size()));
         if (any_mixed)
             JSVD_CHECK(plan->q_scratch.alloc(plan->v_elems));
+        if (any_precond)
+        {
+            // The square entries of X for the stages after the QR.
+            plan->pre_items = plan->items;
+            for (auto &w : plan->pre_items)
+            {
+                w.m = w.n;
+                w.a_off = w.v_off;
+            }
+            JSVD_CHECK(plan->d_pre_items.alloc(plan->pre_items.size()));
+            JSVD_HIP(hipMemcpy(plan->d_pre_items.ptr, plan->pre_items.data(),
+                               sizeof(jsvd::WorkItem) * plan->pre_items.size(), hipMemcpyHostToDevice));
+            JSVD_CHECK(plan->pc_refl.alloc(plan->a_elems));
+            JSVD_CHECK(plan->pc_tau.alloc(plan->s_elems));
+            JSVD_CHECK(plan->pc_perm.alloc(plan->s_elems));
+            for (DevBuf<double> *buf : {&plan->pc_x, &plan->pc_b, &plan->pc_ux, &plan->pc_vx})
+                JSVD_CHECK(buf->alloc(plan->v_elems));
+        }
         const size_t streams = std::min<size_t>(plan->buckets.size(), 4);
         // Buckets to streams, longest first onto the least loaded stream;
         // a bucket's length is its work per resident workgroup (a bucket of
@@ -1115,33 +1195,68 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
         const jsvd::WorkItem *items = p->d_items.ptr + bk.first;
         unsigned int *ctr = p->d_counters.ptr + kCountersPerBucket * b;
         const bool mixed = bk.f32cls >= 0;
+        // A preconditioned bucket: pivoted QR of each A, then every mixed
+        // stage on X = R^T (square entries, preconditioner scratch), then
+        // U = Q1 Vx, V = P Ux.
+        const bool pre = bk.precond;
+        const jsvd::WorkItem *sq_items = pre ? p->d_pre_items.ptr + bk.first : items;
+        jsvd::PrecondArgs pargs{};
+        if (pre)
+        {
+            pargs.items = items;
+            pargs.count = static_cast<int>(bk.count);
+            pargs.a = d_a;
+            pargs.refl = p->pc_refl.ptr;
+            pargs.tau = p->pc_tau.ptr;
+            pargs.perm = p->pc_perm.ptr;
+            pargs.x = p->pc_x.ptr;
+            pargs.ux = p->pc_ux.ptr;
+            pargs.vx = p->pc_vx.ptr;
+            pargs.sigma = d_sigma;
+            pargs.u = d_u;
+            pargs.v = d_v;
+            pargs.eps = p->cfg.epsilon;
+            pargs.counters = ctr + 8;
+            JSVD_HIP(jsvd::launch_qrcp_f64(bk.qr_variant, bk.grid_qr, bk.lds_qr, pargs, st));
+        }
         if (mixed)
         {
             // FP32 sweeps -> Q = orth(V32) -> B = A Q (into the U buffer),
             // then the FP64 refinement below starts from B and Q.
             jsvd::MixedArgs margs{};
-            margs.items = items;
+            margs.items = sq_items;
             margs.count = static_cast<int>(bk.count);
-            margs.a = d_a;
+            margs.a = pre ? p->pc_x.ptr : d_a;
             margs.q = p->q_scratch.ptr;
-            margs.u = d_u;
+            margs.u = pre ? p->pc_b.ptr : d_u;
             margs.sweeps = d_sweeps;
             margs.eps = p->cfg.epsilon;
             margs.max_sweeps = p->cfg.max_sweeps;
             margs.counters = ctr;
             JSVD_HIP(jsvd::launch_mixed_sweeps_f32(bk.f32cls, bk.grid32, bk.lds32, margs, st));
-            JSVD_HIP(jsvd::launch_orthonormalize_f64(bk.grid_ortho, bk.lds_ortho, margs, st));
-            JSVD_HIP(jsvd::launch_gemm_aq_f64(bk.grid_gemm, margs, st));
+            if (pre)
+            {
+                // Q by two Newton-Schulz steps and B = X Q, on MFMA; E and the
+                // intermediate Q live in the B and Ux buffers meanwhile (B is
+                // written last, Ux only by the refinement).
+                JSVD_HIP(jsvd::launch_polar_bq_f64(bk.max_n, bk.grid_polar, pargs, margs, p->pc_b.ptr, p->pc_ux.ptr,
+                                                   st));
+            }
+            else
+            {
+                JSVD_HIP(jsvd::launch_orthonormalize_f64(bk.grid_ortho, bk.lds_ortho, margs, st));
+                JSVD_HIP(jsvd::launch_gemm_aq_f64(bk.grid_gemm, margs, st));
+            }
         }
         if (bk.cls >= 0)
         {
             jsvd::ClassArgs args{};
-            args.items = items;
+            args.items = sq_items;
             args.count = static_cast<int>(bk.count);
-            args.a = d_a;
-            args.u = d_u;
+            args.a = pre ? p->pc_x.ptr : d_a;
+            args.u = pre ? p->pc_ux.ptr : d_u;
             args.sigma = d_sigma;
-            args.v = d_v;
+            args.v = pre ? p->pc_vx.ptr : d_v;
             args.sweeps = d_sweeps;
             args.stop = d_stop;
             args.eps = p->cfg.epsilon;
@@ -1154,12 +1269,17 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
             args.priority = bk.per_cu == 1 ? 1 : 0;
             if (mixed)
             {
-                args.b_cm = d_u;
+                args.b_cm = pre ? p->pc_b.ptr : d_u;
                 args.q_init = p->q_scratch.ptr;
                 args.add_sweeps = 1;
                 args.max_sweeps = p->cfg.refine_sweeps > 0 ? p->cfg.refine_sweeps : p->cfg.max_sweeps;
+                // X's left vectors become A's V: every nonzero column is
+                // normalised (the U rule applies to A's U, in the back-transform).
+                args.u_all_nonzero = pre ? 1 : 0;
             }
             JSVD_HIP(jsvd::launch_lds_class_f64(bk.cls, bk.grid, bk.lds_bytes, args, st));
+            if (pre)
+                JSVD_HIP(jsvd::launch_precond_finish_f64(bk.max_m, bk.grid_fin, bk.lds_fin, pargs, st));
         }
         else
             JSVD_HIP(jsvd::launch_lds_batch_f64(bk.block, bk.grid, bk.lds_bytes, items, static_cast<int>(bk.count),

@@ -191,6 +191,31 @@ namespace jsvd::dev
         return x;
     }
 
+    // Two independent group all-reduces with their DPP levels interleaved.
+    template <int G>
+    __device__ __forceinline__ void group_sum2_dpp(double &a, double &b)
+    {
+        if constexpr (G >= 2)
+        {
+            const double ta = dpp_f64<0xB1>(a), tb = dpp_f64<0xB1>(b);
+            a += ta;
+            b += tb;
+        }
+        if constexpr (G >= 4)
+        {
+            const double ta = dpp_f64<0x4E>(a), tb = dpp_f64<0x4E>(b);
+            a += ta;
+            b += tb;
+        }
+        if constexpr (G >= 8)
+        {
+            const double ta = dpp_f64<0x141>(a), tb = dpp_f64<0x141>(b);
+            a += ta;
+            b += tb;
+        }
+        static_assert(G <= 8, "two-value DPP reduction covers groups of up to 8 lanes");
+    }
+
     // Three independent group all-reduces with their DPP levels interleaved
     // (each level's moves read registers the previous level's adds wrote, so
     // three chains fill the wait states one chain would idle through).  Same

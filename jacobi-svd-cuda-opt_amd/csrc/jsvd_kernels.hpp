@@ -57,6 +57,7 @@ namespace jsvd
         int32_t *sweeps;
         int32_t *stop;        // optional: 0 converged, 1 stopped at max_sweeps (slot w.index)
         int add_sweeps;       // 1: add this solve's sweeps to the slot instead of storing them
+        int u_all_nonzero;    // 1: U = A / sigma wherever sigma > 0 (0: the reference's sigma > eps rule)
         double eps;
         int max_sweeps;
         double *vlog;         // rotation-log classes: per-workgroup tangent log
@@ -104,6 +105,43 @@ namespace jsvd
     const void *gemm_aq_symbol();
     int gemm_aq_block();
     hipError_t launch_gemm_aq_f64(int grid, const MixedArgs &args, hipStream_t stream);
+
+    // QR preconditioner of the mixed path: A P = Q1 R (pivoted Householder
+    // QR, FP64), Jacobi on X = R^T, then U = Q1 Vx and V = P Ux.
+    struct PrecondArgs
+    {
+        const WorkItem *items;  // the batch's own shapes and offsets
+        int count;
+        const double *a;        // packed row-major A (a_off)
+        double *refl;           // packed column-major m x n reflectors (a_off)
+        double *tau;            // n per matrix (s_off)
+        int32_t *perm;          // n per matrix (s_off): column k of A P is column perm[k] of A
+        double *x;              // packed row-major n x n X = R^T (v_off)
+        const double *ux;       // packed row-major n x n left vectors of X (v_off)
+        const double *vx;       // packed row-major n x n right vectors of X (v_off)
+        const double *sigma;    // sigma of X = sigma of A (s_off)
+        double *u;              // output U (a_off)
+        double *v;              // output V (v_off)
+        double eps;             // U rule: columns with sigma <= eps are zero
+        unsigned int *counters; // four: two per kernel
+    };
+    bool precond_fits(int64_t m, int64_t n);
+    int qrcp_variant(int64_t m, int64_t n); // QR kernel shape for an m x n matrix, -1 if none
+    int64_t qrcp_lds_bytes(int variant);
+    int64_t precond_finish_lds_bytes(int64_t m, int64_t n);
+    const void *qrcp_symbol(int variant);
+    int qrcp_block(int variant);
+    const void *precond_finish_symbol(int64_t m); // kernel instance for a bucket's largest m
+    int precond_finish_block();
+    hipError_t launch_qrcp_f64(int variant, int grid, int lds_bytes, const PrecondArgs &args, hipStream_t stream);
+    // Q = orthonormal factor of V32 (two Newton-Schulz steps) and B = X Q on
+    // FP64 MFMA for a preconditioned bucket (ma: its square entries, q, u =
+    // B buffer, counters 2-3); tmp_e / tmp_q: n x n scratch per matrix.
+    const void *polar_bq_symbol(int64_t n); // instance for a bucket's largest n (<= 128)
+    hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, double *tmp_e,
+                                   double *tmp_q, hipStream_t stream);
+    hipError_t launch_precond_finish_f64(int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
+                                         hipStream_t stream);
 
     // SVD adjoint: gA from (U, s, V) and upstream (gU, gs, gV); any upstream
     // pointer may be null (zero).  One workgroup per work-list entry.

@@ -574,6 +574,9 @@ namespace jsvd
             }
             __syncthreads();
             double *ug = args.u + w.a_off;
+            // U rule: the reference's absolute eps, or sigma > 0 when the
+            // caller asks for every nonzero column normalised.
+            const double u_floor = args.u_all_nonzero ? 0.0 : args.eps;
             double *vg = args.v + w.v_off;
             // U and V columns of the slots listed in colmap[0 .. cnt-1]
             // (other slots: colmap >= n), row by row.
@@ -584,7 +587,7 @@ namespace jsvd
                     if (j < n)
                     {
                         const double sj = S[j];
-                        ug[row * n + j] = (sj > args.eps) ? (As[sl * lda + row] / sj) : 0.0;
+                        ug[row * n + j] = (sj > u_floor) ? (As[sl * lda + row] / sj) : 0.0;
                     }
                 }
                 for (int idx = tid; idx < cnt * n; idx += BS)
@@ -833,12 +836,15 @@ namespace jsvd
             }
             __syncthreads();
             double *ug = args.u + w.a_off;
+            // U rule: the reference's absolute eps, or sigma > 0 when the
+            // caller asks for every nonzero column normalised.
+            const double u_floor = args.u_all_nonzero ? 0.0 : args.eps;
             const int mn = m * n, nn = n * n;
             for (int idx = tid; idx < mn; idx += BS)
             {
                 const int i = idx / n, j = idx - i * n;
                 const double sj = S[j];
-                ug[idx] = (sj > args.eps) ? (A[j * lda + i] / sj) : 0.0;
+                ug[idx] = (sj > u_floor) ? (A[j * lda + i] / sj) : 0.0;
             }
             if constexpr (VLOG)
             {

@@ -87,3 +87,31 @@ def test_mixed_uneven_batch():
     refs = oracle.solve_batch(mats, 1e-14, 30)
     for k, (a, g, r) in enumerate(zip(mats, got, refs)):
         _check(a, g, r[1], 1e-14, f"uneven{k}")
+
+
+def test_mixed_rank_deficient_keeps_v_orthogonal():
+    # Exactly zero columns: the FP64 oracle gives exactly zero singular
+    # values (zero columns never rotate); the QR-preconditioned mixed path
+    # gives rounding-level ones (the zero columns of A become zero rows of R).
+    # Either way U's columns there are zero (U rule) and V is orthogonal; an
+    # all-zero matrix gives exact zeros and a completed orthogonal V.
+    _note("mixed rank-deficient")
+    rng = np.random.default_rng(21)
+    mats = []
+    for shape, zero in [((40, 30), [3, 17]), ((128, 128), [0, 64, 127]), ((90, 45), [44])]:
+        a = rng.standard_normal(shape)
+        a[:, zero] = 0.0
+        mats.append(a)
+    mats.append(np.zeros((50, 20)))
+    got = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=40, precision=jsvd.PRECISION_MIXED)
+    refs = oracle.solve_batch(mats, 1e-14, 40)
+    for a, g, r in zip(mats, got, refs):
+        u, s, v, _ = g
+        tiny = s <= 1e-14 * max(s.max(), 1e-300)
+        assert np.sum(tiny) == np.sum(r[1] == 0.0)
+        assert np.all(u[:, tiny] == 0.0)
+        assert metrics.orthogonality(v) <= 1e-12 * (a.shape[1] / 64 + 1)
+        if np.linalg.norm(a) > 0:
+            _check(a, g, r[1], 1e-14, f"rank-deficient {a.shape}")
+        else:
+            assert np.all(s == 0.0) and np.all(u == 0.0)
