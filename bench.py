@@ -14,10 +14,13 @@ against the MI355X peak and their LDS stores against the measured LDS store
 rate (HIP events on the launch stream), the
 HBM traffic measured with rocprofv3 counters, the CPU oracle timed on a
 bounded sample, the mixed-precision vs FP64 comparison on 1024 matrices
-of 128 x 128 with kappa = 10^U(3,8), the blocked solve of one 8192 x 2048
-matrix (sweeps, HBM GB/s and FP64 MFMA rate per sweep, the oracle's sweep
-beside it), the plain solve of one 256 x 256 matrix beside the oracle, and
-the adjoint kernel on 512 matrices of 96 x 64 (FP64 and FP32).
+of 128 x 128 with kappa = 10^U(3,8) and on the uneven batch (both legs'
+sigma against the FP64 oracle, per matrix), the blocked and plain solves of
+one 8192 x 2048 matrix (sweeps, HBM GB/s and FP64 rate per sweep on the
+algorithmic definitions, sigma against the oracle's solve of it), the plain
+solve of one 256 x 256 matrix beside the oracle (one thread and
+round-parallel) and the host-API call latency, and the adjoint kernel on
+512 matrices of 96 x 64 (FP64 and FP32).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-mixed] [--no-blocked] [--no-single]
                     [--no-backward] [--no-tune]
@@ -64,6 +67,29 @@ def compulsory_bytes(m, n):
     m = np.asarray(m, dtype=np.float64)
     n = np.asarray(n, dtype=np.float64)
     return 8 * (2 * m * n + n * n + n)
+
+
+def cpu_model():
+    """Host CPU model name (for every CPU figure in the report)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Host threads for the CPU legs: the affinity set, capped by
+    OMP_NUM_THREADS when the environment sets it (and at 64)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, min(n, 64))
 
 
 def load_json(name):
@@ -114,7 +140,7 @@ def cpu_baseline(mats, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle
 
-    cores = min(os.cpu_count() or 1, 64)
+    cores = host_cores()
     t0 = time.perf_counter()
     oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
     once = time.perf_counter() - t0
@@ -124,17 +150,34 @@ def cpu_baseline(mats, budget_s=12.0):
         oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
     dt = time.perf_counter() - t0
     return {"value": reps * len(mats) / dt, "unit": "matrices/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(),
             "sample": f"the full {len(mats)}-matrix batch solved {reps} times ({dt:.1f} s) by the C oracle, "
                       f"one matrix per OpenMP thread on {cores} threads"}
 
 
-def mixed_vs_fp64(torch, jsvd, gen, metrics, cpu, steps=3):
-    """1024 matrices 128 x 128, kappa = 10^U(3,8): FP64 plan vs mixed plan (and,
-    with cpu, the oracle on a sample of the batch)."""
-    mats, _ = gen.conditioned_batch(1024)
+def sigma_error_stats(metrics, got, ref):
+    """Per-matrix sigma errors (sorted, normwise and componentwise) of got
+    against ref: max and percentiles."""
+    norm = np.array([metrics.sigma_errors(g, r)[0] for g, r in zip(got, ref)])
+    comp = np.array([metrics.sigma_errors(g, r)[1] for g, r in zip(got, ref)])
+    return {"normwise_max": float(norm.max()), "normwise_p50": float(np.percentile(norm, 50)),
+            "normwise_p99": float(np.percentile(norm, 99)), "componentwise_max": float(comp.max()),
+            "matrices_above_1e-12": int(np.sum(norm > 1e-12))}
+
+
+def mixed_vs_fp64(torch, jsvd, gen, metrics, cpu, workload, steps=3):
+    """FP64 plan vs mixed-precision plan on one batch: the 1024 x 128x128
+    batch of kappa = 10^U(3,8) ("cfg4") or the 4096 uneven batch ("cfg2").
+    With cpu, every matrix of both legs is compared with the FP64 oracle's
+    solve of it (sigma sorted), and the oracle's rate is reported."""
+    if workload == "cfg4":
+        mats, _ = gen.conditioned_batch(1024)
+        desc = "1024 x 128x128, kappa=10^U(3,8), sigma_k = kappa^(-k/127)"
+    else:
+        mats, _ = gen.uneven_batch(4096)
+        desc = "4096 uneven, n ~ U{8..64}, m ~ U{n..2n}, sigma logspace(0,-6,n)"
     flat, ms, ns = gen.pack(mats)
-    out = {"workload": "1024 x 128x128, kappa=10^U(3,8), sigma_k = kappa^(-k/127)", "epsilon": EPS,
-           "max_sweeps": MAX_SWEEPS}
+    out = {"workload": desc, "epsilon": EPS, "max_sweeps": MAX_SWEEPS}
     sig = {}
     for name, prec in (("fp64", jsvd.PRECISION_FP64), ("mixed", jsvd.PRECISION_MIXED)):
         plan = jsvd.BatchPlan(ms, ns, epsilon=EPS, max_sweeps=MAX_SWEEPS, precision=prec)
@@ -150,54 +193,66 @@ def mixed_vs_fp64(torch, jsvd, gen, metrics, cpu, steps=3):
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
 
-        cores = min(os.cpu_count() or 1, 64)
-        sample = mats[:4 * cores]
+        cores = host_cores()
         t0 = time.perf_counter()
-        oracle.solve_batch(sample, EPS, MAX_SWEEPS, threads=cores)
+        refs = oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
         dt = time.perf_counter() - t0
-        out["oracle_fp64"] = {"matrices_per_s": len(sample) / dt, "cores": cores,
-                              "sample": f"the first {len(sample)} matrices, one per OpenMP thread"}
-    norm = [metrics.sigma_errors(sm, sf)[0] for sm, sf in zip(sig["mixed"], sig["fp64"])]
-    comp = [metrics.sigma_errors(sm, sf)[1] for sm, sf in zip(sig["mixed"], sig["fp64"])]
-    out["mixed_vs_fp64_sigma_err_normwise_max"] = float(np.max(norm))
-    out["mixed_vs_fp64_sigma_err_componentwise_max"] = float(np.max(comp))
+        ref_sig = [r[1] for r in refs]
+        out["oracle_fp64"] = {"matrices_per_s": len(mats) / dt, "cores": cores, "cpu_model": cpu_model(),
+                              "sample": f"the whole batch, one matrix per OpenMP thread"}
+        out["fp64"]["sigma_err_vs_oracle"] = sigma_error_stats(metrics, sig["fp64"], ref_sig)
+        out["mixed"]["sigma_err_vs_oracle"] = sigma_error_stats(metrics, sig["mixed"], ref_sig)
+    out["mixed_vs_fp64_sigma_err_normwise_max"] = float(
+        max(metrics.sigma_errors(sm, sf)[0] for sm, sf in zip(sig["mixed"], sig["fp64"])))
     out["speedup_mixed_over_fp64"] = out["mixed"]["matrices_per_s"] / out["fp64"]["matrices_per_s"]
     return out
 
 
 def blocked_bytes(m, n, b):
-    """HBM bytes of one blocked sweep: each of the Nb(Nb-1)/2 block-pair visits
-    reads the 2b panel columns of A for the Gram (16 b m) and reads and writes
-    the panels of A and V in the update (32 b (m + n)): 8 b Nb (Nb-1) (3m + 2n)."""
+    """Algorithmic HBM bytes of one blocked sweep (the survey's definition):
+    one read and one write of the two b-column panels of A and of V per
+    block-pair visit, 16 b Nb (Nb-1) (m + n)."""
+    nb = -(-n // (2 * b)) * 2
+    return 16.0 * b * nb * (nb - 1) * (m + n)
+
+
+def blocked_flops(m, n, b):
+    """Algorithmic FP64 flops of one blocked sweep (the survey's definition):
+    4 b^2 Nb (Nb-1) (2m + n)."""
+    nb = -(-n // (2 * b)) * 2
+    return 4.0 * b * b * nb * (nb - 1) * (2 * m + n)
+
+
+def blocked_traffic_model(m, n, b):
+    """This build's own HBM byte model of a blocked sweep (kept beside the
+    algorithmic figure): each visit also reads the panels of A once more for
+    the Gram (16 b m), 8 b Nb (Nb-1) (3m + 2n)."""
     nb = -(-n // (2 * b)) * 2
     return 8.0 * b * nb * (nb - 1) * (3 * m + 2 * n)
 
 
-def blocked_flops(m, n, b):
-    """FP64 flops of one blocked sweep: per visit the upper triangle of the
-    2b x 2b Gram (W (W+1) m) and the two panel products (2 W^2 (m + n))."""
-    nb = -(-n // (2 * b)) * 2
-    w = 2 * b
-    return nb * (nb - 1) / 2 * (w * (w + 1) * m + 2.0 * w * w * (m + n))
-
-
 def blocked_tall(torch, jsvd, gen, metrics, cpu):
     """Single 8192 x 2048 matrix, sigma_k = 10^(-10k/2047), Haar factors, blocked
-    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps), and two
-    sweeps of the plain column-pair path.  With cpu, the oracle's plain sweep
-    is timed beside it (one sweep, all host cores)."""
+    Jacobi at block widths 32 and 16 (eps 1e-15, up to 80 sweeps), and the
+    plain column-pair path run to convergence; sigma is compared with the
+    oracle's solve of the same matrix (tests/golden/tall_8192x2048.npz, written
+    once by tests/golden/make_tall_fixture.py).  With cpu, the oracle's plain
+    sweep is timed beside it (one sweep, all host cores)."""
     a, sig = gen.tall_geometric()
     m, n = a.shape
     out = {"workload": "8192x2048, sigma_k=10^(-10k/2047), Haar factors", "epsilon": EPS, "max_sweeps": 80}
+    fixture = np.load(os.path.join(REPO, "tests", "golden", "tall_8192x2048.npz"))
+    out["oracle_sweeps"] = int(fixture["sweeps"])
     if cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
 
-        cores = min(os.cpu_count() or 1, 64)
+        cores = host_cores()
         t0 = time.perf_counter()
         oracle.solve(a, EPS, 1, threads=cores)
         out["oracle_plain_s_per_sweep"] = time.perf_counter() - t0
         out["oracle_cores"] = cores
+        out["oracle_cpu_model"] = cpu_model()
     for bw in (32, 16):
         plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=80, block_width=bw)
         bufs = device_buffers(plan, a.ravel(), torch)
@@ -206,7 +261,8 @@ def blocked_tall(torch, jsvd, gen, metrics, cpu):
         sweeps = int(sw.item())
         Am, Um, Vm = A.view(m, n), U.view(m, n), V.view(n, n)
         res = (torch.linalg.norm(Am - (Um * S) @ Vm.T) / torch.linalg.norm(Am)).item()
-        norm_err, comp_err = metrics.sigma_errors(S.cpu().numpy(), sig)
+        s_h = S.cpu().numpy()
+        norm_err, comp_err = metrics.sigma_errors(s_h, sig)
         per_sweep_s = kernel_ms * 1e-3 / sweeps
         byt, fl = blocked_bytes(m, n, bw), blocked_flops(m, n, bw)
         out[f"block_width_{bw}"] = {
@@ -215,19 +271,24 @@ def blocked_tall(torch, jsvd, gen, metrics, cpu):
             "hbm_frac": byt / per_sweep_s / 1e9 / HBM_PEAK_GBS,
             "flops_per_sweep": fl, "fp64_TFs": fl / per_sweep_s / 1e12,
             "mfma_frac": fl / per_sweep_s / 1e12 / FP64_PEAK_TFLOPS,
-            "residual": res, "sigma_err_normwise": norm_err, "sigma_err_componentwise": comp_err}
+            "traffic_model_bytes_per_sweep": blocked_traffic_model(m, n, bw),
+            "traffic_model_GBs": blocked_traffic_model(m, n, bw) / per_sweep_s / 1e9,
+            "residual": res, "sigma_err_normwise": norm_err, "sigma_err_componentwise": comp_err,
+            "sigma_err_vs_oracle": metrics.sigma_errors(s_h, fixture["sigma_sorted"])[0]}
         plan.close()
-    # Plain column-pair order on the GPU (grouped wave kernel, rows streamed):
-    # two sweeps timed, algorithmic traffic 16 n (n-1) (m+n) bytes per sweep.
-    plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=2)
+    # Plain column-pair order on the GPU (grouped wave kernel, rows streamed)
+    # run to convergence; algorithmic traffic 16 n (n-1) (m+n) bytes per sweep.
+    plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=80)
     bufs = device_buffers(plan, a.ravel(), torch)
-    wall, kernel_ms = time_plan(plan, bufs, 1, 1, torch)
+    wall, kernel_ms = time_plan(plan, bufs, 1, 0, torch)
     sweeps = int(bufs[4].item())
     per_sweep_s = kernel_ms * 1e-3 / sweeps
     byt = 16.0 * n * (n - 1) * (m + n)
-    out["plain_gpu"] = {"sweeps_timed": sweeps, "ms_per_sweep": per_sweep_s * 1e3, "bytes_per_sweep": byt,
-                        "hbm_GBs_per_sweep": byt / per_sweep_s / 1e9,
-                        "hbm_frac": byt / per_sweep_s / 1e9 / HBM_PEAK_GBS}
+    out["plain_gpu"] = {"sweeps": sweeps, "ms": kernel_ms, "ms_per_sweep": per_sweep_s * 1e3,
+                        "bytes_per_sweep": byt, "hbm_GBs_per_sweep": byt / per_sweep_s / 1e9,
+                        "hbm_frac": byt / per_sweep_s / 1e9 / HBM_PEAK_GBS,
+                        "sigma_err_vs_oracle": metrics.sigma_errors(bufs[2].cpu().numpy(),
+                                                                    fixture["sigma_sorted"])[0]}
     plan.close()
     return out
 
@@ -262,6 +323,18 @@ def plain_single(torch, jsvd, gen, cpu):
     out["tall_batch_64x1024x64"] = {"gpu_ms": tall_ms, "matrices_per_s": 64 / (tall_ms * 1e-3),
                                     "mean_sweeps": float(bufs[4].double().mean().item())}
     plan.close()
+    # The host-array entry point (one_sided_jacobi_svd's C-ABI, jsvd_solve_f64):
+    # copies in and out included, the first call building the plan (device
+    # allocations, occupancy queries), later calls reusing the cached one.
+    t0 = time.perf_counter()
+    jsvd.svd(a, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    first = time.perf_counter() - t0
+    calls = 10
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        jsvd.svd(a, epsilon=EPS, max_sweeps=MAX_SWEEPS)
+    out["host_api_ms_per_call"] = {"first_call": first * 1e3, "cached_plan": (time.perf_counter() - t0) / calls * 1e3,
+                                   "device_solve": kernel_ms}
     if cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle
@@ -270,7 +343,14 @@ def plain_single(torch, jsvd, gen, cpu):
         ref = oracle.solve(a, EPS, MAX_SWEEPS, threads=1)
         out["oracle_ms_1_thread"] = (time.perf_counter() - t0) * 1e3
         out["oracle_sweeps"] = int(ref[3])
-        cores = min(os.cpu_count() or 1, 64)
+        cores = host_cores()
+        # Round-parallel oracle: the disjoint pairs of a round over all host
+        # threads, one barrier per round (bitwise equal to the serial solve).
+        t0 = time.perf_counter()
+        oracle.solve(a, EPS, MAX_SWEEPS, threads=cores)
+        out["oracle_ms_round_parallel"] = (time.perf_counter() - t0) * 1e3
+        out["oracle_cores"] = cores
+        out["oracle_cpu_model"] = cpu_model()
         t0 = time.perf_counter()
         oracle.solve_batch(mats, EPS, MAX_SWEEPS, threads=cores)
         dt = time.perf_counter() - t0
@@ -417,7 +497,9 @@ def main():
                    "shape_classes": ({"per_bucket": tune["classes"], "plans_timed": tune["samples"],
                                       "ms_default": tune["ms_initial"], "ms_tuned": tune["ms_tuned"]}
                                      if tune else "default")},
-        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        # The work-list kernel issues FP64 VALU instructions (no MFMA); MI355X's
+        # FP64 vector and matrix peaks are both 78.6 TFLOP/s.
+        "roofline": {"bound": "fp64-valu", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": "lds_class_f64 (all buckets of one plan execution)", "kernel_ms": kernel_ms,
@@ -430,7 +512,8 @@ def main():
     }
     plan.close()
     if not args.no_mixed:
-        result["mixed_vs_fp64"] = mixed_vs_fp64(torch, jsvd, gen, metrics, not args.no_cpu)
+        result["mixed_vs_fp64"] = mixed_vs_fp64(torch, jsvd, gen, metrics, not args.no_cpu, "cfg4")
+        result["mixed_vs_fp64_uneven4096"] = mixed_vs_fp64(torch, jsvd, gen, metrics, not args.no_cpu, "cfg2")
     if not args.no_blocked:
         result["blocked_8192x2048"] = blocked_tall(torch, jsvd, gen, metrics, not args.no_cpu)
     if not args.no_backward:

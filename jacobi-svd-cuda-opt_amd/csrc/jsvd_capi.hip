@@ -331,6 +331,7 @@ struct jsvd_plan
     DevBuf<int32_t> ws_flags;
     DevBuf<unsigned int> ws_bar;
     int big_block = 256;
+    int big_resident = 0; // resident workgroups of the grid-barrier kernel (occupancy x CUs)
     hipEvent_t big_in = nullptr, big_out = nullptr;
     // Large matrices with m <= wave_sweeps_max_rows() (and no blocked
     // path): the grouped wave kernel, one column-major workspace per matrix.
@@ -481,15 +482,10 @@ namespace
         JSVD_HIP(jsvd::launch_transpose_f64(a, p.ws_a.ptr, m, n, nullptr, 0.0, stream));
         JSVD_HIP(jsvd::launch_identity_f64(p.ws_v.ptr, n, stream));
         JSVD_HIP(jsvd::launch_fill_i32(p.ws_flags.ptr, 0, p.cfg.max_sweeps, stream));
-        hipDeviceProp_t prop;
-        JSVD_HIP(hipGetDeviceProperties(&prop, p.device));
         const int half = (n + (n & 1)) / 2;
         int32_t *sweeps_slot = d_sweeps ? d_sweeps + k : nullptr;
-        int occupancy = 0;
-        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occupancy, jsvd::persistent_sweeps_f64_symbol(p.big_block), p.big_block, 0));
-        const int resident = std::max(1, occupancy) * prop.multiProcessorCount;
-        const int grid = std::max(1, std::min(resident, half));
+        // Resident workgroups of the grid-barrier kernel, from the plan.
+        const int grid = std::max(1, std::min(p.big_resident, half));
         JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
                                                     p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
                                                     d_stop ? d_stop + k : nullptr, stream));
@@ -687,12 +683,46 @@ namespace
         return JSVD_OK;
     }
 
+    // Plans of the host-array entry points, reused while the batch shapes,
+    // the configuration and the shape-class table are the same (a repeated
+    // one_sided_jacobi_svd call of one shape then allocates nothing and
+    // queries nothing).  At most kPlanCache per device, least recently used
+    // dropped first.  Kept until the process ends (not destroyed at exit,
+    // when the runtime may already be gone).
+    constexpr size_t kPlanCache = 4;
+
+    bool same_config(const jsvd_config &a, const jsvd_config &b)
+    {
+        return a.epsilon == b.epsilon && a.max_sweeps == b.max_sweeps && a.threads_per_block == b.threads_per_block &&
+               a.layout_mode == b.layout_mode && a.layout_min_columns == b.layout_min_columns &&
+               a.layout_min_elements == b.layout_min_elements && a.layout_auto_tune == b.layout_auto_tune &&
+               a.layout_bench_reps == b.layout_bench_reps && a.layout_bench_sweeps == b.layout_bench_sweeps &&
+               a.precision == b.precision && a.block_width == b.block_width && a.refine_sweeps == b.refine_sweeps;
+    }
+
+    struct CachedPlan
+    {
+        std::vector<int64_t> m, n;
+        jsvd_config cfg{};
+        std::vector<int> tune; // the shape-class table the plan was built with
+        jsvd_plan *plan = nullptr;
+        uint64_t last_use = 0;
+    };
+
+    std::vector<int> tune_table()
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        return std::vector<int>(g_tuned, g_tuned + kTuneBuckets);
+    }
+
     struct DeviceContext
     {
         std::mutex mu;
         hipStream_t stream = nullptr;
         DevBuf<double> a, u, s, v;
         DevBuf<int32_t> sweeps;
+        std::vector<CachedPlan> plans;
+        uint64_t clock = 0;
     };
 
     std::mutex g_ctx_mu;
@@ -1140,6 +1170,12 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     if (!plan->large.empty())
     {
         plan->big_block = persistent_kernel_block(cfg.threads_per_block);
+        {
+            int occupancy = 0;
+            JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &occupancy, jsvd::persistent_sweeps_f64_symbol(plan->big_block), plan->big_block, 0));
+            plan->big_resident = std::max(1, occupancy) * prop.multiProcessorCount;
+        }
         JSVD_CHECK(plan->ws_a.alloc(big_mn));
         JSVD_CHECK(plan->ws_v.alloc(big_nn));
         JSVD_CHECK(plan->ws_s.alloc(big_n));
@@ -1468,11 +1504,35 @@ jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_
     }
     DeviceContext *ctx = nullptr;
     JSVD_CHECK(device_context(&ctx));
-    jsvd_plan *plan = nullptr;
-    JSVD_CHECK(jsvd_plan_create(count, mm.data(), nn.data(), cfg, &plan));
-    std::unique_ptr<jsvd_plan, void (*)(jsvd_plan *)> guard(plan, jsvd_plan_destroy);
-
     std::lock_guard<std::mutex> lock(ctx->mu);
+    jsvd_config c;
+    jsvd_config_default(&c);
+    if (cfg)
+        c = *cfg;
+    const std::vector<int> tune = tune_table();
+    jsvd_plan *plan = nullptr;
+    for (CachedPlan &e : ctx->plans)
+        if (e.m == mm && e.n == nn && same_config(e.cfg, c) && e.tune == tune)
+        {
+            plan = e.plan;
+            e.last_use = ++ctx->clock;
+            break;
+        }
+    if (!plan)
+    {
+        JSVD_CHECK(jsvd_plan_create(count, mm.data(), nn.data(), &c, &plan));
+        if (ctx->plans.size() >= kPlanCache)
+        {
+            auto lru = std::min_element(ctx->plans.begin(), ctx->plans.end(),
+                                        [](const CachedPlan &x, const CachedPlan &y) { return x.last_use < y.last_use; });
+            // The context's stream is idle here (every call ends with a
+            // stream synchronisation), so the plan can go.
+            jsvd_plan_destroy(lru->plan);
+            ctx->plans.erase(lru);
+        }
+        ctx->plans.push_back({mm, nn, c, tune, plan, ++ctx->clock});
+    }
+
     JSVD_CHECK(ctx->a.reserve(plan->a_elems));
     JSVD_CHECK(ctx->u.reserve(plan->a_elems));
     JSVD_CHECK(ctx->s.reserve(plan->s_elems));
