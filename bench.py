@@ -26,6 +26,8 @@ round-parallel) and the host-API call latency, and the adjoint kernel on
                     [--no-backward] [--no-tune]
     python bench.py --traffic-probe    # one 256 MiB copy + one plan execution,
                                         # for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    python bench.py --blocked-probe    # one 256 MiB copy + two blocked sweeps of
+                                        # 8192 x 2048 at widths 32 and 16, for --pmc
 """
 from __future__ import annotations
 
@@ -422,6 +424,31 @@ def traffic_probe():
     print("traffic probe done", flush=True)
 
 
+def blocked_probe():
+    """One 256 MiB device copy (the counter calibration) and two blocked
+    sweeps of the 8192 x 2048 matrix at block widths 32 and 16, for
+    rocprofv3 --pmc runs on gram_partial / gram_solve / block_update."""
+    import torch
+
+    import jsvd
+    from jsvd import gen
+
+    a, _ = gen.tall_geometric()
+    m, n = a.shape
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    for bw in (32, 16):
+        plan = jsvd.BatchPlan([m], [n], epsilon=EPS, max_sweeps=2, block_width=bw)
+        bufs = device_buffers(plan, a.ravel(), torch)
+        plan.execute(*bufs)
+        torch.cuda.synchronize()
+        plan.close()
+    print("blocked probe done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,6 +463,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default shape classes (skip the batch autotune before the timed region)")
     ap.add_argument("--traffic-probe", action="store_true")
+    ap.add_argument("--blocked-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -444,6 +472,9 @@ def main():
         sys.exit(2)
     if args.traffic_probe:
         traffic_probe()
+        return
+    if args.blocked_probe:
+        blocked_probe()
         return
 
     import torch
