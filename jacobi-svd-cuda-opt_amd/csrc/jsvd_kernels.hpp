@@ -259,4 +259,7 @@ namespace jsvd
     // V <- I (n x n), flags[0..count) <- 0, sweeps slot <- 0.
     hipError_t launch_identity_f64(double *v, int n, hipStream_t stream);
     hipError_t launch_fill_i32(int32_t *p, int32_t value, int64_t count, hipStream_t stream);
+    // count 64-bit words at p from network (big-endian) to host byte order,
+    // in place (*.mat payloads staged as read from the file).
+    hipError_t launch_bswap64(void *p, int64_t count, hipStream_t stream);
 } // namespace jsvd

@@ -758,6 +758,14 @@ namespace jsvd
                 v[idx] = (idx % (n + 1) == 0) ? 1.0 : 0.0;
         }
 
+        // Network (big-endian) to host byte order, in place.
+        __global__ void bswap64_inplace(uint64_t *p, int64_t count)
+        {
+            const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+            if (idx < count)
+                p[idx] = __builtin_bswap64(p[idx]);
+        }
+
         __global__ void fill_i32(int32_t *p, int32_t value, int64_t count)
         {
             const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -963,6 +971,14 @@ namespace jsvd
         if (total == 0)
             return hipSuccess;
         identity_f64<<<(unsigned)((total + 255) / 256), 256, 0, stream>>>(v, n);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_bswap64(void *p, int64_t count, hipStream_t stream)
+    {
+        if (count == 0)
+            return hipSuccess;
+        bswap64_inplace<<<(unsigned)((count + 255) / 256), 256, 0, stream>>>(static_cast<uint64_t *>(p), count);
         return hipGetLastError();
     }
 

@@ -9,6 +9,10 @@
 // and packing the next batch overlaps the device work of the previous ones;
 // the oldest slot is always drained first, which keeps the output in input
 // order (U, Sigma as 1 x n, V for every testcase, as the reference writes).
+// A *.mat input is read with MatDispatchReader straight into the slot's
+// pinned staging, payloads back to back in network byte order as they are
+// stored in the file; one device pass swaps the bytes after the copy to
+// HBM, so the host never decodes a value.
 // A slot whose next batch has the same shapes as its last one replays a
 // captured hipGraph of copy-in, solve and copy-out instead of re-issuing the
 // launches.
@@ -24,10 +28,16 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <span>
 #include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
+
+namespace jsvd
+{
+    hipError_t launch_bswap64(void *p, int64_t count, hipStream_t stream);
+}
 
 namespace jacobi::svd::pipeline
 {
@@ -111,6 +121,24 @@ namespace jacobi::svd::pipeline
                 count = n;
                 return true;
             }
+            // Capacity for at least n elements, keeping the first `keep`;
+            // grows geometrically.  Returns true when the buffer moved.
+            bool grow_keep(std::size_t n, std::size_t keep)
+            {
+                if (n <= count)
+                    return false;
+                const std::size_t cap = std::max(n, 2 * count);
+                T *p = nullptr;
+                hip_check(hipHostMalloc(reinterpret_cast<void **>(&p), cap * sizeof(T), 0), "hipHostMalloc");
+                if (ptr)
+                {
+                    std::memcpy(p, ptr, keep * sizeof(T));
+                    hip_check(hipHostFree(ptr), "hipHostFree");
+                }
+                ptr = p;
+                count = cap;
+                return true;
+            }
         };
 
         template <typename T>
@@ -148,6 +176,11 @@ namespace jacobi::svd::pipeline
             std::size_t a_elems = 0, v_elems = 0, s_elems = 0;
             bool capturable = false;
             bool metrics = false; // error metrics requested (PipelineConfig::report_errors)
+            bool raw_be = false;  // staging holds *.mat payloads in network byte order
+            // The batch being staged: shapes and elements so far.
+            std::vector<int64_t> bm, bn;
+            std::size_t staged = 0;
+            bool in_moved = false; // h_in reallocated since the last submit
             Pinned<double> h_in, h_u, h_s, h_v, h_met;
             Pinned<int32_t> h_sw;  // sweeps, then the stop reasons
             Device<double> d_a, d_u, d_s, d_v, d_met;
@@ -179,10 +212,33 @@ namespace jacobi::svd::pipeline
                 plan = nullptr;
             }
 
+            void begin()
+            {
+                bm.clear();
+                bn.clear();
+                staged = 0;
+            }
+
+            // Room for one more testcase of rows x cols in the staging;
+            // returns where its payload goes.
+            std::byte *stage(std::size_t rows, std::size_t cols)
+            {
+                const std::size_t elems = rows * cols;
+                in_moved |= h_in.grow_keep(staged + elems, staged);
+                double *at = h_in.ptr + staged;
+                staged += elems;
+                bm.push_back(static_cast<int64_t>(rows));
+                bn.push_back(static_cast<int64_t>(cols));
+                return reinterpret_cast<std::byte *>(at);
+            }
+
             void issue()
             {
                 hip_check(hipMemcpyAsync(d_a.ptr, h_in.ptr, a_elems * sizeof(double), hipMemcpyHostToDevice, stream),
                           "hipMemcpyAsync(H2D A)");
+                if (raw_be)
+                    hip_check(jsvd::launch_bswap64(d_a.ptr, static_cast<int64_t>(a_elems), stream),
+                              "bswap64 launch");
                 const std::size_t count = m.size();
                 rethrow(jsvd_plan_execute_status_f64(plan, d_a.ptr, d_u.ptr, d_s.ptr, d_v.ptr, d_sw.ptr,
                                                      d_sw.ptr + count, stream));
@@ -203,26 +259,14 @@ namespace jacobi::svd::pipeline
                               "hipMemcpyAsync(D2H metrics)");
             }
 
-            void submit(const std::vector<io::Matrix> &batch, std::size_t first_index, const jsvd_config &cfg)
+            // Solves the staged batch (bm, bn, h_in[0 .. staged)).
+            void submit(const jsvd_config &cfg)
             {
-                std::vector<int64_t> mm(batch.size()), nn(batch.size());
-                for (std::size_t k = 0; k < batch.size(); ++k)
-                {
-                    const io::Matrix &x = batch[k];
-                    if (x.rows == 0 || x.columns == 0)
-                        throw std::invalid_argument("Testcase[" + std::to_string(first_index + k) +
-                                                    "] has zero dimension.");
-                    if (x.values.size() != x.rows * x.columns)
-                        throw std::invalid_argument("Testcase[" + std::to_string(first_index + k) +
-                                                    "] payload size does not match rows*columns.");
-                    mm[k] = static_cast<int64_t>(x.rows);
-                    nn[k] = static_cast<int64_t>(x.columns);
-                }
-                if (!plan || mm != m || nn != n)
+                if (!plan || bm != m || bn != n)
                 {
                     drop_plan();
-                    m = std::move(mm);
-                    n = std::move(nn);
+                    m = bm;
+                    n = bn;
                     rethrow(jsvd_plan_create(static_cast<int64_t>(m.size()), m.data(), n.data(), &cfg, &plan));
                     jsvd_plan_info info{};
                     rethrow(jsvd_plan_get_info(plan, &info));
@@ -234,8 +278,9 @@ namespace jacobi::svd::pipeline
                     // run (its first call on a plan uploads the tile list).
                     capturable = info.large_matrices == 0 && !metrics;
                 }
-                bool moved = false;
-                moved |= h_in.reserve(a_elems);
+                if (a_elems != staged)
+                    throw std::logic_error("Pipeline staging does not match the plan's packed size.");
+                bool moved = std::exchange(in_moved, false);
                 moved |= h_u.reserve(a_elems);
                 moved |= h_s.reserve(s_elems);
                 moved |= h_v.reserve(v_elems);
@@ -254,12 +299,6 @@ namespace jacobi::svd::pipeline
                 {
                     (void)hipGraphExecDestroy(graph);
                     graph = nullptr;
-                }
-                std::size_t off = 0;
-                for (const io::Matrix &x : batch)
-                {
-                    std::memcpy(h_in.ptr + off, x.values.data(), x.values.size() * sizeof(double));
-                    off += x.values.size();
                 }
                 if (capturable)
                 {
@@ -391,19 +430,12 @@ namespace jacobi::svd::pipeline
             cfg.layout_min_elements = tuned.recommended_min_elements;
         }
 
-        std::function<bool(io::Matrix &)> read;
-        std::unique_ptr<io::MatReader> mat_in;
+        std::unique_ptr<io::MatDispatchReader> mat_in;
         std::unique_ptr<io::TxtReader> txt_in;
         if (in_fmt == MatrixFileFormat::mat)
-        {
-            mat_in = std::make_unique<io::MatReader>(config_.input_path);
-            read = [&](io::Matrix &x) { return io::MatFilePolicy::read_next(*mat_in, x); };
-        }
+            mat_in = std::make_unique<io::MatDispatchReader>(config_.input_path);
         else
-        {
             txt_in = std::make_unique<io::TxtReader>(config_.input_path);
-            read = [&](io::Matrix &x) { return io::TxtFilePolicy::read_next(*txt_in, x); };
-        }
         // Reference sigma for the error report: a matrix stream of 1 x n rows
         // in the input's formats (extension or the input format).
         std::function<bool(io::Matrix &)> reference;
@@ -451,34 +483,67 @@ namespace jacobi::svd::pipeline
 
         const std::size_t window = std::clamp<std::size_t>(config_.max_queued_results, 1, 8);
         std::vector<std::unique_ptr<Slot>> slots;
-        std::size_t next_slot = 0, submitted = 0;
-        std::vector<io::Matrix> batch;
-        std::size_t batch_elems = 0;
-        auto submit = [&]() {
-            if (batch.empty())
-                return;
+        std::size_t next_slot = 0, submitted = 0, read_count = 0;
+        // The slot whose staging receives the batch being read: the oldest
+        // one, drained first, while the other slots' batches run.
+        Slot *fill = nullptr;
+        auto acquire = [&]() -> Slot & {
             if (slots.size() < window)
             {
                 slots.push_back(std::make_unique<Slot>());
                 slots.back()->metrics = config_.report_errors;
+                slots.back()->raw_be = in_fmt == MatrixFileFormat::mat;
             }
             Slot &slot = *slots[next_slot];
             if (slot.busy)
                 slot.drain(write, report, reference);
-            slot.submit(batch, submitted, cfg);
-            submitted += batch.size();
-            next_slot = (next_slot + 1) % window;
-            batch.clear();
-            batch_elems = 0;
+            slot.begin();
+            return slot;
         };
-        io::Matrix x;
-        while (read(x))
-        {
-            batch_elems += x.values.size();
-            batch.push_back(std::move(x));
-            x = io::Matrix{};
-            if (batch.size() >= kBatchMatrices || batch_elems >= kBatchElements)
+        auto submit = [&]() {
+            if (!fill || fill->bm.empty())
+                return;
+            fill->submit(cfg);
+            submitted += fill->bm.size();
+            next_slot = (next_slot + 1) % window;
+            fill = nullptr;
+        };
+        auto check_shape = [&](std::size_t rows, std::size_t cols) {
+            if (rows == 0 || cols == 0)
+                throw std::invalid_argument("Testcase[" + std::to_string(read_count) + "] has zero dimension.");
+        };
+        auto added = [&]() {
+            ++read_count;
+            if (fill->bm.size() >= kBatchMatrices || fill->staged >= kBatchElements)
                 submit();
+        };
+        if (mat_in)
+        {
+            std::size_t rows = 0, cols = 0;
+            while (mat_in->read_header(rows, cols))
+            {
+                check_shape(rows, cols);
+                if (!fill)
+                    fill = &acquire();
+                std::byte *at = fill->stage(rows, cols);
+                (void)mat_in->read_payload(std::span<std::byte>(at, rows * cols * sizeof(double)));
+                added();
+            }
+        }
+        else
+        {
+            io::Matrix x;
+            while (io::TxtFilePolicy::read_next(*txt_in, x))
+            {
+                check_shape(x.rows, x.columns);
+                if (x.values.size() != x.rows * x.columns)
+                    throw std::invalid_argument("Testcase[" + std::to_string(read_count) +
+                                                "] payload size does not match rows*columns.");
+                if (!fill)
+                    fill = &acquire();
+                std::memcpy(fill->stage(x.rows, x.columns), x.values.data(), x.values.size() * sizeof(double));
+                added();
+            }
         }
         submit();
         // Drain the remaining slots oldest first.

@@ -48,6 +48,37 @@ def test_cxx_api_binary():
     assert "api_tests passed" in r.stdout
 
 
+def test_dispatch_reader_binary():
+    # MatDispatchReader / PinnedHostTaskBuffer with real pinned memory.
+    r = _run([os.path.join(BIN, "jsvd-dispatch-tests")])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "dispatch_tests passed" in r.stdout
+
+
+def test_cli_mat_device_byte_swap_matches_txt(tmp_path):
+    # A *.mat stream is staged in network byte order and swapped on the
+    # device; a *.txt stream (max_digits10, exact) is decoded on the host.
+    # Both must hand the solver the same bits, so the outputs are identical.
+    rng = np.random.default_rng(17)
+    mats = [rng.standard_normal((int(m), int(n))) * 10.0 ** rng.uniform(-3, 3)
+            for n in rng.integers(1, 40, size=300) for m in [n + rng.integers(0, 40)]]
+    mats.append(rng.standard_normal((700, 300)))  # a large matrix next to the batched ones
+    src_m, src_t = tmp_path / "in.mat", tmp_path / "in.txt"
+    dst_m, dst_t = tmp_path / "out_m.mat", tmp_path / "out_t.mat"
+    jio.write_mat(src_m, mats)
+    jio.write_txt(src_t, mats)
+    for src, dst in ((src_m, dst_m), (src_t, dst_t)):
+        r = _run([CLI, src, dst, "-e", "1e-14", "-s", "40", "-q"])
+        assert r.returncode == 0, r.stderr
+    a, b = jio.read_mat(dst_m), jio.read_mat(dst_t)
+    assert len(a) == len(b) == 3 * len(mats)
+    for x, y in zip(a, b):
+        assert x.shape == y.shape and np.array_equal(x, y)
+    refs = oracle.solve_batch(mats[:40], 1e-14, 40)
+    for k, ref in enumerate(refs):
+        assert np.max(np.abs(a[3 * k + 1][0] - ref[1])) <= 1e-12 * ref[1].max()
+
+
 def test_cli_mat_stream_two_batches(tmp_path):
     # 5000 uneven matrices: two device batches (4096 + 904) in flight.
     mats, sig = gen.uneven_batch(5000, seed=321)
