@@ -134,6 +134,29 @@ typedef struct jsvd_plan_info
     int32_t blocked_matrices;  /* large matrices on the blocked path */
 } jsvd_plan_info;
 
+/* Packed layout of a batch (no reference counterpart: the reference indexes
+ * one matrix with 32-bit int arithmetic, src/domain/matrix_index.cuh:12-27,
+ * which overflows at 2^31 elements).  Host only; needs no device.  Checks
+ * every shape as jsvd_plan_create does, then writes the 64-bit element
+ * offsets of matrix k in the packed A / U, V and sigma buffers (each output
+ * array may be NULL) and the totals {A / U, V, sigma} to totals[0..2] (may be
+ * NULL).  JSVD_EINVAL when a buffer would exceed 2^60 elements. */
+jsvd_status jsvd_packed_layout(int64_t count, const int64_t *m, const int64_t *n, int64_t *a_off, int64_t *v_off,
+                               int64_t *s_off, int64_t *totals);
+
+/* Device workspace of one matrix on the blocked path (no reference
+ * counterpart).  Host only.  For block_width 16 or 32 and an m x n matrix
+ * with n >= 2 * block_width: rows padded to lda (a multiple of 64), columns
+ * to nv (an even number of blocks), the block pairs per round, the Gram row
+ * chunks, and the element counts of the column-major A and V workspaces and
+ * of the partial-Gram buffer.  JSVD_EINVAL for other widths or shapes. */
+typedef struct jsvd_blocked_layout
+{
+    int64_t lda, nv, blocks, pairs, chunk_rows, chunks;
+    int64_t a_elements, v_elements, gram_elements;
+} jsvd_blocked_layout;
+jsvd_status jsvd_blocked_layout_f64(int64_t m, int64_t n, int32_t block_width, jsvd_blocked_layout *out);
+
 /* Builds the device work-list for count matrices of shapes m[k] x n[k].
  * Allocates all device memory the plan needs; execute allocates nothing. */
 jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, const jsvd_config *cfg,

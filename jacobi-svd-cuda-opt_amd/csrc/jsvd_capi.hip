@@ -109,6 +109,47 @@ namespace
         return JSVD_OK;
     }
 
+    // Largest packed buffer, in elements (2^63 bytes).
+    constexpr int64_t kMaxPackedElements = int64_t(1) << 60;
+
+    // Offsets of the packed layout (every shape checked first); any output
+    // may be null.  All arithmetic is 64-bit, with m * n compared by
+    // division so that no product can overflow.
+    jsvd_status packed_layout(int64_t count, const int64_t *m, const int64_t *n, int64_t *a_off, int64_t *v_off,
+                              int64_t *s_off, int64_t *totals)
+    {
+        if (count < 0 || (count > 0 && (!m || !n)))
+            return fail(JSVD_EINVAL, "Input batch description is invalid.");
+        int64_t ao = 0, vo = 0, so = 0;
+        for (int64_t k = 0; k < count; ++k)
+        {
+            if (m[k] <= 0 || n[k] <= 0)
+                return fail(JSVD_EINVAL, "Input matrix shape must be non-zero.");
+            JSVD_CHECK(validate_shape(static_cast<uint64_t>(m[k]), static_cast<uint64_t>(n[k])));
+            if (m[k] > kMaxPackedElements / n[k])
+                return fail(JSVD_EINVAL, "Packed batch exceeds 2^60 elements.");
+            const int64_t mn = m[k] * n[k], nn = n[k] * n[k]; // n <= 2^30: nn <= 2^60
+            if (ao > kMaxPackedElements - mn || vo > kMaxPackedElements - nn || so > kMaxPackedElements - n[k])
+                return fail(JSVD_EINVAL, "Packed batch exceeds 2^60 elements.");
+            if (a_off)
+                a_off[k] = ao;
+            if (v_off)
+                v_off[k] = vo;
+            if (s_off)
+                s_off[k] = so;
+            ao += mn;
+            vo += nn;
+            so += n[k];
+        }
+        if (totals)
+        {
+            totals[0] = ao;
+            totals[1] = vo;
+            totals[2] = so;
+        }
+        return JSVD_OK;
+    }
+
     // threads_per_block maps to a workgroup size: a multiple of 64 in [64,1024]
     // (the reference rounds to a multiple of 32 in [32,1024],
     // src/domain/jacobi_schedule.cuh:64-69).
@@ -791,12 +832,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     if (cfg_in)
         cfg = *cfg_in;
     JSVD_CHECK(validate_config(cfg));
-    for (int64_t k = 0; k < count; ++k)
-    {
-        if (m[k] <= 0 || n[k] <= 0)
-            return fail(JSVD_EINVAL, "Input matrix shape must be non-zero.");
-        JSVD_CHECK(validate_shape(static_cast<uint64_t>(m[k]), static_cast<uint64_t>(n[k])));
-    }
+    std::vector<int64_t> a_off(count), v_off(count), s_off(count);
+    int64_t totals[3];
+    JSVD_CHECK(packed_layout(count, m, n, a_off.data(), v_off.data(), s_off.data(), totals));
 
     auto plan = std::make_unique<jsvd_plan>();
     JSVD_HIP(hipGetDevice(&plan->device));
@@ -804,10 +842,10 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     plan->count = count;
     plan->m.assign(m, m + count);
     plan->n.assign(n, n + count);
-    plan->a_off.resize(count);
-    plan->v_off.resize(count);
-    plan->s_off.resize(count);
-    int64_t ao = 0, vo = 0, so = 0, big_mn = 0, big_nn = 0, big_n = 0;
+    plan->a_off = std::move(a_off);
+    plan->v_off = std::move(v_off);
+    plan->s_off = std::move(s_off);
+    int64_t big_mn = 0, big_nn = 0, big_n = 0;
     int64_t big_gram = 0, big_q = 0, big_pairs = 0;
     hipDeviceProp_t prop;
     JSVD_HIP(hipGetDeviceProperties(&prop, plan->device));
@@ -843,12 +881,6 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     };
     for (int64_t k = 0; k < count; ++k)
     {
-        plan->a_off[k] = ao;
-        plan->v_off[k] = vo;
-        plan->s_off[k] = so;
-        ao += m[k] * n[k];
-        vo += n[k] * n[k];
-        so += n[k];
         int cls = preferred_class(m[k], n[k], any_four_wave);
         int64_t need = cls >= 0 ? jsvd::class_lds_bytes(cls, m[k], n[k]) : jsvd::lds_bytes_f64(m[k], n[k]);
         if (need <= jsvd::kMaxLdsBytes)
@@ -911,8 +943,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         }
         else if (!uses_blocked(cfg, n[k]) && m[k] <= jsvd::wave_sweeps_max_rows())
             plan->wave.push_back(k);
-        else if (!uses_blocked(cfg, n[k]) && n[k] <= kWaveMaxColumns)
-            plan->wave.push_back(k);
+        else if (!uses_blocked(cfg, n[k]) && n[k] <= kWaveMaxColumns && m[k] <= INT32_MAX)
+            plan->wave.push_back(k); // the wave kernels count rows in 32-bit ints
         else
         {
             plan->large.push_back(k);
@@ -933,9 +965,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             big_n = std::max(big_n, n[k]);
         }
     }
-    plan->a_elems = ao;
-    plan->v_elems = vo;
-    plan->s_elems = so;
+    plan->a_elems = totals[0];
+    plan->v_elems = totals[1];
+    plan->s_elems = totals[2];
 
     // Longest-processing-time-first inside each bucket (a sweep costs about
     // n^2 (m + n)), and the buckets with the most work are launched first.
@@ -1363,7 +1395,18 @@ jsvd_status jsvd_plan_metrics_f64(jsvd_plan *p, const double *d_a, const double 
     if (p->mt_count < 0)
     {
         // 32 x 32 output tiles per matrix: U^T U and V^T V (n x n each), then
-        // the m x n reconstruction.
+        // the m x n reconstruction.  The tile kernel reads the row count as
+        // a 32-bit int and the tile list is indexed by 32-bit ints.
+        int64_t tile_total = 0;
+        for (int64_t k = 0; k < p->count; ++k)
+        {
+            if (p->m[k] > INT32_MAX)
+                return fail(JSVD_EINVAL, "metrics: matrices of more than 2^31 - 1 rows are not supported.");
+            const int64_t tn = (p->n[k] + 31) / 32, tm = (p->m[k] + 31) / 32;
+            tile_total += tn * (2 * tn + tm);
+            if (tile_total > int64_t(0x7fffffff))
+                return fail(JSVD_EINVAL, "metrics: the batch needs more than 2^31 - 1 output tiles.");
+        }
         std::vector<jsvd::MetricTile> tiles;
         std::vector<int64_t> first(p->count + 1, 0);
         for (int64_t k = 0; k < p->count; ++k)
@@ -1376,8 +1419,6 @@ jsvd_status jsvd_plan_metrics_f64(jsvd_plan *p, const double *d_a, const double 
                         tiles.push_back({static_cast<int32_t>(k), kind, ti, tj});
         }
         first[p->count] = static_cast<int64_t>(tiles.size());
-        if (tiles.size() > size_t(0x7fffffff))
-            return fail(JSVD_EINVAL, "metrics: the batch needs more than 2^31 - 1 output tiles.");
         JSVD_CHECK(p->mt_tiles.alloc(tiles.size()));
         JSVD_CHECK(p->mt_first.alloc(first.size()));
         JSVD_CHECK(p->mt_partial.alloc(2 * std::max<size_t>(tiles.size(), 1)));
@@ -1390,6 +1431,40 @@ jsvd_status jsvd_plan_metrics_f64(jsvd_plan *p, const double *d_a, const double 
     JSVD_HIP(jsvd::launch_metrics_f64(p->mt_tiles.ptr, p->mt_count, p->mt_first.ptr, p->d_all.ptr,
                                       static_cast<int>(p->count), d_a, d_u, d_sigma, d_v, p->cfg.epsilon,
                                       p->mt_partial.ptr, d_out, static_cast<hipStream_t>(stream_ptr)));
+    return JSVD_OK;
+}
+
+jsvd_status jsvd_packed_layout(int64_t count, const int64_t *m, const int64_t *n, int64_t *a_off, int64_t *v_off,
+                               int64_t *s_off, int64_t *totals)
+{
+    g_last_error.clear();
+    return packed_layout(count, m, n, a_off, v_off, s_off, totals);
+}
+
+jsvd_status jsvd_blocked_layout_f64(int64_t m, int64_t n, int32_t block_width, jsvd_blocked_layout *out)
+{
+    g_last_error.clear();
+    if (!out)
+        return fail(JSVD_EINVAL, "blocked layout output pointer is null.");
+    if (block_width != 16 && block_width != 32)
+        return fail(JSVD_EINVAL, "block_width must be 0, 16 or 32.");
+    if (m <= 0 || n <= 0)
+        return fail(JSVD_EINVAL, "Input matrix shape must be non-zero.");
+    JSVD_CHECK(validate_shape(static_cast<uint64_t>(m), static_cast<uint64_t>(n)));
+    if (n < 2 * int64_t(block_width))
+        return fail(JSVD_EINVAL, "The blocked path needs n >= 2 * block_width.");
+    const BlockedShape sh = blocked_shape(block_width, m, n);
+    if (sh.lda > kMaxPackedElements / sh.nv)
+        return fail(JSVD_EINVAL, "Packed batch exceeds 2^60 elements.");
+    out->lda = sh.lda;
+    out->nv = sh.nv;
+    out->blocks = sh.nb;
+    out->pairs = sh.pairs;
+    out->chunk_rows = sh.chunk_rows;
+    out->chunks = sh.chunks;
+    out->a_elements = sh.lda * sh.nv;
+    out->v_elements = int64_t(sh.nv) * sh.nv;
+    out->gram_elements = int64_t(sh.pairs) * sh.chunks * 4 * int64_t(block_width) * block_width;
     return JSVD_OK;
 }
 

@@ -22,6 +22,8 @@
 #include "jsvd_device.hpp"
 #include "jsvd_kernels.hpp"
 
+#include <algorithm>
+
 namespace jsvd
 {
     namespace
@@ -690,37 +692,45 @@ namespace jsvd
 
         // Tiled transpose through a padded LDS tile (32 x 33 doubles: the
         // extra column puts the 32 elements of a tile column in distinct bank
-        // pairs).  Block 32 x 8.
+        // pairs).  Block 32 x 8.  Tiles are numbered row of tiles by row of
+        // tiles on a one-dimensional grid that strides over them, so any
+        // shape launches (a two-dimensional grid would cap the tile rows at
+        // 65535, i.e. about 2M matrix rows).
         __global__ __launch_bounds__(256) void transpose_f64(const double *__restrict__ in, int64_t ld_in,
                                                              double *__restrict__ out, int64_t ld_out, int64_t rows,
                                                              int64_t cols, const double *__restrict__ row_scale,
                                                              double eps)
         {
             __shared__ double tile[32][33];
-            const int64_t c0 = (int64_t)blockIdx.x * 32, r0 = (int64_t)blockIdx.y * 32;
+            const int64_t tiles_c = (cols + 31) / 32, tiles = tiles_c * ((rows + 31) / 32);
             const int tx = threadIdx.x, ty = threadIdx.y;
-#pragma unroll
-            for (int k = 0; k < 32; k += 8)
+            for (int64_t t = blockIdx.x; t < tiles; t += gridDim.x)
             {
-                const int64_t r = r0 + ty + k, c = c0 + tx;
-                if (r < rows && c < cols)
+                const int64_t r0 = (t / tiles_c) * 32, c0 = (t % tiles_c) * 32;
+#pragma unroll
+                for (int k = 0; k < 32; k += 8)
                 {
-                    double x = in[r * ld_in + c];
-                    if (row_scale)
+                    const int64_t r = r0 + ty + k, c = c0 + tx;
+                    if (r < rows && c < cols)
                     {
-                        const double sc = row_scale[r];
-                        x = (sc > eps) ? (x / sc) : 0.0;
+                        double x = in[r * ld_in + c];
+                        if (row_scale)
+                        {
+                            const double sc = row_scale[r];
+                            x = (sc > eps) ? (x / sc) : 0.0;
+                        }
+                        tile[ty + k][tx] = x;
                     }
-                    tile[ty + k][tx] = x;
                 }
-            }
-            __syncthreads();
+                __syncthreads();
 #pragma unroll
-            for (int k = 0; k < 32; k += 8)
-            {
-                const int64_t c = c0 + ty + k, r = r0 + tx;
-                if (c < cols && r < rows)
-                    out[c * ld_out + r] = tile[tx][ty + k];
+                for (int k = 0; k < 32; k += 8)
+                {
+                    const int64_t c = c0 + ty + k, r = r0 + tx;
+                    if (c < cols && r < rows)
+                        out[c * ld_out + r] = tile[tx][ty + k];
+                }
+                __syncthreads();
             }
         }
 
@@ -945,7 +955,8 @@ namespace jsvd
     {
         if (rows == 0 || cols == 0)
             return hipSuccess;
-        dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32));
+        const int64_t tiles = ((cols + 31) / 32) * ((rows + 31) / 32);
+        const unsigned grid = static_cast<unsigned>(std::min<int64_t>(tiles, int64_t(1) << 20));
         transpose_f64<<<grid, dim3(32, 8), 0, stream>>>(in, ld_in, out, ld_out, rows, cols, row_scale, eps);
         return hipGetLastError();
     }

@@ -19,7 +19,7 @@ from ._lib import (JsvdConfig, JsvdError, JsvdInvalidArgument, LAYOUT_AUTO, LAYO
 __all__ = [
     "JsvdConfig", "JsvdError", "JsvdInvalidArgument", "PRECISION_FP64", "PRECISION_MIXED", "LAYOUT_AUTO",
     "STOP_CONVERGED", "STOP_MAX_SWEEPS",
-    "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets",
+    "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets", "blocked_layout",
     "autotune_batch", "tune_class", "solve_device_noalloc",
 ]
 
@@ -74,12 +74,31 @@ def svd_batched(mats, cfg: JsvdConfig | None = None, with_stop: bool = False, **
 
 
 def pack_offsets(ms, ns):
-    """Element offsets of the packed layout in include/jsvd_c.h: (a_off, v_off, s_off, totals)."""
-    ms = np.asarray(ms, dtype=np.int64)
-    ns = np.asarray(ns, dtype=np.int64)
-    a_sz, v_sz, s_sz = ms * ns, ns * ns, ns
-    off = lambda sz: np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64)
-    return off(a_sz), off(v_sz), off(s_sz), (int(a_sz.sum()), int(v_sz.sum()), int(s_sz.sum()))
+    """Element offsets of the packed layout in include/jsvd_c.h: (a_off, v_off, s_off, totals).
+
+    Computed by the library (``jsvd_packed_layout``, 64-bit and overflow
+    checked, the same code the plans use); needs no device.
+    """
+    lib = load()
+    ms = np.ascontiguousarray(ms, dtype=np.int64)
+    ns = np.ascontiguousarray(ns, dtype=np.int64)
+    count = len(ms)
+    if len(ns) != count:
+        raise ValueError("ms and ns must have the same length")
+    a_off, v_off, s_off = (np.zeros(count, dtype=np.int64) for _ in range(3))
+    totals = np.zeros(3, dtype=np.int64)
+    p = lambda x: x.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    _lib.check(lib.jsvd_packed_layout(count, p(ms), p(ns), p(a_off), p(v_off), p(s_off), p(totals)))
+    return a_off, v_off, s_off, (int(totals[0]), int(totals[1]), int(totals[2]))
+
+
+def blocked_layout(m, n, block_width):
+    """Padded shape and workspace sizes of one matrix on the blocked path
+    (``jsvd_blocked_layout_f64``); needs no device."""
+    lib = load()
+    out = _lib.JsvdBlockedLayout()
+    _lib.check(lib.jsvd_blocked_layout_f64(int(m), int(n), int(block_width), ctypes.byref(out)))
+    return {name: int(getattr(out, name)) for name, _ in out._fields_}
 
 
 class BatchPlan:

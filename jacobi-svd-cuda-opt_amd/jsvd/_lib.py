@@ -78,6 +78,11 @@ class JsvdBatchTuneReport(ctypes.Structure):
     ]
 
 
+class JsvdBlockedLayout(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_int64) for name in ("lda", "nv", "blocks", "pairs", "chunk_rows", "chunks",
+                                                    "a_elements", "v_elements", "gram_elements")]
+
+
 class JsvdError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(message)
@@ -109,6 +114,9 @@ SIGNATURES = {
                                                      ctypes.POINTER(JsvdConfig), ctypes.POINTER(_dp),
                                                      ctypes.POINTER(_dp), ctypes.POINTER(_dp), _i32p, _i32p]),
     "jsvd_plan_execute_status_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "jsvd_packed_layout": (ctypes.c_int, [ctypes.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p, _i64p]),
+    "jsvd_blocked_layout_f64": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int64, ctypes.c_int32,
+                                               ctypes.POINTER(JsvdBlockedLayout)]),
     "jsvd_plan_create": (ctypes.c_int, [ctypes.c_int64, _i64p, _i64p, ctypes.POINTER(JsvdConfig),
                                         ctypes.POINTER(_vp)]),
     "jsvd_plan_execute_f64": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
