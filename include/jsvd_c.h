@@ -212,9 +212,9 @@ jsvd_status jsvd_solve_device_noalloc_f64(int64_t count, const int64_t *m, const
  * NULL = zero), writes
  *   gA = U [ (F o (U^T gU - gU^T U)) S + S (F o (V^T gV - gV^T V)) + diag(gsigma) ] V^T
  *        + (I - U U^T) gU S^-1 V^T,   F_ij = 1/(s_j^2 - s_i^2) (i != j),
- * packed like A.  Requires distinct nonzero singular values and
- * (n'^2 + m'(n'+1)) doubles of LDS per matrix, m' and n' being m and n rounded
- * up to 16 (n <= 80 at m = 2n), in FP64 and FP32 alike.
+ * packed like A.  Requires distinct nonzero singular values, m < 2^31, and
+ * (n'^2 + max(n', 64)(n'+1)) doubles of LDS per matrix, n' being n rounded up
+ * to 16 (n <= 96, any m), in FP64 and FP32 alike.
  * Asynchronous on `stream`; allocates nothing. */
 jsvd_status jsvd_plan_backward_f64(jsvd_plan *plan, const double *d_u, const double *d_sigma, const double *d_v,
                                    const double *d_gu, const double *d_gsigma, const double *d_gv, double *d_ga,

@@ -361,6 +361,7 @@ struct jsvd_plan
     // largest LDS need of the adjoint in FP64 / FP32.
     DevBuf<jsvd::WorkItem> d_all;
     int64_t bw_lds_f64 = 0, bw_lds_f32 = 0;
+    int64_t max_rows = 0; // largest m of the plan
     // Buckets beyond the first run on plan-owned streams forked from the
     // caller's stream and joined back (event record / wait only).
     std::vector<hipStream_t> aux;
@@ -1179,6 +1180,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             all[k].a_off = plan->a_off[k];
             all[k].v_off = plan->v_off[k];
             all[k].s_off = plan->s_off[k];
+            plan->max_rows = std::max(plan->max_rows, m[k]);
             plan->bw_lds_f64 = std::max(plan->bw_lds_f64, jsvd::backward_lds_bytes(m[k], n[k], 8));
             plan->bw_lds_f32 = std::max(plan->bw_lds_f32, jsvd::backward_lds_bytes(m[k], n[k], 4));
         }
@@ -1510,8 +1512,10 @@ namespace
             return fail(JSVD_EINVAL, "device buffer pointer is null.");
         const int64_t need = sizeof(T) == 8 ? p->bw_lds_f64 : p->bw_lds_f32;
         if (need > jsvd::kMaxLdsBytes)
-            return fail(JSVD_EINVAL, "SVD backward needs (n'^2 + m'(n'+1)) doubles of LDS per matrix (m', n' = m, n "
+            return fail(JSVD_EINVAL, "SVD backward needs (n'^2 + max(n', 64)(n'+1)) doubles of LDS per matrix (n' = n "
                                      "rounded up to 16); a matrix of this plan exceeds the 159 KiB budget.");
+        if (p->max_rows > INT32_MAX)
+            return fail(JSVD_EINVAL, "SVD backward supports matrices of up to 2^31 - 1 rows.");
         int dev = 0;
         JSVD_HIP(hipGetDevice(&dev));
         if (dev != p->device)

@@ -7,18 +7,31 @@
 //   gA = U M V^T + (I - U U^T) gU diag(s)^-1 V^T
 // which is rewritten as gA = (U N + gU diag(s)^-1) V^T with
 //   N = M - J diag(s)^-1
-// so that only four small products are needed.  One workgroup per matrix
-// forms them on FP64 MFMA (v_mfma_f64_16x16x4f64) in 16 x 16 tiles, the four
-// waves taking tiles round-robin; operands are read straight from HBM/L2
-// (U, gU, V, gV) or from LDS (N, W), with zero padding of m and n up to
-// multiples of 16.  LDS holds N (np x np) and W (mp x (np + 1), which holds K
-// while N is formed): (np^2 + mp (np + 1)) doubles, np/mp = n/m rounded up to
-// 16.  FP32 inputs are widened to FP64 on load and the result rounded once.
-// The result is invariant to a common permutation of (U, s, V) columns, so
-// the unsorted column order of the solver needs no sorting.  Distinct
-// nonzero singular values are required (F and 1/s are unbounded otherwise),
-// as for any SVD adjoint.
+// so that only four small products are needed.  One workgroup of four
+// waves per matrix forms them on FP64 MFMA (v_mfma_f64_16x16x4f64) in
+// 16 x 16 tiles, with m and n padded to multiples of 16 by zeros:
+//  1. J and K: a wave takes a row of output tiles (one 16-column block of
+//     U or V against up to four 16-column blocks of gU or gV), loading
+//     32 rows of every operand into registers before the eight MFMAs that
+//     use them, so a whole chunk of loads is in flight at once;
+//  2. N from J, K, s, gs, elementwise in LDS;
+//  3. per 16-row block of A, by one wave: W = U N + gU diag(s)^-1 (16 x n',
+//     kept in the wave's own LDS slice), then at once gA = W V^T for that
+//     block, so W never exists whole and no barrier separates the two.
+// LDS: N (n'^2 doubles) and a region of max(n', 64) x (n' + 1) doubles that
+// holds K while N is formed and then the four waves' 16-row W slices; it
+// does not depend on m (66 KB at n = 64: two workgroups per CU).  FP32
+// inputs are widened to FP64 on load and the result rounded once: F and
+// 1/s amplify the rounding of every product by up to the inverse relative
+// gap of the singular values, which FP32 products would carry straight into
+// the gradient, and the kernel is bound by load latency, not by MFMA
+// throughput.  The result is invariant to a common permutation of (U, s, V)
+// columns, so the unsorted column order of the solver needs no sorting.
+// Distinct nonzero singular values are required (F and 1/s are unbounded
+// otherwise), as for any SVD adjoint.
 #include "jsvd_kernels.hpp"
+
+#include <algorithm>
 
 namespace jsvd
 {
@@ -26,10 +39,18 @@ namespace jsvd
     {
         typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+        constexpr int kChunk = 8; // MFMAs per register chunk: 32 rows of the summation index
+        constexpr int kTileGroup = 4; // output tiles per operand chunk
+
         template <typename T>
         __device__ __forceinline__ double load_or_zero(const T *p, int64_t idx, bool ok)
         {
             return ok ? static_cast<double>(p[idx]) : 0.0;
+        }
+
+        __device__ __forceinline__ void wave_fence()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
 
         template <typename T, int BS>
@@ -49,46 +70,72 @@ namespace jsvd
             const int m = w.m, n = w.n;
             const int np = (n + 15) & ~15, mp = (m + 15) & ~15;
             const int tn = np / 16, tm = mp / 16;
-            const int ldw = np + 1; // odd stride: the A-operand reads of W spread over the banks
+            const int ld = np + 1; // odd stride of K and W: A-operand reads spread over the banks
             const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, i = l & 15, g = l >> 4;
             const T *u = U + w.a_off, *gu = gU ? gU + w.a_off : nullptr;
             const T *v = V + w.v_off, *gv = gV ? gV + w.v_off : nullptr;
             const T *s = S + w.s_off, *gs = gS ? gS + w.s_off : nullptr;
             double *Nm = lds;           // np x np row-major: J, then N
-            double *Wm = lds + np * np; // mp x ldw row-major: K (first np rows), then W
+            double *Rm = lds + np * np; // K (np x ld), then the waves' W slices (16 x ld each)
             // MFMA operand element (l % 16, l / 16) of A and (l / 16, l % 16) of
             // B; result item r of lane l is tile row l/16 + 4r, column l % 16.
 
-            // J = U^T gU and K = V^T gV.
-            for (int t = wave; t < 2 * tn * tn; t += WAVES)
+            // 1. J = U^T gU (tile rows 0 .. tn-1) and K = V^T gV (tn .. 2tn-1).
+            for (int job = wave; job < 2 * tn; job += WAVES)
             {
-                const bool is_k = t >= tn * tn;
-                const int tt = is_k ? t - tn * tn : t;
-                const int ti = tt / tn, tj = tt - (tt / tn) * tn;
+                const bool is_k = job >= tn;
+                const int ti = is_k ? job - tn : job;
                 const T *a = is_k ? v : u, *b = is_k ? gv : gu;
                 const int rows = is_k ? n : m;
-                f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-                if (b)
+                double *dst = is_k ? Rm : Nm;
+                const int ldd = is_k ? ld : np;
+                const int ca = ti * 16 + i;
+                for (int tj0 = 0; tj0 < tn; tj0 += kTileGroup)
                 {
-                    const int ca = ti * 16 + i, cb = tj * 16 + i;
-#pragma unroll 4
-                    for (int k0 = 0; k0 < rows; k0 += 4)
-                    {
-                        const int k = k0 + g;
-                        const double av = load_or_zero(a, int64_t(k) * n + ca, k < rows && ca < n);
-                        const double bv = load_or_zero(b, int64_t(k) * n + cb, k < rows && cb < n);
-                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-                    }
-                }
-                double *dst = is_k ? Wm : Nm;
-                const int ld = is_k ? ldw : np;
+                    f64x4 acc[kTileGroup];
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    dst[(ti * 16 + g + 4 * r) * ld + tj * 16 + i] = acc[r];
+                    for (int t = 0; t < kTileGroup; ++t)
+                        acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    if (b)
+                    {
+                        for (int k0 = 0; k0 < rows; k0 += 4 * kChunk)
+                        {
+                            double av[kChunk], bv[kTileGroup][kChunk];
+#pragma unroll
+                            for (int c = 0; c < kChunk; ++c)
+                            {
+                                const int k = k0 + 4 * c + g;
+                                av[c] = load_or_zero(a, int64_t(k) * n + ca, k < rows && ca < n);
+#pragma unroll
+                                for (int t = 0; t < kTileGroup; ++t)
+                                {
+                                    const int cb = (tj0 + t) * 16 + i;
+                                    bv[t][c] = load_or_zero(b, int64_t(k) * n + cb, k < rows && cb < n);
+                                }
+                            }
+#pragma unroll
+                            for (int t = 0; t < kTileGroup; ++t)
+                                if (tj0 + t < tn)
+                                {
+#pragma unroll
+                                    for (int c = 0; c < kChunk; ++c)
+                                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[t][c], acc[t], 0, 0, 0);
+                                }
+                        }
+                    }
+#pragma unroll
+                    for (int t = 0; t < kTileGroup; ++t)
+                        if (tj0 + t < tn)
+                        {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                dst[(ti * 16 + g + 4 * r) * ldd + (tj0 + t) * 16 + i] = acc[t][r];
+                        }
+                }
             }
             __syncthreads();
-            // N in place of J: the thread of (r, c), r <= c, also writes (c, r);
-            // padding rows and columns become zero.
+            // 2. N in place of J: the thread of (r, c), r <= c, also writes
+            // (c, r); padding rows and columns become zero.
             for (int idx = threadIdx.x; idx < np * np; idx += BS)
             {
                 const int r = idx / np, c = idx - (idx / np) * np;
@@ -105,7 +152,7 @@ namespace jsvd
                         const double sc = static_cast<double>(s[c]);
                         const double f = 1.0 / (sc * sc - sr * sr); // F_rc; F_cr = -F_rc
                         const double jrc = Nm[r * np + c], jcr = Nm[c * np + r];
-                        const double krc = Wm[r * ldw + c], kcr = Wm[c * ldw + r];
+                        const double krc = Rm[r * ld + c], kcr = Rm[c * ld + r];
                         nrc = ((jrc - jcr) * sc + sr * (krc - kcr)) * f - jrc / sc;
                         ncr = ((jcr - jrc) * sr + sc * (kcr - krc)) * (-f) - jcr / sr;
                     }
@@ -115,65 +162,111 @@ namespace jsvd
                     Nm[c * np + r] = ncr;
             }
             __syncthreads();
-            // W = U N + gU diag(s)^-1.
-            for (int t = wave; t < tm * tn; t += WAVES)
-            {
-                const int ti = t / tn, tj = t - (t / tn) * tn;
-                f64x4 acc;
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                {
-                    const int row = ti * 16 + g + 4 * r, col = tj * 16 + i;
-                    acc[r] = (gu && row < m && col < n)
-                                 ? static_cast<double>(gu[int64_t(row) * n + col]) / static_cast<double>(s[col])
-                                 : 0.0;
-                }
-                const int row = ti * 16 + i;
-#pragma unroll 4
-                for (int k0 = 0; k0 < np; k0 += 4)
-                {
-                    const int k = k0 + g;
-                    const double av = load_or_zero(u, int64_t(row) * n + k, row < m && k < n);
-                    const double bv = Nm[k * np + tj * 16 + i];
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    Wm[(ti * 16 + g + 4 * r) * ldw + tj * 16 + i] = acc[r];
-            }
-            __syncthreads();
-            // gA = W V^T.
+            // 3. Per 16-row block of A: W = U N + gU diag(s)^-1, then gA = W V^T.
+            double *Wb = Rm + wave * 16 * ld;
             T *ga = gA + w.a_off;
-            for (int t = wave; t < tm * tn; t += WAVES)
+            for (int ti = wave; ti < tm; ti += WAVES)
             {
-                const int ti = t / tn, tj = t - (t / tn) * tn;
-                f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-                const int c = tj * 16 + i;
-#pragma unroll 4
-                for (int k0 = 0; k0 < np; k0 += 4)
+                const int ra = ti * 16 + i; // this lane's A-operand row of U
+                for (int tj0 = 0; tj0 < tn; tj0 += kTileGroup)
                 {
-                    const int k = k0 + g;
-                    const double av = Wm[(ti * 16 + i) * ldw + k];
-                    const double bv = load_or_zero(v, int64_t(c) * n + k, c < n && k < n);
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-                }
+                    f64x4 acc[kTileGroup];
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                {
-                    const int row = ti * 16 + g + 4 * r;
-                    if (row < m && c < n)
-                        ga[int64_t(row) * n + c] = static_cast<T>(acc[r]);
+                    for (int t = 0; t < kTileGroup; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                        {
+                            const int row = ti * 16 + g + 4 * r, col = (tj0 + t) * 16 + i;
+                            acc[t][r] = (gu && row < m && col < n)
+                                            ? static_cast<double>(gu[int64_t(row) * n + col]) /
+                                                  static_cast<double>(s[col])
+                                            : 0.0;
+                        }
+                    for (int k0 = 0; k0 < np; k0 += 4 * kChunk)
+                    {
+                        double av[kChunk], bv[kTileGroup][kChunk];
+#pragma unroll
+                        for (int c = 0; c < kChunk; ++c)
+                        {
+                            const int k = k0 + 4 * c + g;
+                            av[c] = load_or_zero(u, int64_t(ra) * n + k, ra < m && k < n);
+#pragma unroll
+                            for (int t = 0; t < kTileGroup; ++t)
+                                bv[t][c] = (k < np && tj0 + t < tn) ? Nm[k * np + (tj0 + t) * 16 + i] : 0.0;
+                        }
+#pragma unroll
+                        for (int t = 0; t < kTileGroup; ++t)
+                            if (tj0 + t < tn)
+                            {
+#pragma unroll
+                                for (int c = 0; c < kChunk; ++c)
+                                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[t][c], acc[t], 0, 0, 0);
+                            }
+                    }
+#pragma unroll
+                    for (int t = 0; t < kTileGroup; ++t)
+                        if (tj0 + t < tn)
+                        {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                Wb[(g + 4 * r) * ld + (tj0 + t) * 16 + i] = acc[t][r];
+                        }
                 }
+                wave_fence(); // the slice is written by every lane and read across lanes below
+                for (int tj0 = 0; tj0 < tn; tj0 += kTileGroup)
+                {
+                    f64x4 acc[kTileGroup];
+#pragma unroll
+                    for (int t = 0; t < kTileGroup; ++t)
+                        acc[t] = f64x4{0.0, 0.0, 0.0, 0.0};
+                    for (int k0 = 0; k0 < np; k0 += 4 * kChunk)
+                    {
+                        double av[kChunk], bv[kTileGroup][kChunk];
+#pragma unroll
+                        for (int c = 0; c < kChunk; ++c)
+                        {
+                            const int k = k0 + 4 * c + g;
+                            av[c] = k < np ? Wb[i * ld + k] : 0.0;
+#pragma unroll
+                            for (int t = 0; t < kTileGroup; ++t)
+                            {
+                                const int cv = (tj0 + t) * 16 + i;
+                                bv[t][c] = load_or_zero(v, int64_t(cv) * n + k, cv < n && k < n);
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < kTileGroup; ++t)
+                            if (tj0 + t < tn)
+                            {
+#pragma unroll
+                                for (int c = 0; c < kChunk; ++c)
+                                    acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[c], bv[t][c], acc[t], 0, 0, 0);
+                            }
+                    }
+#pragma unroll
+                    for (int t = 0; t < kTileGroup; ++t)
+                    {
+                        const int c = (tj0 + t) * 16 + i;
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                        {
+                            const int row = ti * 16 + g + 4 * r;
+                            if (tj0 + t < tn && row < m && c < n)
+                                ga[int64_t(row) * n + c] = static_cast<T>(acc[t][r]);
+                        }
+                    }
+                }
+                wave_fence(); // the slice's reads are done before the next block overwrites it
             }
         }
     } // namespace
 
     constexpr int kBackwardBlock = 256;
 
-    int64_t backward_lds_bytes(int64_t m, int64_t n, int /*elem*/)
+    int64_t backward_lds_bytes(int64_t /*m*/, int64_t n, int /*elem*/)
     {
-        const int64_t np = (n + 15) / 16 * 16, mp = (m + 15) / 16 * 16;
-        return (np * np + mp * (np + 1)) * 8;
+        const int64_t np = (n + 15) / 16 * 16;
+        return (np * np + std::max<int64_t>(np, 16 * (kBackwardBlock / 64)) * (np + 1)) * 8;
     }
 
     const void *backward_symbol(int elem)

@@ -695,7 +695,7 @@ namespace jsvd
         // pairs).  Block 32 x 8.  Tiles are numbered row of tiles by row of
         // tiles on a one-dimensional grid that strides over them, so any
         // shape launches (a two-dimensional grid would cap the tile rows at
-        // 65535, i.e. about 2M matrix rows).
+        // gfx950's grid-y limit of 65536, i.e. about 2.1M matrix rows).
         __global__ __launch_bounds__(256) void transpose_f64(const double *__restrict__ in, int64_t ld_in,
                                                              double *__restrict__ out, int64_t ld_out, int64_t rows,
                                                              int64_t cols, const double *__restrict__ row_scale,

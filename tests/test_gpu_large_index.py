@@ -7,8 +7,9 @@ large-matrix path (column-major workspace, scaled transpose epilogue) on:
     32-bit int holds, so it takes the grid-barrier kernel, whose row
     indices are 64-bit.  It is built on the device with known singular
     values and vectors;
-  * a (3 * 2^20 + 7) x 5 matrix on the wave kernels: more than 65535 row
-    tiles of 32, which a two-dimensional transpose grid could not launch.
+  * a (3 * 2^20 + 7) x 5 matrix on the wave kernels: 98305 row tiles of 32,
+    more than the grid-y limit (65536) a two-dimensional transpose grid
+    would need.
 Expected values come from the construction (closed form) or from LAPACK.
 Tolerances follow the summation length: a thread of the big solve adds 2^21
 products per dot and a thread of the norm pass 2^23 squares, so results carry
@@ -92,7 +93,7 @@ def test_two_column_matrix_with_more_rows_than_int32():
     np.testing.assert_allclose(norms, [1.0, 1.0], rtol=1e-8)
 
 
-def test_tall_matrix_beyond_65535_row_tiles():
+def test_tall_matrix_beyond_65536_row_tiles():
     import torch
 
     m, n = 3 * (1 << 20) + 7, 5
