@@ -49,7 +49,15 @@ enum
 enum
 {
     JSVD_STOP_CONVERGED = 0,  /* the last sweep rotated no pair: every pair passed the threshold test */
-    JSVD_STOP_MAX_SWEEPS = 1  /* max_sweeps sweeps ran and the last one still rotated a pair */
+    JSVD_STOP_MAX_SWEEPS = 1, /* max_sweeps sweeps ran and the last one still rotated a pair */
+    /* Matrices too large for LDS run kernels whose workgroups wait for each
+     * other (software barriers over a grid sized for full residency).  A
+     * wait longer than 10 s plus 1 s per GB of the launch's matrices (or
+     * the environment variable JSVD_BARRIER_TIMEOUT_MS, read at each
+     * execution) ends the launch instead of hanging: its matrices get this
+     * code and their results are invalid.  The synchronous entry points then
+     * return JSVD_EDEVICE. */
+    JSVD_STOP_BARRIER_TIMEOUT = 2
 };
 
 /* Layout policy values (reference LayoutTransposeMode,

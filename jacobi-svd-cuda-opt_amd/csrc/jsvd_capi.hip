@@ -109,6 +109,10 @@ namespace
         return JSVD_OK;
     }
 
+    const char *const kBarrierTimeoutMessage =
+        "A device barrier timed out: the workgroups of a barrier kernel were not all resident (other kernels "
+        "holding CUs?); the results of the matrices reported with stop reason JSVD_STOP_BARRIER_TIMEOUT are invalid.";
+
     // Largest packed buffer, in elements (2^63 bytes).
     constexpr int64_t kMaxPackedElements = int64_t(1) << 60;
 
@@ -372,6 +376,8 @@ struct jsvd_plan
     DevBuf<double> ws_a, ws_v, ws_s;
     DevBuf<int32_t> ws_flags;
     DevBuf<unsigned int> ws_bar;
+    DevBuf<unsigned int> d_fault; // set by a barrier kernel whose wait hit its spin bound
+    double wave_bytes = 0.0;      // bytes of the grouped-wave matrices (spin bound)
     int big_block = 256;
     int big_resident = 0; // resident workgroups of the grid-barrier kernel (occupancy x CUs)
     hipEvent_t big_in = nullptr, big_out = nullptr;
@@ -432,6 +438,38 @@ namespace
     // such kernel is on the GPU at a time (see launch_persistent_sweeps_f64).
     std::mutex g_barrier_mu;
     std::vector<hipStream_t> g_barrier_streams;
+
+    // Wall-clock ticks a software barrier may wait before its launch is
+    // marked faulted: 10 s plus 1 s per GB of the launch's matrices (a round
+    // streams the largest of them about three times), or
+    // JSVD_BARRIER_TIMEOUT_MS when that is set (0 makes every wait fail at
+    // once, which is how the failure path is tested).
+    uint64_t barrier_spin_limit(int device, double bytes)
+    {
+        static std::mutex mu;
+        static std::vector<int> khz;
+        int rate = 0;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (static_cast<int>(khz.size()) <= device)
+                khz.resize(device + 1, 0);
+            if (khz[device] <= 0)
+            {
+                int r = 0;
+                if (hipDeviceGetAttribute(&r, hipDeviceAttributeWallClockRate, device) != hipSuccess || r <= 0)
+                {
+                    (void)hipGetLastError();
+                    r = 100000; // 100 MHz, the usual rate
+                }
+                khz[device] = r;
+            }
+            rate = khz[device];
+        }
+        double seconds = 10.0 + bytes / 1e9;
+        if (const char *env = std::getenv("JSVD_BARRIER_TIMEOUT_MS"))
+            seconds = std::max(0.0, std::atof(env)) / 1000.0;
+        return static_cast<uint64_t>(seconds * rate * 1000.0);
+    }
 
     jsvd_status barrier_stream(int device, hipStream_t *out)
     {
@@ -526,11 +564,16 @@ namespace
         JSVD_HIP(jsvd::launch_fill_i32(p.ws_flags.ptr, 0, p.cfg.max_sweeps, stream));
         const int half = (n + (n & 1)) / 2;
         int32_t *sweeps_slot = d_sweeps ? d_sweeps + k : nullptr;
-        // Resident workgroups of the grid-barrier kernel, from the plan.
+        // Resident workgroups of the grid-barrier kernel, from the plan.  The
+        // barrier words and the fault word start every launch at zero (a
+        // launch that timed out leaves them anywhere).
         const int grid = std::max(1, std::min(p.big_resident, half));
-        JSVD_HIP(jsvd::launch_persistent_sweeps_f64(p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon,
-                                                    p.cfg.max_sweeps, p.ws_flags.ptr, p.ws_bar.ptr, sweeps_slot,
-                                                    d_stop ? d_stop + k : nullptr, stream));
+        JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.ws_bar.ptr), 0, 2, stream));
+        JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.d_fault.ptr), 0, 1, stream));
+        JSVD_HIP(jsvd::launch_persistent_sweeps_f64(
+            p.big_block, grid, p.ws_a.ptr, p.ws_v.ptr, m, n, p.cfg.epsilon, p.cfg.max_sweeps, p.ws_flags.ptr,
+            p.ws_bar.ptr, sweeps_slot, d_stop ? d_stop + k : nullptr, p.d_fault.ptr,
+            barrier_spin_limit(p.device, 8.0 * double(m) * n), stream));
         return finish_large(p, k, m, n, m, n, d_u, d_sigma, d_v, stream);
     }
 
@@ -596,13 +639,17 @@ namespace
             args.bars = p.wv_bars.ptr;
             args.sweeps = d_sweeps;
             args.stop = d_stop;
+            args.fault = p.d_fault.ptr;
+            args.spin_limit = barrier_spin_limit(p.device, p.wave_bytes);
             if (l.members == 0)
                 JSVD_HIP(jsvd::launch_wave_block_sweeps_f64(l.m_max, l.count, args, stream));
             else
             {
-                // The group barriers' flag words start every launch at zero.
+                // The group barriers' flag words and the fault word start
+                // every launch at zero.
                 JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.wv_bars.ptr), 0,
                                                int64_t(l.groups) * l.members, stream));
+                JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.d_fault.ptr), 0, 1, stream));
                 JSVD_HIP(jsvd::launch_wave_group_sweeps_f64(l.m_max, l.waves, l.groups, args, stream));
             }
         }
@@ -655,6 +702,7 @@ namespace
             p.wave_a_off[e] = a_total;
             p.wave_v_off[e] = v_total;
             a_total += p.m[k] * p.n[k];
+            p.wave_bytes += 8.0 * double(p.m[k]) * double(p.n[k]);
             v_total += p.n[k] * p.n[k];
         }
         int64_t max_flags = 1;
@@ -1200,6 +1248,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     {
         JSVD_HIP(hipEventCreateWithFlags(&plan->big_in, hipEventDisableTiming));
         JSVD_HIP(hipEventCreateWithFlags(&plan->big_out, hipEventDisableTiming));
+        JSVD_CHECK(plan->d_fault.alloc(1));
+        JSVD_HIP(hipMemset(plan->d_fault.ptr, 0, sizeof(unsigned int)));
     }
     if (!plan->large.empty())
     {
@@ -1639,6 +1689,8 @@ jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_
         std::copy(sw.begin(), sw.begin() + count, sweeps);
     if (stop)
         std::copy(sw.begin() + count, sw.end(), stop);
+    if (std::find(sw.begin() + count, sw.end(), JSVD_STOP_BARRIER_TIMEOUT) != sw.end())
+        return fail(JSVD_EDEVICE, kBarrierTimeoutMessage);
     return JSVD_OK;
 }
 

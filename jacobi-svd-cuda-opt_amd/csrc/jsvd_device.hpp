@@ -274,6 +274,31 @@ namespace jsvd::dev
         return x;
     }
 
+    // Bound on the software barriers' waits.  A wait that lasts more than
+    // `limit` ticks of the constant-rate wall clock (wall_clock64, rate from
+    // hipDeviceAttributeWallClockRate) stores 1 to *fault and gives up; any
+    // wait that finds *fault set gives up at once, so every workgroup of the
+    // launch leaves its loops within one bound and the grid drains.  Such a
+    // wait means the workgroups were not all resident (another process's
+    // kernels holding CUs), which the occupancy-sized grids assume.
+    struct SpinBound
+    {
+        unsigned int *fault; // zero at launch
+        uint64_t limit;      // wall-clock ticks
+    };
+
+    __device__ __forceinline__ bool spin_expired(const SpinBound &b, uint64_t t0)
+    {
+        if (__hip_atomic_load(b.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+            return true;
+        if (static_cast<uint64_t>(wall_clock64()) - t0 > b.limit)
+        {
+            __hip_atomic_store(b.fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return true;
+        }
+        return false;
+    }
+
     // Sense-free generation barrier across `nblocks` co-resident workgroups
     // (a launch's grid, or one group of it; grid <= resident workgroups, one
     // such launch on the GPU at a time).  bar[0] counts arrivals, bar[1] is
@@ -285,9 +310,11 @@ namespace jsvd::dev
     // (an acquire load would invalidate this XCD's L2 on every iteration)
     // and acquire once, after the generation has changed.  So each
     // workgroup pays one L2 write-back and one invalidate per barrier.
-    __device__ __forceinline__ void grid_barrier(unsigned int *bar, unsigned int nblocks)
+    // Returns false (in every thread) when the wait hit the spin bound.
+    __device__ __forceinline__ bool grid_barrier(unsigned int *bar, unsigned int nblocks, const SpinBound &bound)
     {
         __syncthreads();
+        int failed = 0;
         if (threadIdx.x == 0)
         {
             const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -301,12 +328,20 @@ namespace jsvd::dev
             }
             else
             {
+                const uint64_t t0 = static_cast<uint64_t>(wall_clock64());
                 while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+                {
+                    if (spin_expired(bound, t0))
+                    {
+                        failed = 1;
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(1);
+                }
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             }
         }
-        __syncthreads();
+        return !__syncthreads_or(failed);
     }
     // Barrier among `members` (at most 256) co-resident workgroups through one
     // flag word each: workgroup `member` publishes its barrier count `epoch`
@@ -316,9 +351,12 @@ namespace jsvd::dev
     // coherence per barrier (the store, the loads that see the last store)
     // instead of the arrival RMW, the generation store and the poll of the
     // counter barrier.  Flags start at zero and only grow within a launch.
-    __device__ __forceinline__ void flag_barrier(unsigned int *flags, int members, int member, unsigned int epoch)
+    // Returns false (in every thread) when the wait hit the spin bound.
+    __device__ __forceinline__ bool flag_barrier(unsigned int *flags, int members, int member, unsigned int epoch,
+                                                 const SpinBound &bound)
     {
         __syncthreads();
+        int failed = 0;
         if (threadIdx.x < 64)
         {
             if (threadIdx.x == 0)
@@ -327,6 +365,7 @@ namespace jsvd::dev
                 __hip_atomic_store(&flags[member], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
             const int l = threadIdx.x;
+            const uint64_t t0 = static_cast<uint64_t>(wall_clock64());
             for (;;)
             {
                 bool ready = true;
@@ -334,10 +373,17 @@ namespace jsvd::dev
                     ready &= __hip_atomic_load(&flags[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= epoch;
                 if (__all(ready))
                     break;
+                // Lane 0 decides for the wave, so the loop is left together.
+                const bool expired = l == 0 && spin_expired(bound, t0);
+                if (__any(expired))
+                {
+                    failed = 1;
+                    break;
+                }
                 __builtin_amdgcn_s_sleep(1);
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
-        __syncthreads();
+        return !__syncthreads_or(failed);
     }
 } // namespace jsvd::dev

@@ -173,9 +173,12 @@ namespace jsvd
                                   hipStream_t stream);
 
     // Persistent plain-FP64 sweep kernel over one column-major matrix.
+    // fault: one word, zero at launch; spin_limit: wall-clock ticks a grid
+    // barrier may wait before the launch is marked faulted (stop reason 2).
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
-                                            int32_t *sweeps_out, int32_t *stop_out, hipStream_t stream);
+                                            int32_t *sweeps_out, int32_t *stop_out, unsigned int *fault,
+                                            uint64_t spin_limit, hipStream_t stream);
     const void *persistent_sweeps_f64_symbol(int block);
     // Matrices too large for LDS with m up to wave_sweeps_max_rows(): one
     // wave per column pair (workgroups of 8 or 16 waves), solved by groups of
@@ -200,7 +203,9 @@ namespace jsvd
         int max_sweeps;
         unsigned int *bars;     // one flag word per workgroup (members per group), zero at launch
         int32_t *sweeps;        // optional
-        int32_t *stop;          // optional: 0 converged, 1 stopped at max_sweeps
+        int32_t *stop;          // optional: 0 converged, 1 stopped at max_sweeps, 2 barrier timed out
+        unsigned int *fault;    // one word, zero at launch: set when a barrier wait hit spin_limit
+        uint64_t spin_limit;    // wall-clock ticks a barrier may wait
     };
     int wave_sweeps_max_rows();
     // Kernel instance for matrices of up to m rows (all matrices of one

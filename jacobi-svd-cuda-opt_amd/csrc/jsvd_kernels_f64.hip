@@ -253,14 +253,16 @@ namespace jsvd
         // Persistent sweeps over one column-major matrix too large for LDS.
         // flags[s] is set when sweep s rotated any pair; it is read by every
         // workgroup after the sweep's last grid barrier, so all workgroups
-        // leave the sweep loop together.
+        // leave the sweep loop together.  A barrier that hits its spin bound
+        // ends every workgroup's loops; the stop reason is then 2.
         template <int BS, int RMAX>
         __global__ __launch_bounds__(BS) void persistent_sweeps_f64(double *__restrict__ A, double *__restrict__ V,
                                                                     int64_t m, int n, double eps, int max_sweeps,
                                                                     int32_t *__restrict__ flags,
                                                                     unsigned int *__restrict__ bar,
                                                                     int32_t *__restrict__ sweeps_out,
-                                                                    int32_t *__restrict__ stop_out)
+                                                                    int32_t *__restrict__ stop_out,
+                                                                    dev::SpinBound bound)
         {
             constexpr int WAVES = BS / 64;
             __shared__ double red[3][WAVES];
@@ -269,8 +271,8 @@ namespace jsvd
             const int half = even >> 1;
             const int rounds = n < 2 ? 0 : even - 1;
             const bool in_regs = m <= (int64_t)BS * RMAX;
-            int executed = 0, clean = 0;
-            for (int sweep = 0; sweep < max_sweeps; ++sweep)
+            int executed = 0, clean = 0, failed = 0;
+            for (int sweep = 0; sweep < max_sweeps && !failed; ++sweep)
             {
                 int any = 0;
                 for (int r = 0; r < rounds; ++r)
@@ -372,8 +374,14 @@ namespace jsvd
                     }
                     if (r == rounds - 1 && any && tid == 0)
                         __hip_atomic_fetch_or(&flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    dev::grid_barrier(bar, gridDim.x);
+                    if (!dev::grid_barrier(bar, gridDim.x, bound))
+                    {
+                        failed = 1;
+                        break;
+                    }
                 }
+                if (failed)
+                    break;
                 executed = sweep + 1;
                 const int f = __hip_atomic_load(&flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (f == 0)
@@ -385,7 +393,11 @@ namespace jsvd
             if (blockIdx.x == 0 && tid == 0 && sweeps_out)
                 *sweeps_out = executed;
             if (blockIdx.x == 0 && tid == 0 && stop_out)
-                *stop_out = clean ? 0 : 1;
+            {
+                const bool faulted =
+                    failed || __hip_atomic_load(bound.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                *stop_out = faulted ? 2 : (clean ? 0 : 1);
+            }
         }
 
         // One column pair of a column-major m x n matrix (m <= 64 RW, n <= m)
@@ -552,6 +564,8 @@ namespace jsvd
             unsigned int *bar = g.bars + group * g.members;
             unsigned int epoch = 0;
             const int stride = g.members * WAVES;
+            const dev::SpinBound bound{g.fault, g.spin_limit};
+            int failed = 0;
             for (int idx = group; idx < g.count; idx += groups)
             {
                 const WaveMatrix w = g.mats[idx];
@@ -562,7 +576,7 @@ namespace jsvd
                 const int even = n + (n & 1), half = even >> 1;
                 const int rounds = n < 2 ? 0 : even - 1;
                 int executed = 0, clean = 0;
-                for (int sweep = 0; sweep < g.max_sweeps; ++sweep)
+                for (int sweep = 0; sweep < g.max_sweeps && !failed; ++sweep)
                 {
                     int any = 0;
                     for (int r = 0; r < rounds; ++r)
@@ -580,8 +594,14 @@ namespace jsvd
                         }
                         if (r == rounds - 1 && any && lane == 0)
                             __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        dev::flag_barrier(bar, g.members, member, ++epoch);
+                        if (!dev::flag_barrier(bar, g.members, member, ++epoch, bound))
+                        {
+                            failed = 1;
+                            break;
+                        }
                     }
+                    if (failed)
+                        break;
                     executed = sweep + 1;
                     const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (f == 0)
@@ -593,7 +613,25 @@ namespace jsvd
                 if (member == 0 && threadIdx.x == 0 && g.sweeps)
                     g.sweeps[w.index] = executed;
                 if (member == 0 && threadIdx.x == 0 && g.stop)
-                    g.stop[w.index] = clean ? 0 : 1;
+                {
+                    const bool faulted =
+                        failed || __hip_atomic_load(g.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                    g.stop[w.index] = faulted ? 2 : (clean ? 0 : 1);
+                }
+                if (failed)
+                {
+                    // The group stops here: its remaining matrices are
+                    // reported as not solved.
+                    for (int rest = idx + groups; rest < g.count; rest += groups)
+                        if (member == 0 && threadIdx.x == 0)
+                        {
+                            if (g.sweeps)
+                                g.sweeps[g.mats[rest].index] = 0;
+                            if (g.stop)
+                                g.stop[g.mats[rest].index] = 2;
+                        }
+                    break;
+                }
             }
         }
 
@@ -928,22 +966,23 @@ namespace jsvd
 
     hipError_t launch_persistent_sweeps_f64(int block, int grid, double *a_cm, double *v_cm, int64_t m, int n,
                                             double eps, int max_sweeps, int32_t *flags, unsigned int *barrier,
-                                            int32_t *sweeps_out, int32_t *stop_out, hipStream_t stream)
+                                            int32_t *sweeps_out, int32_t *stop_out, unsigned int *fault,
+                                            uint64_t spin_limit, hipStream_t stream)
     {
+        const dev::SpinBound bound{fault, spin_limit};
         switch (block)
         {
         case 256:
-            persistent_sweeps_f64<256, kPersistentRmax><<<grid, 256, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps,
-                                                                                flags, barrier, sweeps_out, stop_out);
+            persistent_sweeps_f64<256, kPersistentRmax><<<grid, 256, 0, stream>>>(
+                a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier, sweeps_out, stop_out, bound);
             break;
         case 512:
-            persistent_sweeps_f64<512, kPersistentRmax><<<grid, 512, 0, stream>>>(a_cm, v_cm, m, n, eps, max_sweeps,
-                                                                                flags, barrier, sweeps_out, stop_out);
+            persistent_sweeps_f64<512, kPersistentRmax><<<grid, 512, 0, stream>>>(
+                a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier, sweeps_out, stop_out, bound);
             break;
         case 1024:
-            persistent_sweeps_f64<1024, kPersistentRmax><<<grid, 1024, 0, stream>>>(a_cm, v_cm, m, n, eps,
-                                                                                  max_sweeps, flags, barrier,
-                                                                                  sweeps_out, stop_out);
+            persistent_sweeps_f64<1024, kPersistentRmax><<<grid, 1024, 0, stream>>>(
+                a_cm, v_cm, m, n, eps, max_sweeps, flags, barrier, sweeps_out, stop_out, bound);
             break;
         default: return hipErrorInvalidValue;
         }

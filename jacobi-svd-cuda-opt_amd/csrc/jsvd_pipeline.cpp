@@ -339,6 +339,12 @@ namespace jacobi::svd::pipeline
             {
                 hip_check(hipEventSynchronize(done), "hipEventSynchronize");
                 const std::size_t count = m.size();
+                for (std::size_t k = 0; k < count; ++k)
+                    if (h_sw.ptr[count + k] == JSVD_STOP_BARRIER_TIMEOUT)
+                        throw CudaError(("Testcase[" + std::to_string(report.matrix_sweeps.size() + k) +
+                                         "]: a device barrier timed out (the workgroups of a barrier kernel were "
+                                         "not all resident); the batch's results are invalid.")
+                                            .c_str());
                 std::size_t a = 0, v = 0, s = 0;
                 io::Matrix ref;
                 for (std::size_t k = 0; k < count; ++k)

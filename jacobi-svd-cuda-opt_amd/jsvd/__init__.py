@@ -14,11 +14,11 @@ import numpy as np
 
 from . import _lib
 from ._lib import (JsvdConfig, JsvdError, JsvdInvalidArgument, LAYOUT_AUTO, LAYOUT_OFF, LAYOUT_ON,
-                   PRECISION_FP64, PRECISION_MIXED, STOP_CONVERGED, STOP_MAX_SWEEPS, default_config, load)
+                   PRECISION_FP64, PRECISION_MIXED, STOP_CONVERGED, STOP_MAX_SWEEPS, STOP_BARRIER_TIMEOUT, default_config, load)
 
 __all__ = [
     "JsvdConfig", "JsvdError", "JsvdInvalidArgument", "PRECISION_FP64", "PRECISION_MIXED", "LAYOUT_AUTO",
-    "STOP_CONVERGED", "STOP_MAX_SWEEPS",
+    "STOP_CONVERGED", "STOP_MAX_SWEEPS", "STOP_BARRIER_TIMEOUT",
     "LAYOUT_ON", "LAYOUT_OFF", "default_config", "load", "svd", "svd_batched", "BatchPlan", "pack_offsets", "blocked_layout",
     "autotune_batch", "tune_class", "solve_device_noalloc",
 ]

@@ -16,7 +16,7 @@ HEADER_PATH = os.path.join(REPO_DIR, "include", "jsvd_c.h")
 
 JSVD_OK, JSVD_EINVAL, JSVD_EDEVICE, JSVD_ENOMEM = 0, 1, 2, 3
 PRECISION_FP64, PRECISION_MIXED = 0, 1
-STOP_CONVERGED, STOP_MAX_SWEEPS = 0, 1
+STOP_CONVERGED, STOP_MAX_SWEEPS, STOP_BARRIER_TIMEOUT = 0, 1, 2
 LAYOUT_AUTO, LAYOUT_ON, LAYOUT_OFF = 0, 1, 2
 
 
