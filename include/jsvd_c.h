@@ -173,8 +173,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
 /* Runs the plan on device buffers (packed layout above) on `stream`
  * (a hipStream_t, or NULL for the null stream).  Asynchronous.  d_sweeps may
  * be NULL.  For JSVD_PRECISION_MIXED the inputs and outputs are still FP64.
- * A plan with blocked matrices (info.blocked_matrices > 0) waits for the
- * device once per sweep of each of them to read its convergence flag. */
+ * The convergence decisions of every path stay on the device (the blocked
+ * path's later sweeps return at once after convergence), so an execution
+ * never waits for the device and can be captured in a HIP graph. */
 jsvd_status jsvd_plan_execute_f64(jsvd_plan *plan, const double *d_a, double *d_u, double *d_sigma, double *d_v,
                                   int32_t *d_sweeps, void *stream);
 

@@ -406,7 +406,8 @@ struct jsvd_plan
     int64_t blocked = 0;
     DevBuf<double> bk_gram, bk_q;
     DevBuf<int32_t> bk_flags;
-    int32_t *h_flag = nullptr;
+    int32_t *h_flag = nullptr;            // two pinned copies of the blocked solve's done word
+    hipEvent_t bk_seen[2] = {nullptr, nullptr};
     int64_t a_elems = 0, v_elems = 0, s_elems = 0;
     // Error metrics (jsvd_plan_metrics_f64): tile list, per-matrix tile
     // ranges and partial sums, built on the first metrics call.
@@ -429,6 +430,9 @@ struct jsvd_plan
             (void)hipEventDestroy(big_out);
         if (h_flag)
             (void)hipHostFree(h_flag);
+        for (hipEvent_t e : bk_seen)
+            if (e)
+                (void)hipEventDestroy(e);
     }
 };
 
@@ -508,9 +512,10 @@ namespace
         const int64_t m = p.m[k];
         const int n = static_cast<int>(p.n[k]);
         const BlockedShape sh = blocked_shape(p.cfg.block_width, m, n);
-        // Zero padding rows and columns, then A (row-major) into the
-        // column-major workspace with leading dimension lda.
-        JSVD_HIP(hipMemsetAsync(p.ws_a.ptr, 0, sizeof(double) * sh.lda * sh.nv, stream));
+        // Zero padding rows and columns (a fill kernel, so the sequence can
+        // be captured), then A (row-major) into the column-major workspace
+        // with leading dimension lda.
+        JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.ws_a.ptr), 0, 2 * sh.lda * sh.nv, stream));
         JSVD_HIP(jsvd::launch_transpose_ld_f64(d_a + p.a_off[k], n, p.ws_a.ptr, sh.lda, m, n, nullptr, 0.0, stream));
         JSVD_HIP(jsvd::launch_identity_f64(p.ws_v.ptr, sh.nv, stream));
         jsvd::BlockedArgs args{};
@@ -526,27 +531,50 @@ namespace
         args.gram = p.bk_gram.ptr;
         args.q = p.bk_q.ptr;
         args.pair_rotated = p.bk_flags.ptr;
-        args.sweep_flag = p.bk_flags.ptr + sh.pairs;
+        args.state = p.bk_flags.ptr + sh.pairs;
         const int rounds = sh.nb - 1;
-        int executed = 0, clean = 0;
+        // The convergence decision stays on the device: every sweep starts
+        // with a one-thread step kernel that closes the previous sweep, and
+        // once a sweep has rotated nothing every later launch returns at
+        // once.  The host never waits.  Outside stream capture it stops
+        // enqueueing once it sees, without blocking, that an earlier
+        // sweep's done word has reached host memory set; under capture the
+        // whole max_sweeps sequence is recorded.
+        hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+        JSVD_HIP(hipStreamIsCapturing(stream, &capture));
+        const bool watch = capture == hipStreamCaptureStatusNone;
+        JSVD_HIP(jsvd::launch_fill_i32(args.state, 0, 4, stream));
+        if (watch)
+            p.h_flag[0] = p.h_flag[1] = 0;
+        int enqueued = 0, probes = 0;
         for (int sweep = 0; sweep < p.cfg.max_sweeps; ++sweep)
         {
-            JSVD_HIP(jsvd::launch_fill_i32(args.sweep_flag, 0, 1, stream));
+            JSVD_HIP(jsvd::launch_blocked_sweep_step(args.state, sweep, stream));
             for (int r = 0; r < rounds; ++r)
                 JSVD_HIP(jsvd::launch_blocked_round(args, r, stream));
-            JSVD_HIP(hipMemcpyAsync(p.h_flag, args.sweep_flag, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-            JSVD_HIP(hipStreamSynchronize(stream));
-            executed = sweep + 1;
-            if (*p.h_flag == 0)
+            enqueued = sweep + 1;
+            if (watch)
             {
-                clean = 1;
-                break;
+                const int slot = probes & 1;
+                if (probes > 0)
+                {
+                    const int prev = slot ^ 1;
+                    const hipError_t q = hipEventQuery(p.bk_seen[prev]);
+                    if (q == hipSuccess && p.h_flag[prev] != 0)
+                        break;
+                    if (q != hipSuccess && q != hipErrorNotReady)
+                        JSVD_HIP(q);
+                    if (q == hipErrorNotReady)
+                        continue; // the previous probe is still in flight: keep its slot
+                }
+                JSVD_HIP(hipMemcpyAsync(p.h_flag + slot, args.state + 1, sizeof(int32_t), hipMemcpyDeviceToHost,
+                                        stream));
+                JSVD_HIP(hipEventRecord(p.bk_seen[slot], stream));
+                ++probes;
             }
         }
-        if (d_sweeps)
-            JSVD_HIP(jsvd::launch_fill_i32(d_sweeps + k, executed, 1, stream));
-        if (d_stop)
-            JSVD_HIP(jsvd::launch_fill_i32(d_stop + k, clean ? 0 : 1, 1, stream));
+        JSVD_HIP(jsvd::launch_blocked_sweep_close(args.state, enqueued, d_sweeps ? d_sweeps + k : nullptr,
+                                                  d_stop ? d_stop + k : nullptr, stream));
         return finish_large(p, k, m, n, sh.lda, sh.nv, d_u, d_sigma, d_v, stream);
     }
 
@@ -1270,8 +1298,10 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         {
             JSVD_CHECK(plan->bk_gram.alloc(big_gram));
             JSVD_CHECK(plan->bk_q.alloc(big_q));
-            JSVD_CHECK(plan->bk_flags.alloc(big_pairs + 1));
-            JSVD_HIP(hipHostMalloc(reinterpret_cast<void **>(&plan->h_flag), sizeof(int32_t), hipHostMallocDefault));
+            JSVD_CHECK(plan->bk_flags.alloc(big_pairs + 4));
+            JSVD_HIP(hipHostMalloc(reinterpret_cast<void **>(&plan->h_flag), 2 * sizeof(int32_t), hipHostMallocDefault));
+            for (hipEvent_t &e : plan->bk_seen)
+                JSVD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         }
     }
     JSVD_HIP(hipDeviceSynchronize());

@@ -242,9 +242,17 @@ namespace jsvd
         double *gram;          // pairs * chunks * (2 block)^2
         double *q;             // pairs * (2 block)^2
         int32_t *pair_rotated; // pairs
-        int32_t *sweep_flag;   // one int, raised when any rotation was applied
+        // Sweep state, four ints: [0] raised when the running sweep rotated
+        // a pair, [1] done (every later launch returns at once), [2]
+        // converged, [3] sweeps executed.  Zero before the first sweep.
+        int32_t *state;
     };
     hipError_t launch_blocked_round(const BlockedArgs &args, int round, hipStream_t stream);
+    // Before sweep `sweep` of a blocked solve (closes the previous one).
+    hipError_t launch_blocked_sweep_step(int32_t *state, int sweep, hipStream_t stream);
+    // After the last enqueued sweep: sweeps and stop reason of the matrix.
+    hipError_t launch_blocked_sweep_close(int32_t *state, int enqueued, int32_t *sweeps_out, int32_t *stop_out,
+                                          hipStream_t stream);
 
     // out (cols x rows, row-major) = transpose of in (rows x cols, row-major);
     // when row_scale is given, input row r is divided by row_scale[r] (and

@@ -58,8 +58,10 @@ namespace jsvd
         template <int B>
         __global__ __launch_bounds__(256) void gram_partial(const double *__restrict__ A, int64_t lda, int nb,
                                                            int round, int64_t chunk_rows, int chunks,
-                                                           double *__restrict__ gram)
+                                                           double *__restrict__ gram, const int32_t *__restrict__ done)
         {
+            if (*done)
+                return; // the solve has converged: the remaining launches are no-ops
             constexpr int W = 2 * B;
             constexpr int T16 = W / 16;
             constexpr int UT = T16 * (T16 + 1) / 2; // upper-triangle tiles (a <= b), row by row
@@ -169,8 +171,10 @@ namespace jsvd
         __global__ __launch_bounds__(NT) void gram_solve(const double *__restrict__ gram, int chunks, int nb, int round,
                                                          double eps, double *__restrict__ qout,
                                                          int32_t *__restrict__ pair_rotated,
-                                                         int32_t *__restrict__ sweep_flag)
+                                                         int32_t *__restrict__ sweep_flag, const int32_t *__restrict__ done)
         {
+            if (*done)
+                return;
             constexpr int W = 2 * B;
             constexpr int LDG = W + 1;
             constexpr int NBLK = B * (B + 1) / 2; // 2 x 2 blocks (i <= j) of the inner pairs
@@ -343,8 +347,11 @@ namespace jsvd
         __global__ __launch_bounds__(256) void block_update(double *__restrict__ A, double *__restrict__ V, int64_t lda,
                                                            int nv, int nb, int round, int ua, int uv, int64_t units,
                                                            const double *__restrict__ qin,
-                                                           const int32_t *__restrict__ pair_rotated)
+                                                           const int32_t *__restrict__ pair_rotated,
+                                                           const int32_t *__restrict__ done)
         {
+            if (*done)
+                return;
             constexpr int W = 2 * B;
             constexpr int T16 = W / 16;
             constexpr int KS = W / 4; // MFMA k-steps
@@ -452,21 +459,72 @@ namespace jsvd
         hipError_t launch_round(const BlockedArgs &a, int round, hipStream_t stream)
         {
             const int pairs = (a.nb + (a.nb & 1)) / 2;
+            const int32_t *done = a.state + 1;
             gram_partial<B><<<pairs * a.chunks, 256, 0, stream>>>(a.a, a.lda, a.nb, round, a.chunk_rows, a.chunks,
-                                                                a.gram);
+                                                                a.gram, done);
             constexpr int NT = B == 32 ? 1024 : 512;
             gram_solve<B, NT><<<pairs, NT, 0, stream>>>(a.gram, a.chunks, a.nb, round, a.eps, a.q, a.pair_rotated,
-                                                    a.sweep_flag);
+                                                    a.state, done);
             const int ua = static_cast<int>((a.lda + kUnitRows - 1) / kUnitRows);
             const int uv = (a.nv + kUnitRows - 1) / kUnitRows;
             const int64_t units = int64_t(pairs) * (ua + uv);
             const int slots = resident_workgroups(reinterpret_cast<const void *>(&block_update<B>), B == 32 ? 0 : 1);
             const int grid = static_cast<int>(units < slots ? units : slots);
             block_update<B><<<grid, 256, 0, stream>>>(a.a, a.v, a.lda, a.nv, a.nb, round, ua, uv, units, a.q,
-                                                     a.pair_rotated);
+                                                     a.pair_rotated, done);
             return hipGetLastError();
         }
+
+        // Sweep bookkeeping on the device (one thread).  At the start of
+        // sweep s > 0 it closes sweep s - 1: that sweep counts as executed,
+        // and a sweep that rotated no pair ends the solve (done, converged),
+        // after which every launch of the later sweeps returns at once.
+        __global__ void blocked_sweep_step(int32_t *state, int sweep)
+        {
+            if (state[1])
+                return;
+            if (sweep > 0)
+            {
+                state[3] = sweep;
+                if (state[0] == 0)
+                {
+                    state[1] = 1;
+                    state[2] = 1;
+                    return;
+                }
+            }
+            state[0] = 0;
+        }
+
+        // After the last enqueued sweep: closes it if the solve is still
+        // running, then reports sweeps and stop reason.
+        __global__ void blocked_sweep_close(int32_t *state, int enqueued, int32_t *sweeps_out, int32_t *stop_out)
+        {
+            if (!state[1])
+            {
+                state[3] = enqueued;
+                state[2] = state[0] == 0 ? 1 : 0;
+                state[1] = 1;
+            }
+            if (sweeps_out)
+                *sweeps_out = state[3];
+            if (stop_out)
+                *stop_out = state[2] ? 0 : 1;
+        }
     } // namespace
+
+    hipError_t launch_blocked_sweep_step(int32_t *state, int sweep, hipStream_t stream)
+    {
+        blocked_sweep_step<<<1, 1, 0, stream>>>(state, sweep);
+        return hipGetLastError();
+    }
+
+    hipError_t launch_blocked_sweep_close(int32_t *state, int enqueued, int32_t *sweeps_out, int32_t *stop_out,
+                                          hipStream_t stream)
+    {
+        blocked_sweep_close<<<1, 1, 0, stream>>>(state, enqueued, sweeps_out, stop_out);
+        return hipGetLastError();
+    }
 
     hipError_t launch_blocked_round(const BlockedArgs &args, int round, hipStream_t stream)
     {

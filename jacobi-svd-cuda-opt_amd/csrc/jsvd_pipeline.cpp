@@ -273,10 +273,11 @@ namespace jacobi::svd::pipeline
                     a_elems = static_cast<std::size_t>(info.a_elements);
                     v_elems = static_cast<std::size_t>(info.v_elements);
                     s_elems = static_cast<std::size_t>(info.sigma_elements);
-                    // The large-matrix paths stay out of graphs (the blocked one
-                    // reads its sweep flag on the host), and so does a metrics
-                    // run (its first call on a plan uploads the tile list).
-                    capturable = info.large_matrices == 0 && !metrics;
+                    // A metrics run stays out of graphs (its first call on a
+                    // plan uploads the tile list synchronously); every solve
+                    // path, the blocked one included, keeps its decisions on
+                    // the device and can be captured.
+                    capturable = !metrics;
                 }
                 if (a_elems != staged)
                     throw std::logic_error("Pipeline staging does not match the plan's packed size.");
