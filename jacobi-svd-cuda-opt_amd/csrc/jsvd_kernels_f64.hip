@@ -469,15 +469,40 @@ namespace jsvd
         // of 256 (4 per lane): the dots accumulate rows lane + 64 j in
         // increasing j, the same FMA chain as wave_pair_f64 (equal dots, so
         // equal rotation decisions); the update re-reads the columns and
-        // rounds like wave_pair_f64's (dev::rot_first / rot_second).
+        // rounds like wave_pair_f64's (dev::rot_first / rot_second).  The
+        // first HELD chunks stay in registers from the dots to the update,
+        // so only the rows beyond them are read twice (a lone wide matrix
+        // runs one wave per SIMD, whose registers are otherwise idle); the
+        // arithmetic and its order do not depend on HELD.
+        template <int HELD>
         __device__ __forceinline__ int wave_pair_stream_f64(double *__restrict__ A, double *__restrict__ V,
                                                             int64_t m, int n, int p, int q, int lane, double eps)
         {
             constexpr int CH = 4;
+            constexpr int64_t HELD_ROWS = int64_t(HELD) * 64 * CH;
             double *ap = A + (int64_t)p * m;
             double *aq = A + (int64_t)q * m;
             double pp = 0.0, qq = 0.0, pq = 0.0;
-            for (int64_t r0 = 0; r0 < m; r0 += 64 * CH)
+            double hp[HELD > 0 ? HELD : 1][CH], hq[HELD > 0 ? HELD : 1][CH];
+#pragma unroll
+            for (int h = 0; h < HELD; ++h)
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    const int64_t i = int64_t(h) * 64 * CH + lane + 64 * j;
+                    hp[h][j] = i < m ? ap[i] : 0.0;
+                    hq[h][j] = i < m ? aq[i] : 0.0;
+                }
+#pragma unroll
+            for (int h = 0; h < HELD; ++h)
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    pp = fma(hp[h][j], hp[h][j], pp);
+                    qq = fma(hq[h][j], hq[h][j], qq);
+                    pq = fma(hp[h][j], hq[h][j], pq);
+                }
+            for (int64_t r0 = HELD_ROWS; r0 < m; r0 += 64 * CH)
             {
                 double xp[CH], xq[CH];
 #pragma unroll
@@ -500,7 +525,23 @@ namespace jsvd
                 return 0;
             double c, s;
             dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
-            for (int64_t r0 = 0; r0 < m; r0 += 64 * CH)
+#pragma unroll
+            for (int h = 0; h < HELD; ++h)
+#pragma unroll
+                for (int j = 0; j < CH; ++j)
+                {
+                    const int64_t i = int64_t(h) * 64 * CH + lane + 64 * j;
+                    if (i < m)
+                    {
+                        ap[i] = dev::rot_first(c, s, hp[h][j], hq[h][j]);
+                        aq[i] = dev::rot_second(c, s, hp[h][j], hq[h][j]);
+                    }
+                }
+            // The streamed rows in reverse: the chunks the dots read last
+            // are the likeliest to be still in L2.
+            const int64_t last =
+                m > HELD_ROWS ? HELD_ROWS + ((m - HELD_ROWS - 1) / (64 * CH)) * (64 * CH) : HELD_ROWS - 64 * CH;
+            for (int64_t r0 = last; r0 >= HELD_ROWS; r0 -= 64 * CH)
             {
                 double xp[CH], xq[CH];
 #pragma unroll
@@ -531,6 +572,11 @@ namespace jsvd
             }
             return 1;
         }
+
+        // Chunks of 256 rows a streamed pair keeps in registers: twelve (192
+        // VGPRs) with one wave per SIMD (four-wave groups), none otherwise.
+        template <int WAVES>
+        constexpr int kStreamHeld = WAVES == 4 ? 20 : 0;
 
         // Matrices too large for LDS with at most 64 * RW rows (RW = 0: any
         // height, rows streamed by wave_pair_stream_f64), solved by
@@ -588,7 +634,7 @@ namespace jsvd
                             if (p < 0 || q < 0)
                                 continue;
                             if constexpr (RW == 0)
-                                any |= wave_pair_stream_f64(A, V, m, n, p, q, lane, g.eps);
+                                any |= wave_pair_stream_f64<kStreamHeld<WAVES>>(A, V, m, n, p, q, lane, g.eps);
                             else
                                 any |= wave_pair_f64<RW>(A, V, m, n, p, q, lane, g.eps);
                         }
@@ -710,7 +756,7 @@ namespace jsvd
                             const int q = dev::column_at(even - 1 - k, r, n, even);
                             if (p < 0 || q < 0)
                                 continue;
-                            any |= wave_pair_stream_f64(w.a, w.v, m, n, p, q, lane, g.eps);
+                            any |= wave_pair_stream_f64<0>(w.a, w.v, m, n, p, q, lane, g.eps);
                         }
                         __syncthreads();
                     }

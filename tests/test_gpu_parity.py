@@ -174,6 +174,28 @@ def test_tall_batch_streamed_against_oracle():
         assert single[3] == got[k][3]
 
 
+def test_tall_single_streamed_held_rows():
+    # A lone plain-path matrix of more than 1024 rows and at most 512 columns
+    # runs four-wave groups with its rows streamed; each pair keeps its first
+    # 20 chunks of 256 rows (5120 rows) in registers from the dots to the
+    # update and streams the rest twice, the update pass in reverse.  6100 x 40
+    # has a ragged streamed tail (980 rows beyond the held ones), 3000 x 60
+    # is held whole.  Against the oracle, and bitwise against the same
+    # matrices inside a batch of 100 that runs one workgroup per matrix
+    # without held rows (the arithmetic must not depend on them).
+    _note("tall singles, held + streamed rows")
+    rng = np.random.default_rng(907)
+    for shape in ((6100, 40), (3000, 60)):
+        a = rng.standard_normal(shape)
+        single = jsvd.svd(a, epsilon=1e-13, max_sweeps=30)
+        _assert_parity(a, single, oracle.solve(a, 1e-13, 30), 1e-13, f"held {shape}")
+        batch = [a] + [rng.standard_normal(shape) for _ in range(99)]
+        got = jsvd.svd_batched(batch, epsilon=1e-13, max_sweeps=30)
+        for x, y in zip(single[:3], got[0][:3]):
+            np.testing.assert_array_equal(x, y, err_msg=f"single vs batch {shape}")
+        assert single[3] == got[0][3]
+
+
 def test_grouped_wave_launch_with_narrow_groups():
     # 40 matrices of 512 x 512 stay in one grouped launch (below the
     # 4 + 256^2 / 200 matrices of the one-workgroup-per-matrix form), and 40
