@@ -392,6 +392,8 @@ struct jsvd_plan
         int members = 1; // workgroups per group; 0 = one workgroup per matrix
         int groups = 1;
         int waves = 8;   // waves per workgroup of the grouped form
+        bool ring = false;      // ring form (wave_ring_sweeps_f64)
+        int64_t mbox_group = 0; // ring form: mailbox words per group
     };
     std::vector<int64_t> wave;                 // batch index per entry
     std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry
@@ -400,6 +402,7 @@ struct jsvd_plan
     DevBuf<double> wv_a, wv_v;
     DevBuf<int32_t> wv_flags;
     DevBuf<unsigned int> wv_bars;
+    DevBuf<unsigned long long> wv_mbox; // ring-form column mailboxes
     // Blocked Jacobi (cfg.block_width 16 or 32): per-round partial Grams,
     // rotations Q and flags, sized for the largest blocked matrix; the sweep
     // flag is read back through pinned host memory once per sweep.
@@ -623,6 +626,14 @@ namespace
     // of 256 x 256, and tie at 96 of 512 x 512.  Wider matrices keep the
     // barrier kernels whatever their count (one workgroup would take dozens
     // of pairs per wave and round).
+    // The ring form of the grouped wave solves is the default; JSVD_WAVE_RING=0
+    // keeps the barrier form (for comparisons).
+    bool ring_enabled()
+    {
+        const char *env = std::getenv("JSVD_WAVE_RING");
+        return !(env && env[0] == '0');
+    }
+
     bool wave_block_mode(int64_t count, int64_t max_half)
     {
         return max_half <= 256 && count > 4 + max_half * max_half / 200;
@@ -671,6 +682,19 @@ namespace
             args.spin_limit = barrier_spin_limit(p.device, p.wave_bytes);
             if (l.members == 0)
                 JSVD_HIP(jsvd::launch_wave_block_sweeps_f64(l.m_max, l.count, args, stream));
+            else if (l.ring)
+            {
+                // Flag words, the fault word and every mailbox word start the
+                // launch at zero (a stale round tag must never match).
+                JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.wv_bars.ptr), 0,
+                                               int64_t(l.groups) * l.members, stream));
+                JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.d_fault.ptr), 0, 1, stream));
+                JSVD_HIP(jsvd::launch_fill_i32(reinterpret_cast<int32_t *>(p.wv_mbox.ptr), 0,
+                                               2 * int64_t(l.groups) * l.mbox_group, stream));
+                args.mbox = p.wv_mbox.ptr;
+                args.mbox_group = l.mbox_group;
+                JSVD_HIP(jsvd::launch_wave_ring_sweeps_f64(l.m_max, l.groups, args, stream));
+            }
             else
             {
                 // The group barriers' flag words and the fault word start
@@ -733,7 +757,7 @@ namespace
             p.wave_bytes += 8.0 * double(p.m[k]) * double(p.n[k]);
             v_total += p.n[k] * p.n[k];
         }
-        int64_t max_flags = 1;
+        int64_t max_flags = 1, max_mbox = 0;
         for (int64_t e = 0; e < entries;)
         {
             jsvd_plan::WaveLaunch l;
@@ -748,6 +772,31 @@ namespace
             }
             l.count = static_cast<int>(e - l.first);
             const bool streamed = l.m_max > jsvd::wave_sweeps_max_rows();
+            // Ring form: a full group (one wave per pair, every column pair
+            // held in registers) for at least one matrix at a time; groups
+            // take the launch's matrices in turn when fewer groups than
+            // matrices are resident.
+            if (ring_enabled() && l.m_max <= jsvd::ring_max_rows() && max_half >= 2)
+            {
+                int occupancy = 0;
+                JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                    &occupancy, jsvd::wave_ring_sweeps_f64_symbol(l.m_max), 64 * jsvd::ring_waves(),
+                    jsvd::ring_lds_bytes(l.m_max)));
+                const int64_t resident = int64_t(occupancy) * cus;
+                const int64_t members = (max_half + jsvd::ring_waves() - 1) / jsvd::ring_waves();
+                if (occupancy > 0 && members <= 256 && members <= resident)
+                {
+                    l.ring = true;
+                    l.waves = jsvd::ring_waves();
+                    l.members = static_cast<int>(members);
+                    l.groups = static_cast<int>(std::min<int64_t>(l.count, resident / members));
+                    l.mbox_group = jsvd::ring_mailbox_words(l.m_max, l.members);
+                    max_flags = std::max<int64_t>(max_flags, int64_t(l.groups) * l.members);
+                    max_mbox = std::max<int64_t>(max_mbox, int64_t(l.groups) * l.mbox_group);
+                    p.wave_launches.push_back(l);
+                    continue;
+                }
+            }
             if (streamed ? tall_block_mode(l.count, max_half) : wave_block_mode(l.count, max_half))
             {
                 l.members = 0;
@@ -785,6 +834,7 @@ namespace
         JSVD_CHECK(p.wv_v.alloc(v_total));
         JSVD_CHECK(p.wv_flags.alloc(entries * p.cfg.max_sweeps));
         JSVD_CHECK(p.wv_bars.alloc(max_flags));
+        JSVD_CHECK(p.wv_mbox.alloc(max_mbox));
         std::vector<jsvd::WaveMatrix> mats(entries);
         for (int64_t e = 0; e < entries; ++e)
         {
@@ -823,6 +873,7 @@ namespace
         std::vector<int64_t> m, n;
         jsvd_config cfg{};
         std::vector<int> tune; // the shape-class table the plan was built with
+        bool ring = true;      // whether the ring form of the wave solves was enabled
         jsvd_plan *plan = nullptr;
         uint64_t last_use = 0;
     };
@@ -1669,9 +1720,10 @@ jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_
     if (cfg)
         c = *cfg;
     const std::vector<int> tune = tune_table();
+    const bool ring = ring_enabled();
     jsvd_plan *plan = nullptr;
     for (CachedPlan &e : ctx->plans)
-        if (e.m == mm && e.n == nn && same_config(e.cfg, c) && e.tune == tune)
+        if (e.m == mm && e.n == nn && same_config(e.cfg, c) && e.tune == tune && e.ring == ring)
         {
             plan = e.plan;
             e.last_use = ++ctx->clock;
@@ -1689,7 +1741,7 @@ jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_
             jsvd_plan_destroy(lru->plan);
             ctx->plans.erase(lru);
         }
-        ctx->plans.push_back({mm, nn, c, tune, plan, ++ctx->clock});
+        ctx->plans.push_back({mm, nn, c, tune, ring, plan, ++ctx->clock});
     }
 
     JSVD_CHECK(ctx->a.reserve(plan->a_elems));

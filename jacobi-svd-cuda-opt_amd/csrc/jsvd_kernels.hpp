@@ -206,8 +206,25 @@ namespace jsvd
         int32_t *stop;          // optional: 0 converged, 1 stopped at max_sweeps, 2 barrier timed out
         unsigned int *fault;    // one word, zero at launch: set when a barrier wait hit spin_limit
         uint64_t spin_limit;    // wall-clock ticks a barrier may wait
+        // Ring form only: column mailboxes, ring_mailbox_words(m, members)
+        // 64-bit words per group, zero at launch.
+        unsigned long long *mbox;
+        int64_t mbox_group;
     };
     int wave_sweeps_max_rows();
+    // Ring form of the grouped solves (one wave per column pair, every
+    // pair's columns held in registers for the whole solve; columns move to
+    // the neighbouring pair positions through LDS inside a workgroup and
+    // through tagged mailbox words between neighbouring workgroups, so a
+    // round has no group barrier).  For matrices of at most
+    // ring_max_rows() rows (m >= n), groups of ceil(half / ring_waves())
+    // workgroups of ring_waves() waves.
+    int ring_waves();
+    int ring_max_rows();
+    int ring_lds_bytes(int64_t m);
+    int64_t ring_mailbox_words(int64_t m, int members);
+    const void *wave_ring_sweeps_f64_symbol(int64_t m);
+    hipError_t launch_wave_ring_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream);
     // Kernel instance for matrices of up to m rows (all matrices of one
     // launch share it: ceil(m / 64) row registers per lane) and workgroups
     // of `waves` (8 or 16) waves.

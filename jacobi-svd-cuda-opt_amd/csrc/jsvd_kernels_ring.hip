@@ -1,0 +1,396 @@
+// jsvd_kernels_ring.hip -- ring form of the grouped plain-order sweeps: one
+// or a few mid-size matrices (at most 512 rows) solved by groups of
+// workgroups that never meet at a group barrier inside a sweep.
+//
+// Every column pair position k (0 <= k < half) belongs to one wave for the
+// whole solve: wave k % W of workgroup k / W holds the pair's two columns of
+// A and of V in registers (lane l: rows l + 64 j).  In the reference's circle
+// (src/domain/jacobi_schedule.cuh:16-57, src/domain/jacobi_rotation_kernels.cuh:18-42)
+// every column except column 0 moves to the next circle slot each round, so
+// after a round's rotations
+//   top(k) -> top(k + 1)                 for 1 <= k <= half - 2,
+//   top(half - 1) -> bottom(half - 1),
+//   bottom(k) -> bottom(k - 1)           for 1 <= k <= half - 1,
+//   bottom(0) -> top(1),
+// and top(0) keeps column 0.  Inside a workgroup the moved columns go
+// through LDS (two workgroup barriers per round); between workgroups only
+// two columns cross each boundary per round: the last wave's top forward and
+// the first wave's bottom back.  They travel through mailboxes of 64-bit
+// words, each carrying half of a double in its low 32 bits and the round
+// number in its high 32 bits, written and read with relaxed agent-scope
+// atomics: a reader takes a word once its tag is the current round, so no
+// fence, flag or barrier orders the data (two mailboxes per direction,
+// alternating by round parity; neighbours are never more than one round
+// apart, because each needs the other's previous round).  A sweep ends with
+// the group's flag barrier and the shared sweep flag, as in the grouped
+// kernel.
+//
+// The pair arithmetic is wave_pair_f64's (same dot chains over the same row
+// order, same reduction, rotation parameters and update roundings) and the
+// pairs come in the same order, so the results equal the grouped kernel's
+// bitwise.  After every whole sweep the columns are back at their starting
+// positions (top(k) = column k, bottom(k) = column E - 1 - k), where they are
+// written back to the workspace.
+#include "jsvd_device.hpp"
+#include "jsvd_kernels.hpp"
+
+namespace jsvd
+{
+    namespace
+    {
+        constexpr int kRingWaves = 8;
+        constexpr int kRingRowsMax = 8; // row registers per lane: m <= 512
+
+        // Words of one mailbox message: the A column then the V column, R
+        // words of 64 lanes x two halves each.
+        template <int R>
+        constexpr int kRingMsg = 4 * 64 * R;
+
+        __device__ __forceinline__ unsigned long long tagged(unsigned int tag, unsigned int bits)
+        {
+            return (static_cast<unsigned long long>(tag) << 32) | bits;
+        }
+
+        template <int R>
+        __device__ __forceinline__ void ring_send(unsigned long long *msg, unsigned int tag, const double (&a)[R],
+                                                  const double (&v)[R], int lane)
+        {
+#pragma unroll
+            for (int part = 0; part < 2; ++part)
+#pragma unroll
+                for (int j = 0; j < R; ++j)
+                {
+                    const unsigned long long bits =
+                        static_cast<unsigned long long>(__double_as_longlong(part == 0 ? a[j] : v[j]));
+                    unsigned long long *w = msg + (part * R + j) * 128 + lane * 2;
+                    __hip_atomic_store(w, tagged(tag, static_cast<unsigned int>(bits)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(w + 1, tagged(tag, static_cast<unsigned int>(bits >> 32)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                }
+        }
+
+        // Waits (bounded) until every word of the message carries `tag`, then
+        // assembles the two columns.  Returns false when the wait expired.
+        template <int R>
+        __device__ __forceinline__ bool ring_receive(const unsigned long long *msg, unsigned int tag, double (&a)[R],
+                                                     double (&v)[R], int lane, const dev::SpinBound &bound)
+        {
+            const uint64_t t0 = static_cast<uint64_t>(wall_clock64());
+            for (;;)
+            {
+                bool ready = true;
+#pragma unroll
+                for (int part = 0; part < 2; ++part)
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                    {
+                        const unsigned long long *w = msg + (part * R + j) * 128 + lane * 2;
+                        const unsigned long long lo =
+                            __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        const unsigned long long hi =
+                            __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ready &= static_cast<unsigned int>(lo >> 32) == tag && static_cast<unsigned int>(hi >> 32) == tag;
+                        const double x = __longlong_as_double(static_cast<long long>((hi << 32) | (lo & 0xffffffffull)));
+                        if (part == 0)
+                            a[j] = x;
+                        else
+                            v[j] = x;
+                    }
+                if (__all(ready))
+                    return true;
+                const bool expired = lane == 0 && dev::spin_expired(bound, t0);
+                if (__any(expired))
+                    return false;
+            }
+        }
+
+        // wave_pair_f64 on register-held columns: p is the top column, q the
+        // bottom one.  Rows beyond m (and V rows beyond n) hold zeros, which
+        // leave the dot chains and the updates unchanged.
+        template <int R>
+        __device__ __forceinline__ int ring_pair(double (&xp)[R], double (&xq)[R], double (&yp)[R], double (&yq)[R],
+                                                 double eps)
+        {
+            double pp = 0.0, qq = 0.0, pq = 0.0;
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                pp = fma(xp[j], xp[j], pp);
+                qq = fma(xq[j], xq[j], qq);
+                pq = fma(xp[j], xq[j], pq);
+            }
+            dev::group_sum3_dpp<64>(pp, qq, pq);
+            if (!dev::rotation_needed(pp, qq, pq, eps))
+                return 0;
+            double c, s;
+            dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                const double a = xp[j], b = xq[j];
+                xp[j] = dev::rot_first(c, s, a, b);
+                xq[j] = dev::rot_second(c, s, a, b);
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                const double a = yp[j], b = yq[j];
+                yp[j] = dev::rot_first(c, s, a, b);
+                yq[j] = dev::rot_second(c, s, a, b);
+            }
+            return 1;
+        }
+
+        template <int R>
+        __device__ __forceinline__ void lds_put(double *slot, const double (&a)[R], const double (&v)[R], int lane)
+        {
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                slot[j * 64 + lane] = a[j];
+                slot[(R + j) * 64 + lane] = v[j];
+            }
+        }
+
+        template <int R>
+        __device__ __forceinline__ void lds_get(const double *slot, double (&a)[R], double (&v)[R], int lane)
+        {
+#pragma unroll
+            for (int j = 0; j < R; ++j)
+            {
+                a[j] = slot[j * 64 + lane];
+                v[j] = slot[(R + j) * 64 + lane];
+            }
+        }
+
+        // LDS: per wave a top slot and a bottom slot of 2 R x 64 doubles (A
+        // column, then V column).  Mailboxes of a group: for each boundary b
+        // (between workgroups b and b + 1), direction d (0: top forward, 1:
+        // bottom back) and round parity, one message.
+        template <int R>
+        __global__ __launch_bounds__(64 * kRingWaves) void wave_ring_sweeps_f64(WaveGroupArgs g)
+        {
+            constexpr int W = kRingWaves, MSG = kRingMsg<R>, SLOT = 2 * R * 64;
+            extern __shared__ double ring_lds[];
+            const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
+            const int groups = gridDim.x / g.members;
+            unsigned int *bar = g.bars + group * g.members;
+            unsigned long long *mb = g.mbox + group * g.mbox_group;
+            const dev::SpinBound bound{g.fault, g.spin_limit};
+            double *my_top = ring_lds + (2 * wave) * SLOT;
+            double *my_bot = my_top + SLOT;
+            const int k = member * W + wave;
+            unsigned int epoch = 0, tag = 0;
+            int failed = 0;
+            for (int idx = group; idx < g.count; idx += groups)
+            {
+                const WaveMatrix w = g.mats[idx];
+                const int64_t m = w.m;
+                const int n = w.n;
+                const int even = n + (n & 1), half = even >> 1;
+                const int rounds = n < 2 ? 0 : even - 1;
+                const bool active = k < half;
+                const int ct = k, cb = even - 1 - k;
+                double tA[R], tV[R], bA[R], bV[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j)
+                {
+                    const int64_t i = lane + 64 * j;
+                    tA[j] = (active && ct < n && i < m) ? w.a[int64_t(ct) * m + i] : 0.0;
+                    bA[j] = (active && cb < n && i < m) ? w.a[int64_t(cb) * m + i] : 0.0;
+                    tV[j] = (active && ct < n && i < n) ? w.v[int64_t(ct) * n + i] : 0.0;
+                    bV[j] = (active && cb < n && i < n) ? w.v[int64_t(cb) * n + i] : 0.0;
+                }
+                // Message slots of this workgroup's boundaries.
+                unsigned long long *fwd_out = mb + int64_t(member * 2 + 0) * 2 * MSG;       // top to member + 1
+                unsigned long long *back_in = mb + int64_t(member * 2 + 1) * 2 * MSG;       // bottom from member + 1
+                const int left = member > 0 ? member - 1 : 0; // used only when member > 0
+                unsigned long long *fwd_in = mb + int64_t(left * 2 + 0) * 2 * MSG;  // top from member - 1
+                unsigned long long *back_out = mb + int64_t(left * 2 + 1) * 2 * MSG; // bottom to member - 1
+                // The last wave sends its top forward (and, as position
+                // k + 1 < half exists, receives that position's bottom);
+                // the first wave of a later workgroup sends its bottom back
+                // (and receives the previous position's top).
+                const bool send_fwd = active && wave == W - 1 && k + 1 < half;
+                const bool send_back = active && wave == 0 && member > 0;
+                int executed = 0, clean = 0;
+                for (int sweep = 0; sweep < g.max_sweeps && !failed; ++sweep)
+                {
+                    int any = 0;
+                    for (int r = 0; r < rounds; ++r)
+                    {
+                        ++tag;
+                        const int par = static_cast<int>(tag & 1u) * MSG;
+                        if (active)
+                            any |= ring_pair<R>(tA, bA, tV, bV, g.eps);
+                        // The boundary columns leave at once, so they travel
+                        // while the workgroup exchanges the others.  The
+                        // receiver has read this parity's previous message:
+                        // it needed it to finish the round before the one
+                        // this workgroup just completed.
+                        if (send_fwd)
+                            ring_send<R>(fwd_out + par, tag, tA, tV, lane);
+                        if (send_back)
+                            ring_send<R>(back_out + par, tag, bA, bV, lane);
+                        // Every read of the previous round's slots is done
+                        // (and every wave agrees on a failed wait).
+                        if (__syncthreads_or(failed))
+                        {
+                            failed = 1;
+                            break;
+                        }
+                        if (active)
+                        {
+                            lds_put<R>(my_top, tA, tV, lane);
+                            lds_put<R>(my_bot, bA, bV, lane);
+                        }
+                        __syncthreads();
+                        if (active)
+                        {
+                            // New bottom first: at k = half - 1 it is this
+                            // wave's own top, which the new top replaces
+                            // (with one pair, half = 1, both columns stay).
+                            if (k == half - 1)
+                            {
+                                if (half > 1)
+                                {
+#pragma unroll
+                                    for (int j = 0; j < R; ++j)
+                                    {
+                                        bA[j] = tA[j];
+                                        bV[j] = tV[j];
+                                    }
+                                }
+                            }
+                            else if (wave < W - 1)
+                                lds_get<R>(my_bot + 2 * SLOT, bA, bV, lane); // wave + 1's bottom
+                            else if (!ring_receive<R>(back_in + par, tag, bA, bV, lane, bound))
+                                failed = 1;
+                            if (k == 1)
+                                lds_get<R>(my_bot - 2 * SLOT, tA, tV, lane); // wave 0's bottom
+                            else if (k >= 2 && wave > 0)
+                                lds_get<R>(my_top - 2 * SLOT, tA, tV, lane); // wave - 1's top
+                            else if (k >= 2 && !ring_receive<R>(fwd_in + par, tag, tA, tV, lane, bound))
+                                failed = 1;
+                        }
+                    }
+                    failed = __syncthreads_or(failed);
+                    if (failed)
+                        break;
+                    executed = sweep + 1;
+                    if (any && lane == 0)
+                        __hip_atomic_fetch_or(&w.flags[sweep], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (!dev::flag_barrier(bar, g.members, member, ++epoch, bound))
+                    {
+                        failed = 1;
+                        break;
+                    }
+                    const int f = __hip_atomic_load(&w.flags[sweep], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (f == 0)
+                    {
+                        clean = 1;
+                        break;
+                    }
+                }
+                if (!failed && active)
+                {
+#pragma unroll
+                    for (int j = 0; j < R; ++j)
+                    {
+                        const int64_t i = lane + 64 * j;
+                        if (i < m)
+                        {
+                            w.a[int64_t(ct) * m + i] = tA[j];
+                            if (cb < n)
+                                w.a[int64_t(cb) * m + i] = bA[j];
+                        }
+                        if (i < n)
+                        {
+                            w.v[int64_t(ct) * n + i] = tV[j];
+                            if (cb < n)
+                                w.v[int64_t(cb) * n + i] = bV[j];
+                        }
+                    }
+                }
+                if (member == 0 && threadIdx.x == 0 && g.sweeps)
+                    g.sweeps[w.index] = failed ? 0 : executed;
+                if (member == 0 && threadIdx.x == 0 && g.stop)
+                {
+                    const bool faulted =
+                        failed || __hip_atomic_load(g.fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+                    g.stop[w.index] = faulted ? 2 : (clean ? 0 : 1);
+                }
+                if (failed)
+                {
+                    // The group stops here: its remaining matrices are
+                    // reported as not solved.
+                    for (int rest = idx + groups; rest < g.count; rest += groups)
+                        if (member == 0 && threadIdx.x == 0)
+                        {
+                            if (g.sweeps)
+                                g.sweeps[g.mats[rest].index] = 0;
+                            if (g.stop)
+                                g.stop[g.mats[rest].index] = 2;
+                        }
+                    break;
+                }
+            }
+        }
+
+        template <int R>
+        const void *ring_symbol_t()
+        {
+            return reinterpret_cast<const void *>(&wave_ring_sweeps_f64<R>);
+        }
+        const void *(*const kRingSymbol[])() = {ring_symbol_t<1>, ring_symbol_t<2>, ring_symbol_t<3>,
+                                                ring_symbol_t<4>, ring_symbol_t<5>, ring_symbol_t<6>,
+                                                ring_symbol_t<7>, ring_symbol_t<8>};
+        static_assert(sizeof(kRingSymbol) / sizeof(kRingSymbol[0]) == kRingRowsMax, "one instance per row count");
+
+        int ring_rows(int64_t m)
+        {
+            return static_cast<int>((m + 63) / 64);
+        }
+    } // namespace
+
+    int ring_waves()
+    {
+        return kRingWaves;
+    }
+
+    int ring_max_rows()
+    {
+        return 64 * kRingRowsMax;
+    }
+
+    int ring_lds_bytes(int64_t m)
+    {
+        // Two slots per wave of 2 R x 64 doubles.
+        return kRingWaves * 2 * 2 * ring_rows(m) * 64 * 8;
+    }
+
+    int64_t ring_mailbox_words(int64_t m, int members)
+    {
+        // Per workgroup (as the left end of a boundary): two directions, two
+        // parities, one message each.
+        return int64_t(members) * 2 * 2 * (4 * 64 * ring_rows(m));
+    }
+
+    const void *wave_ring_sweeps_f64_symbol(int64_t m)
+    {
+        const int r = ring_rows(m);
+        return r >= 1 && r <= kRingRowsMax ? kRingSymbol[r - 1]() : nullptr;
+    }
+
+    hipError_t launch_wave_ring_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream)
+    {
+        const void *fn = wave_ring_sweeps_f64_symbol(m);
+        if (!fn || groups < 1 || args.members < 1 || args.count < 1 || !args.mbox)
+            return hipErrorInvalidValue;
+        const unsigned int grid = static_cast<unsigned int>(groups) * static_cast<unsigned int>(args.members);
+        void *params[] = {const_cast<WaveGroupArgs *>(&args)};
+        return hipLaunchKernel(fn, dim3(grid), dim3(64 * kRingWaves), params, ring_lds_bytes(m), stream);
+    }
+} // namespace jsvd

@@ -1,7 +1,9 @@
 """Failure path of the software barriers of the large-matrix kernels.
 
-The grouped wave kernel (flag barriers among a group's workgroups) and the
-grid-barrier kernel assume that every workgroup of their grid is resident.
+The grouped wave kernel (flag barriers among a group's workgroups), its ring
+form (mailbox waits between neighbouring workgroups, a flag barrier per
+sweep) and the grid-barrier kernel assume that every workgroup of their grid
+is resident.
 Each barrier wait is bounded: past the bound the launch is marked faulted,
 every workgroup leaves its loops, the matrices get stop reason
 STOP_BARRIER_TIMEOUT and the synchronous entry points return JSVD_EDEVICE.
@@ -41,7 +43,7 @@ def _run_plan(a_h, eps=1e-13, sweeps=30):
     return plan, (a, u, s, v, sw, stop)
 
 
-@pytest.mark.parametrize("shape", [(1100, 520), (2100, 2050)], ids=["grouped-wave", "grid-barrier"])
+@pytest.mark.parametrize("shape", [(400, 300), (1100, 520), (2100, 2050)], ids=["ring", "grouped-wave", "grid-barrier"])
 def test_barrier_timeout_is_reported_and_the_plan_recovers(shape, monkeypatch):
     import torch
 

@@ -174,6 +174,44 @@ def test_tall_batch_streamed_against_oracle():
         assert single[3] == got[k][3]
 
 
+@pytest.fixture
+def barrier_forms(monkeypatch):
+    """Plans built while this fixture is active use the barrier forms of the
+    wave solves (grouped flag barriers, one workgroup per matrix) instead of
+    the ring form."""
+    monkeypatch.setenv("JSVD_WAVE_RING", "0")
+    yield
+    monkeypatch.delenv("JSVD_WAVE_RING")
+
+
+def test_ring_form_equals_barrier_forms(barrier_forms):
+    # The ring form (columns held in registers, moved through LDS and tagged
+    # mailbox words) does the grouped kernel's pair arithmetic in the same
+    # order, so every result must equal the barrier forms' bitwise: a lone
+    # 256 x 256, a lone odd-width 300 x 151 (a dummy column in the circle),
+    # a mixed batch whose 120 matrices of 160 x 150 the barrier path solves
+    # one workgroup per matrix, and 40 of 512 x 512 (eight rows per lane,
+    # fewer ring groups than matrices, so groups take several in turn).
+    _note("ring form vs barrier forms")
+    rng = np.random.default_rng(4242)
+    batches = [[rng.standard_normal((256, 256))], [rng.standard_normal((300, 151))],
+               [rng.standard_normal(sh) for sh in [(256, 256), (300, 151), (512, 100)] + [(160, 150)] * 120],
+               [gen.gaussian_square(512, seed=gen.BASE_SEED + 100 + k) for k in range(40)]]
+    for mats in batches:
+        barrier = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
+        os.environ["JSVD_WAVE_RING"] = "1"
+        try:
+            ring = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
+        finally:
+            os.environ["JSVD_WAVE_RING"] = "0"
+        for k, (x, y) in enumerate(zip(ring, barrier)):
+            for a, b in zip(x[:3], y[:3]):
+                np.testing.assert_array_equal(a, b, err_msg=f"ring vs barrier {mats[k].shape} #{k}")
+            assert x[3] == y[3] and x[4] == y[4] == jsvd.STOP_CONVERGED
+    a = batches[1][0]
+    _assert_parity(a, jsvd.svd(a, epsilon=1e-14, max_sweeps=30)[:4], oracle.solve(a, 1e-14, 30), 1e-14, "300x151")
+
+
 def test_tall_single_streamed_held_rows():
     # A lone plain-path matrix of more than 1024 rows and at most 512 columns
     # runs four-wave groups with its rows streamed; each pair keeps its first
