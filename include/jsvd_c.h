@@ -97,7 +97,8 @@ void jsvd_config_default(jsvd_config *cfg);
  * the C++ API throws. */
 const char *jsvd_last_error(void);
 
-/* Number of the library's ABI revision. */
+/* Number of the library's ABI revision (2: jsvd_plan_info gained
+ * ring_matrices; a caller built against revision 1 passes a shorter struct). */
 int32_t jsvd_abi_version(void);
 
 /* Single solve from host memory, synchronous.
@@ -140,6 +141,8 @@ typedef struct jsvd_plan_info
     int32_t precision;
     int32_t buckets;           /* work-list launches (shape class x LDS occupancy) */
     int32_t blocked_matrices;  /* large matrices on the blocked path */
+    int32_t ring_matrices;     /* large matrices solved by the ring form of the wave kernels */
+    int32_t pad;
 } jsvd_plan_info;
 
 /* Packed layout of a batch (no reference counterpart: the reference indexes

@@ -387,7 +387,7 @@ struct jsvd_plan
     // first inside a launch.
     struct WaveLaunch
     {
-        int64_t m_max = 0;
+        int64_t m_max = 0, n_max = 0;
         int first = 0, count = 0;
         int members = 1; // workgroups per group; 0 = one workgroup per matrix
         int groups = 1;
@@ -693,7 +693,7 @@ namespace
                                                2 * int64_t(l.groups) * l.mbox_group, stream));
                 args.mbox = p.wv_mbox.ptr;
                 args.mbox_group = l.mbox_group;
-                JSVD_HIP(jsvd::launch_wave_ring_sweeps_f64(l.m_max, l.groups, args, stream));
+                JSVD_HIP(jsvd::launch_wave_ring_sweeps_f64(l.m_max, l.n_max, l.groups, args, stream));
             }
             else
             {
@@ -767,6 +767,7 @@ namespace
             {
                 const int64_t k = p.wave[e];
                 l.m_max = std::max(l.m_max, p.m[k]);
+                l.n_max = std::max(l.n_max, p.n[k]);
                 max_half = std::max<int64_t>(max_half, (p.n[k] + 1) / 2);
                 ++e;
             }
@@ -776,21 +777,22 @@ namespace
             // held in registers) for at least one matrix at a time; groups
             // take the launch's matrices in turn when fewer groups than
             // matrices are resident.
-            if (ring_enabled() && l.m_max <= jsvd::ring_max_rows() && max_half >= 2)
+            if (ring_enabled() && jsvd::ring_fits(l.m_max, l.n_max) && max_half >= 2)
             {
+                const int waves = jsvd::ring_waves(l.m_max, l.n_max);
                 int occupancy = 0;
                 JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                    &occupancy, jsvd::wave_ring_sweeps_f64_symbol(l.m_max), 64 * jsvd::ring_waves(),
-                    jsvd::ring_lds_bytes(l.m_max)));
+                    &occupancy, jsvd::wave_ring_sweeps_f64_symbol(l.m_max, l.n_max), 64 * waves,
+                    jsvd::ring_lds_bytes(l.m_max, l.n_max)));
                 const int64_t resident = int64_t(occupancy) * cus;
-                const int64_t members = (max_half + jsvd::ring_waves() - 1) / jsvd::ring_waves();
+                const int64_t members = (max_half + waves - 1) / waves;
                 if (occupancy > 0 && members <= 256 && members <= resident)
                 {
                     l.ring = true;
-                    l.waves = jsvd::ring_waves();
+                    l.waves = waves;
                     l.members = static_cast<int>(members);
                     l.groups = static_cast<int>(std::min<int64_t>(l.count, resident / members));
-                    l.mbox_group = jsvd::ring_mailbox_words(l.m_max, l.members);
+                    l.mbox_group = jsvd::ring_mailbox_words(l.m_max, l.n_max, l.members);
                     max_flags = std::max<int64_t>(max_flags, int64_t(l.groups) * l.members);
                     max_mbox = std::max<int64_t>(max_mbox, int64_t(l.groups) * l.mbox_group);
                     p.wave_launches.push_back(l);
@@ -942,7 +944,7 @@ const char *jsvd_last_error(void)
 
 int32_t jsvd_abi_version(void)
 {
-    return 1;
+    return 2;
 }
 
 jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, const jsvd_config *cfg_in,
@@ -1617,6 +1619,11 @@ jsvd_status jsvd_plan_get_info(const jsvd_plan *p, jsvd_plan_info *info)
     info->buckets = static_cast<int32_t>(p->buckets.size());
     info->precision = p->cfg.precision;
     info->blocked_matrices = static_cast<int32_t>(p->blocked);
+    int64_t ring = 0;
+    for (const auto &l : p->wave_launches)
+        ring += l.ring ? l.count : 0;
+    info->ring_matrices = static_cast<int32_t>(ring);
+    info->pad = 0;
     return JSVD_OK;
 }
 

@@ -216,15 +216,16 @@ namespace jsvd
     // pair's columns held in registers for the whole solve; columns move to
     // the neighbouring pair positions through LDS inside a workgroup and
     // through tagged mailbox words between neighbouring workgroups, so a
-    // round has no group barrier).  For matrices of at most
-    // ring_max_rows() rows (m >= n), groups of ceil(half / ring_waves())
-    // workgroups of ring_waves() waves.
-    int ring_waves();
-    int ring_max_rows();
-    int ring_lds_bytes(int64_t m);
-    int64_t ring_mailbox_words(int64_t m, int members);
-    const void *wave_ring_sweeps_f64_symbol(int64_t m);
-    hipError_t launch_wave_ring_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream);
+    // round has no group barrier).  A launch whose largest matrix has m rows
+    // and n columns (every matrix m >= n) fits when m <= 512, or m <= 1024
+    // and n <= 512; its groups have ceil(half / ring_waves(m, n)) workgroups.
+    bool ring_fits(int64_t m, int64_t n);
+    int ring_waves(int64_t m, int64_t n);
+    int ring_lds_bytes(int64_t m, int64_t n);
+    int64_t ring_mailbox_words(int64_t m, int64_t n, int members);
+    const void *wave_ring_sweeps_f64_symbol(int64_t m, int64_t n);
+    hipError_t launch_wave_ring_sweeps_f64(int64_t m, int64_t n, int groups, const WaveGroupArgs &args,
+                                           hipStream_t stream);
     // Kernel instance for matrices of up to m rows (all matrices of one
     // launch share it: ceil(m / 64) row registers per lane) and workgroups
     // of `waves` (8 or 16) waves.

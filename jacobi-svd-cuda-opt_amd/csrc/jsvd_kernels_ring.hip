@@ -1,10 +1,11 @@
-// jsvd_kernels_ring.hip -- ring form of the grouped plain-order sweeps: one
-// or a few mid-size matrices (at most 512 rows) solved by groups of
+// jsvd_kernels_ring.hip -- ring form of the grouped plain-order sweeps:
+// mid-size matrices (at most 1024 rows and 512 columns) solved by groups of
 // workgroups that never meet at a group barrier inside a sweep.
 //
 // Every column pair position k (0 <= k < half) belongs to one wave for the
 // whole solve: wave k % W of workgroup k / W holds the pair's two columns of
-// A and of V in registers (lane l: rows l + 64 j).  In the reference's circle
+// A and of V in registers (lane l: rows l + 64 j; RA row registers for A, RV
+// for V; eight-wave workgroups up to 512 rows, four-wave ones above).  In the reference's circle
 // (src/domain/jacobi_schedule.cuh:16-57, src/domain/jacobi_rotation_kernels.cuh:18-42)
 // every column except column 0 moves to the next circle slot each round, so
 // after a round's rotations
@@ -38,31 +39,29 @@ namespace jsvd
 {
     namespace
     {
-        constexpr int kRingWaves = 8;
-        constexpr int kRingRowsMax = 8; // row registers per lane: m <= 512
+        constexpr int kRingRowsMax = 8;   // row registers per lane of the eight-wave form: m <= 512
+        constexpr int kRingTallRows = 16; // the four-wave form: m <= 1024 (and n <= 512)
 
-        // Words of one mailbox message: the A column then the V column, R
-        // words of 64 lanes x two halves each.
-        template <int R>
-        constexpr int kRingMsg = 4 * 64 * R;
+        // Words of one mailbox message: the A column (RA words of 64 lanes x
+        // two halves), then the V column (RV words).
+        template <int RA, int RV>
+        constexpr int kRingMsg = 2 * 64 * (RA + RV);
 
         __device__ __forceinline__ unsigned long long tagged(unsigned int tag, unsigned int bits)
         {
             return (static_cast<unsigned long long>(tag) << 32) | bits;
         }
 
-        template <int R>
-        __device__ __forceinline__ void ring_send(unsigned long long *msg, unsigned int tag, const double (&a)[R],
-                                                  const double (&v)[R], int lane)
+        template <int RA, int RV>
+        __device__ __forceinline__ void ring_send(unsigned long long *msg, unsigned int tag, const double (&a)[RA],
+                                                  const double (&v)[RV], int lane)
         {
 #pragma unroll
-            for (int part = 0; part < 2; ++part)
-#pragma unroll
-                for (int j = 0; j < R; ++j)
+            for (int j = 0; j < RA + RV; ++j)
                 {
                     const unsigned long long bits =
-                        static_cast<unsigned long long>(__double_as_longlong(part == 0 ? a[j] : v[j]));
-                    unsigned long long *w = msg + (part * R + j) * 128 + lane * 2;
+                        static_cast<unsigned long long>(__double_as_longlong(j < RA ? a[j] : v[j - RA]));
+                    unsigned long long *w = msg + j * 128 + lane * 2;
                     __hip_atomic_store(w, tagged(tag, static_cast<unsigned int>(bits)), __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_store(w + 1, tagged(tag, static_cast<unsigned int>(bits >> 32)), __ATOMIC_RELAXED,
@@ -72,49 +71,58 @@ namespace jsvd
 
         // Waits (bounded) until every word of the message carries `tag`, then
         // assembles the two columns.  Returns false when the wait expired.
-        template <int R>
-        __device__ __forceinline__ bool ring_receive(const unsigned long long *msg, unsigned int tag, double (&a)[R],
-                                                     double (&v)[R], int lane, const dev::SpinBound &bound)
+        // The whole message is read at once (one round trip when it has
+        // arrived); while it has not, each lane polls only the last word it
+        // reads (the last one its sender lane writes), so a waiting wave
+        // moves 512 bytes per poll instead of the whole message.
+        template <int RA, int RV>
+        __device__ __forceinline__ bool ring_receive(const unsigned long long *msg, unsigned int tag, double (&a)[RA],
+                                                     double (&v)[RV], int lane, const dev::SpinBound &bound)
         {
             const uint64_t t0 = static_cast<uint64_t>(wall_clock64());
+            const unsigned long long *last = msg + (RA + RV - 1) * 128 + lane * 2 + 1;
             for (;;)
             {
                 bool ready = true;
 #pragma unroll
-                for (int part = 0; part < 2; ++part)
-#pragma unroll
-                    for (int j = 0; j < R; ++j)
+                for (int j = 0; j < RA + RV; ++j)
                     {
-                        const unsigned long long *w = msg + (part * R + j) * 128 + lane * 2;
+                        const unsigned long long *w = msg + j * 128 + lane * 2;
                         const unsigned long long lo =
                             __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         const unsigned long long hi =
                             __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ready &= static_cast<unsigned int>(lo >> 32) == tag && static_cast<unsigned int>(hi >> 32) == tag;
                         const double x = __longlong_as_double(static_cast<long long>((hi << 32) | (lo & 0xffffffffull)));
-                        if (part == 0)
+                        if (j < RA)
                             a[j] = x;
                         else
-                            v[j] = x;
+                            v[j - RA] = x;
                     }
                 if (__all(ready))
                     return true;
-                const bool expired = lane == 0 && dev::spin_expired(bound, t0);
-                if (__any(expired))
-                    return false;
+                for (;;)
+                {
+                    const bool expired = lane == 0 && dev::spin_expired(bound, t0);
+                    if (__any(expired))
+                        return false;
+                    const unsigned long long x = __hip_atomic_load(last, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (__all(static_cast<unsigned int>(x >> 32) == tag))
+                        break;
+                }
             }
         }
 
         // wave_pair_f64 on register-held columns: p is the top column, q the
         // bottom one.  Rows beyond m (and V rows beyond n) hold zeros, which
         // leave the dot chains and the updates unchanged.
-        template <int R>
-        __device__ __forceinline__ int ring_pair(double (&xp)[R], double (&xq)[R], double (&yp)[R], double (&yq)[R],
-                                                 double eps)
+        template <int RA, int RV>
+        __device__ __forceinline__ int ring_pair(double (&xp)[RA], double (&xq)[RA], double (&yp)[RV],
+                                                 double (&yq)[RV], double eps)
         {
             double pp = 0.0, qq = 0.0, pq = 0.0;
 #pragma unroll
-            for (int j = 0; j < R; ++j)
+            for (int j = 0; j < RA; ++j)
             {
                 pp = fma(xp[j], xp[j], pp);
                 qq = fma(xq[j], xq[j], qq);
@@ -126,14 +134,14 @@ namespace jsvd
             double c, s;
             dev::rotation_from_tangent(dev::rotation_tangent(pp, qq, pq), c, s);
 #pragma unroll
-            for (int j = 0; j < R; ++j)
+            for (int j = 0; j < RA; ++j)
             {
                 const double a = xp[j], b = xq[j];
                 xp[j] = dev::rot_first(c, s, a, b);
                 xq[j] = dev::rot_second(c, s, a, b);
             }
 #pragma unroll
-            for (int j = 0; j < R; ++j)
+            for (int j = 0; j < RV; ++j)
             {
                 const double a = yp[j], b = yq[j];
                 yp[j] = dev::rot_first(c, s, a, b);
@@ -142,36 +150,36 @@ namespace jsvd
             return 1;
         }
 
-        template <int R>
-        __device__ __forceinline__ void lds_put(double *slot, const double (&a)[R], const double (&v)[R], int lane)
+        template <int RA, int RV>
+        __device__ __forceinline__ void lds_put(double *slot, const double (&a)[RA], const double (&v)[RV], int lane)
         {
 #pragma unroll
-            for (int j = 0; j < R; ++j)
-            {
+            for (int j = 0; j < RA; ++j)
                 slot[j * 64 + lane] = a[j];
-                slot[(R + j) * 64 + lane] = v[j];
-            }
+#pragma unroll
+            for (int j = 0; j < RV; ++j)
+                slot[(RA + j) * 64 + lane] = v[j];
         }
 
-        template <int R>
-        __device__ __forceinline__ void lds_get(const double *slot, double (&a)[R], double (&v)[R], int lane)
+        template <int RA, int RV>
+        __device__ __forceinline__ void lds_get(const double *slot, double (&a)[RA], double (&v)[RV], int lane)
         {
 #pragma unroll
-            for (int j = 0; j < R; ++j)
-            {
+            for (int j = 0; j < RA; ++j)
                 a[j] = slot[j * 64 + lane];
-                v[j] = slot[(R + j) * 64 + lane];
-            }
+#pragma unroll
+            for (int j = 0; j < RV; ++j)
+                v[j] = slot[(RA + j) * 64 + lane];
         }
 
-        // LDS: per wave a top slot and a bottom slot of 2 R x 64 doubles (A
-        // column, then V column).  Mailboxes of a group: for each boundary b
-        // (between workgroups b and b + 1), direction d (0: top forward, 1:
-        // bottom back) and round parity, one message.
-        template <int R>
-        __global__ __launch_bounds__(64 * kRingWaves) void wave_ring_sweeps_f64(WaveGroupArgs g)
+        // LDS: per wave a top slot and a bottom slot of (RA + RV) x 64
+        // doubles (A column, then V column).  Mailboxes of a group: for each
+        // boundary b (between workgroups b and b + 1), direction d (0: top
+        // forward, 1: bottom back) and round parity, one message.
+        template <int RA, int RV, int W>
+        __global__ __launch_bounds__(64 * W) void wave_ring_sweeps_f64(WaveGroupArgs g)
         {
-            constexpr int W = kRingWaves, MSG = kRingMsg<R>, SLOT = 2 * R * 64;
+            constexpr int MSG = kRingMsg<RA, RV>, SLOT = (RA + RV) * 64;
             extern __shared__ double ring_lds[];
             const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
             const int group = blockIdx.x / g.members, member = blockIdx.x - group * g.members;
@@ -193,13 +201,18 @@ namespace jsvd
                 const int rounds = n < 2 ? 0 : even - 1;
                 const bool active = k < half;
                 const int ct = k, cb = even - 1 - k;
-                double tA[R], tV[R], bA[R], bV[R];
+                double tA[RA], tV[RV], bA[RA], bV[RV];
 #pragma unroll
-                for (int j = 0; j < R; ++j)
+                for (int j = 0; j < RA; ++j)
                 {
                     const int64_t i = lane + 64 * j;
                     tA[j] = (active && ct < n && i < m) ? w.a[int64_t(ct) * m + i] : 0.0;
                     bA[j] = (active && cb < n && i < m) ? w.a[int64_t(cb) * m + i] : 0.0;
+                }
+#pragma unroll
+                for (int j = 0; j < RV; ++j)
+                {
+                    const int i = lane + 64 * j;
                     tV[j] = (active && ct < n && i < n) ? w.v[int64_t(ct) * n + i] : 0.0;
                     bV[j] = (active && cb < n && i < n) ? w.v[int64_t(cb) * n + i] : 0.0;
                 }
@@ -224,16 +237,16 @@ namespace jsvd
                         ++tag;
                         const int par = static_cast<int>(tag & 1u) * MSG;
                         if (active)
-                            any |= ring_pair<R>(tA, bA, tV, bV, g.eps);
+                            any |= ring_pair<RA, RV>(tA, bA, tV, bV, g.eps);
                         // The boundary columns leave at once, so they travel
                         // while the workgroup exchanges the others.  The
                         // receiver has read this parity's previous message:
                         // it needed it to finish the round before the one
                         // this workgroup just completed.
                         if (send_fwd)
-                            ring_send<R>(fwd_out + par, tag, tA, tV, lane);
+                            ring_send<RA, RV>(fwd_out + par, tag, tA, tV, lane);
                         if (send_back)
-                            ring_send<R>(back_out + par, tag, bA, bV, lane);
+                            ring_send<RA, RV>(back_out + par, tag, bA, bV, lane);
                         // Every read of the previous round's slots is done
                         // (and every wave agrees on a failed wait).
                         if (__syncthreads_or(failed))
@@ -243,8 +256,8 @@ namespace jsvd
                         }
                         if (active)
                         {
-                            lds_put<R>(my_top, tA, tV, lane);
-                            lds_put<R>(my_bot, bA, bV, lane);
+                            lds_put<RA, RV>(my_top, tA, tV, lane);
+                            lds_put<RA, RV>(my_bot, bA, bV, lane);
                         }
                         __syncthreads();
                         if (active)
@@ -257,22 +270,22 @@ namespace jsvd
                                 if (half > 1)
                                 {
 #pragma unroll
-                                    for (int j = 0; j < R; ++j)
-                                    {
+                                    for (int j = 0; j < RA; ++j)
                                         bA[j] = tA[j];
+#pragma unroll
+                                    for (int j = 0; j < RV; ++j)
                                         bV[j] = tV[j];
-                                    }
                                 }
                             }
                             else if (wave < W - 1)
-                                lds_get<R>(my_bot + 2 * SLOT, bA, bV, lane); // wave + 1's bottom
-                            else if (!ring_receive<R>(back_in + par, tag, bA, bV, lane, bound))
+                                lds_get<RA, RV>(my_bot + 2 * SLOT, bA, bV, lane); // wave + 1's bottom
+                            else if (!ring_receive<RA, RV>(back_in + par, tag, bA, bV, lane, bound))
                                 failed = 1;
                             if (k == 1)
-                                lds_get<R>(my_bot - 2 * SLOT, tA, tV, lane); // wave 0's bottom
+                                lds_get<RA, RV>(my_bot - 2 * SLOT, tA, tV, lane); // wave 0's bottom
                             else if (k >= 2 && wave > 0)
-                                lds_get<R>(my_top - 2 * SLOT, tA, tV, lane); // wave - 1's top
-                            else if (k >= 2 && !ring_receive<R>(fwd_in + par, tag, tA, tV, lane, bound))
+                                lds_get<RA, RV>(my_top - 2 * SLOT, tA, tV, lane); // wave - 1's top
+                            else if (k >= 2 && !ring_receive<RA, RV>(fwd_in + par, tag, tA, tV, lane, bound))
                                 failed = 1;
                         }
                     }
@@ -297,7 +310,7 @@ namespace jsvd
                 if (!failed && active)
                 {
 #pragma unroll
-                    for (int j = 0; j < R; ++j)
+                    for (int j = 0; j < RA; ++j)
                     {
                         const int64_t i = lane + 64 * j;
                         if (i < m)
@@ -306,6 +319,11 @@ namespace jsvd
                             if (cb < n)
                                 w.a[int64_t(cb) * m + i] = bA[j];
                         }
+                    }
+#pragma unroll
+                    for (int j = 0; j < RV; ++j)
+                    {
+                        const int i = lane + 64 * j;
                         if (i < n)
                         {
                             w.v[int64_t(ct) * n + i] = tV[j];
@@ -339,58 +357,99 @@ namespace jsvd
             }
         }
 
-        template <int R>
+        // Instances: eight-wave workgroups with RA = RV = ceil(m / 64) for m
+        // <= 512; four-wave workgroups with 16 A rows and ceil(n / 64) V rows
+        // per lane for 512 < m <= 1024 (n <= 512).
+        template <int RA, int RV, int W>
         const void *ring_symbol_t()
         {
-            return reinterpret_cast<const void *>(&wave_ring_sweeps_f64<R>);
+            return reinterpret_cast<const void *>(&wave_ring_sweeps_f64<RA, RV, W>);
         }
-        const void *(*const kRingSymbol[])() = {ring_symbol_t<1>, ring_symbol_t<2>, ring_symbol_t<3>,
-                                                ring_symbol_t<4>, ring_symbol_t<5>, ring_symbol_t<6>,
-                                                ring_symbol_t<7>, ring_symbol_t<8>};
+        const void *(*const kRingSymbol[])() = {ring_symbol_t<1, 1, 8>, ring_symbol_t<2, 2, 8>, ring_symbol_t<3, 3, 8>,
+                                                ring_symbol_t<4, 4, 8>, ring_symbol_t<5, 5, 8>, ring_symbol_t<6, 6, 8>,
+                                                ring_symbol_t<7, 7, 8>, ring_symbol_t<8, 8, 8>};
         static_assert(sizeof(kRingSymbol) / sizeof(kRingSymbol[0]) == kRingRowsMax, "one instance per row count");
+        const void *(*const kRingTallSymbol[])() = {
+            ring_symbol_t<16, 1, 4>, ring_symbol_t<16, 2, 4>, ring_symbol_t<16, 3, 4>, ring_symbol_t<16, 4, 4>,
+            ring_symbol_t<16, 5, 4>, ring_symbol_t<16, 6, 4>, ring_symbol_t<16, 7, 4>, ring_symbol_t<16, 8, 4>};
+        static_assert(sizeof(kRingTallSymbol) / sizeof(kRingTallSymbol[0]) == kRingRowsMax,
+                      "one tall instance per V row count");
 
-        int ring_rows(int64_t m)
+        int rows_of(int64_t x)
         {
-            return static_cast<int>((m + 63) / 64);
+            return static_cast<int>((x + 63) / 64);
+        }
+
+        // (A rows, V rows, waves) per lane of the instance for a launch whose
+        // largest matrix has m rows and n columns; RA = 0 when none fits.
+        void ring_shape(int64_t m, int64_t n, int &ra, int &rv, int &waves)
+        {
+            ra = rv = 0;
+            waves = 8;
+            if (m < 1 || n < 1 || n > m)
+                return;
+            if (m <= 64 * kRingRowsMax)
+            {
+                ra = rv = rows_of(m);
+                return;
+            }
+            if (m <= 64 * kRingTallRows && n <= 64 * kRingRowsMax)
+            {
+                ra = kRingTallRows;
+                rv = rows_of(n);
+                waves = 4; // eight waves of this many registers would not be resident together
+            }
         }
     } // namespace
 
-    int ring_waves()
+    bool ring_fits(int64_t m, int64_t n)
     {
-        return kRingWaves;
+        int ra, rv, w;
+        ring_shape(m, n, ra, rv, w);
+        return ra > 0;
     }
 
-    int ring_max_rows()
+    int ring_waves(int64_t m, int64_t n)
     {
-        return 64 * kRingRowsMax;
+        int ra, rv, w;
+        ring_shape(m, n, ra, rv, w);
+        return w;
     }
 
-    int ring_lds_bytes(int64_t m)
+    int ring_lds_bytes(int64_t m, int64_t n)
     {
-        // Two slots per wave of 2 R x 64 doubles.
-        return kRingWaves * 2 * 2 * ring_rows(m) * 64 * 8;
+        // Two slots per wave of (RA + RV) x 64 doubles.
+        int ra, rv, w;
+        ring_shape(m, n, ra, rv, w);
+        return w * 2 * (ra + rv) * 64 * 8;
     }
 
-    int64_t ring_mailbox_words(int64_t m, int members)
+    int64_t ring_mailbox_words(int64_t m, int64_t n, int members)
     {
         // Per workgroup (as the left end of a boundary): two directions, two
-        // parities, one message each.
-        return int64_t(members) * 2 * 2 * (4 * 64 * ring_rows(m));
+        // parities, one message of 2 x 64 x (RA + RV) words each.
+        int ra, rv, w;
+        ring_shape(m, n, ra, rv, w);
+        return int64_t(members) * 2 * 2 * (2 * 64 * (ra + rv));
     }
 
-    const void *wave_ring_sweeps_f64_symbol(int64_t m)
+    const void *wave_ring_sweeps_f64_symbol(int64_t m, int64_t n)
     {
-        const int r = ring_rows(m);
-        return r >= 1 && r <= kRingRowsMax ? kRingSymbol[r - 1]() : nullptr;
+        int ra, rv, w;
+        ring_shape(m, n, ra, rv, w);
+        if (ra == 0)
+            return nullptr;
+        return w == 8 ? kRingSymbol[ra - 1]() : kRingTallSymbol[rv - 1]();
     }
 
-    hipError_t launch_wave_ring_sweeps_f64(int64_t m, int groups, const WaveGroupArgs &args, hipStream_t stream)
+    hipError_t launch_wave_ring_sweeps_f64(int64_t m, int64_t n, int groups, const WaveGroupArgs &args,
+                                           hipStream_t stream)
     {
-        const void *fn = wave_ring_sweeps_f64_symbol(m);
+        const void *fn = wave_ring_sweeps_f64_symbol(m, n);
         if (!fn || groups < 1 || args.members < 1 || args.count < 1 || !args.mbox)
             return hipErrorInvalidValue;
         const unsigned int grid = static_cast<unsigned int>(groups) * static_cast<unsigned int>(args.members);
         void *params[] = {const_cast<WaveGroupArgs *>(&args)};
-        return hipLaunchKernel(fn, dim3(grid), dim3(64 * kRingWaves), params, ring_lds_bytes(m), stream);
+        return hipLaunchKernel(fn, dim3(grid), dim3(64 * ring_waves(m, n)), params, ring_lds_bytes(m, n), stream);
     }
 } // namespace jsvd

@@ -53,6 +53,8 @@ class JsvdPlanInfo(ctypes.Structure):
         ("precision", ctypes.c_int32),
         ("buckets", ctypes.c_int32),
         ("blocked_matrices", ctypes.c_int32),
+        ("ring_matrices", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 

@@ -83,13 +83,13 @@ def test_lds_path_against_oracle(shape):
                                            ((5000, 40), 256), ((1024, 129), 1024), ((1100, 520), 256),
                                            ((600, 513), 1024)])
 def test_persistent_path_against_oracle(shape, threads):
-    # These shapes exceed the LDS budget, so each runs alone in the grouped
-    # wave kernel (wave_group_sweeps_f64): 256 x 256 with eight-wave
-    # workgroups and rows in registers; 300 x 150 and 300 x 151 with
-    # four-wave ones (more than 256 rows, at most 256 columns); 1024 x 129
-    # four-wave, 16 rows per lane in registers; 5000 x 40 four-wave with rows
-    # streamed; 600 x 513 eight-wave with registers and 1100 x 520 eight-wave
-    # streamed (more than 256 column pairs rule out four-wave groups).
+    # These shapes exceed the LDS budget, so each runs alone in a wave
+    # kernel: 256 x 256, 300 x 150 and 300 x 151 in the ring form with
+    # eight-wave workgroups; 1024 x 129 in the ring form with four-wave ones
+    # (16 A rows per lane); 5000 x 40 in four-wave grouped workgroups with rows
+    # streamed (5120 held); 600 x 513 eight-wave grouped with rows in
+    # registers and 1100 x 520 eight-wave grouped streamed (more than 512
+    # columns rule out the ring, more than 256 column pairs four-wave groups).
     # For the two wide Gaussian inputs the final column order is chaotic
     # under rounding: a one-ulp perturbation of the 1100 x 520 input moves
     # about 480 of its 520 sigma positions in the oracle itself (sorted sigma
@@ -107,11 +107,10 @@ def test_persistent_path_against_oracle(shape, threads):
 
 
 def test_grouped_wave_batch_against_oracle_and_single_solves():
-    # Matrices too large for LDS with at most 512 rows go to the wave-per-pair
-    # kernels, one launch per row-register count: the 121 matrices of up to
-    # 192 rows run one workgroup per matrix, the few wider ones grouped
-    # launches (groups of workgroups with barriers of their own).  The pair
-    # arithmetic is the same in both forms and for any group size, so every
+    # Matrices too large for LDS with at most 512 rows go to the ring form of
+    # the wave-per-pair kernels, one launch per row-register count (the 121
+    # matrices of up to 192 rows in groups that take several matrices in
+    # turn).  The pair arithmetic does not depend on the launch, so every
     # matrix must equal its single solve bitwise, and the oracle to tolerance.
     _note("grouped wave batch")
     rng = np.random.default_rng(515)
@@ -190,14 +189,22 @@ def test_ring_form_equals_barrier_forms(barrier_forms):
     # order, so every result must equal the barrier forms' bitwise: a lone
     # 256 x 256, a lone odd-width 300 x 151 (a dummy column in the circle),
     # a mixed batch whose 120 matrices of 160 x 150 the barrier path solves
-    # one workgroup per matrix, and 40 of 512 x 512 (eight rows per lane,
-    # fewer ring groups than matrices, so groups take several in turn).
+    # one workgroup per matrix, 40 of 512 x 512 (eight rows per lane, fewer
+    # ring groups than matrices, so groups take several in turn), and tall
+    # ones of 513 - 1024 rows (four-wave workgroups, 16 A rows and up to
+    # three V rows per lane; the barrier path streams their rows).
     _note("ring form vs barrier forms")
     rng = np.random.default_rng(4242)
     batches = [[rng.standard_normal((256, 256))], [rng.standard_normal((300, 151))],
                [rng.standard_normal(sh) for sh in [(256, 256), (300, 151), (512, 100)] + [(160, 150)] * 120],
-               [gen.gaussian_square(512, seed=gen.BASE_SEED + 100 + k) for k in range(40)]]
+               [gen.gaussian_square(512, seed=gen.BASE_SEED + 100 + k) for k in range(40)],
+               [rng.standard_normal(sh) for sh in [(1024, 129), (777, 31)] + [(700, 90)] * 16]]
     for mats in batches:
+        os.environ["JSVD_WAVE_RING"] = "1"
+        plan = jsvd.BatchPlan([a.shape[0] for a in mats], [a.shape[1] for a in mats], epsilon=1e-14, max_sweeps=30)
+        os.environ["JSVD_WAVE_RING"] = "0"
+        assert plan.info["ring_matrices"] == plan.info["large_matrices"] == len(mats)
+        plan.close()
         barrier = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
         os.environ["JSVD_WAVE_RING"] = "1"
         try:
