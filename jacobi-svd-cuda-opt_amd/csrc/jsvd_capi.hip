@@ -396,7 +396,8 @@ struct jsvd_plan
         int64_t mbox_group = 0; // ring form: mailbox words per group
     };
     std::vector<int64_t> wave;                 // batch index per entry
-    std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry
+    std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry (barrier forms)
+    std::vector<char> wave_ring;                 // entry solved by the ring form (no workspace)
     std::vector<WaveLaunch> wave_launches;
     DevBuf<jsvd::WaveMatrix> d_wave;
     DevBuf<double> wv_a, wv_v;
@@ -660,6 +661,8 @@ namespace
         JSVD_HIP(jsvd::launch_fill_i32(p.wv_flags.ptr, 0, entries * p.cfg.max_sweeps, stream));
         for (int64_t e = 0; e < entries; ++e)
         {
+            if (p.wave_ring[e])
+                continue;
             const int64_t k = p.wave[e];
             const int64_t m = p.m[k];
             const int n = static_cast<int>(p.n[k]);
@@ -693,6 +696,10 @@ namespace
                                                2 * int64_t(l.groups) * l.mbox_group, stream));
                 args.mbox = p.wv_mbox.ptr;
                 args.mbox_group = l.mbox_group;
+                args.in_a = d_a;
+                args.out_u = d_u;
+                args.out_s = d_sigma;
+                args.out_v = d_v;
                 JSVD_HIP(jsvd::launch_wave_ring_sweeps_f64(l.m_max, l.n_max, l.groups, args, stream));
             }
             else
@@ -707,6 +714,8 @@ namespace
         }
         for (int64_t e = 0; e < entries; ++e)
         {
+            if (p.wave_ring[e])
+                continue;
             const int64_t k = p.wave[e];
             const int64_t m = p.m[k];
             const int n = static_cast<int>(p.n[k]);
@@ -745,18 +754,8 @@ namespace
             return cost(x) > cost(y);
         });
         const int64_t entries = static_cast<int64_t>(p.wave.size());
-        p.wave_a_off.resize(entries);
-        p.wave_v_off.resize(entries);
-        int64_t a_total = 0, v_total = 0;
         for (int64_t e = 0; e < entries; ++e)
-        {
-            const int64_t k = p.wave[e];
-            p.wave_a_off[e] = a_total;
-            p.wave_v_off[e] = v_total;
-            a_total += p.m[k] * p.n[k];
-            p.wave_bytes += 8.0 * double(p.m[k]) * double(p.n[k]);
-            v_total += p.n[k] * p.n[k];
-        }
+            p.wave_bytes += 8.0 * double(p.m[p.wave[e]]) * double(p.n[p.wave[e]]);
         int64_t max_flags = 1, max_mbox = 0;
         for (int64_t e = 0; e < entries;)
         {
@@ -832,6 +831,25 @@ namespace
             max_flags = std::max<int64_t>(max_flags, int64_t(l.groups) * l.members);
             p.wave_launches.push_back(l);
         }
+        // Column-major workspaces for the matrices of the barrier forms (the
+        // ring form reads and writes the caller's buffers directly).
+        p.wave_ring.assign(entries, 0);
+        for (const auto &l : p.wave_launches)
+            if (l.ring)
+                std::fill(p.wave_ring.begin() + l.first, p.wave_ring.begin() + l.first + l.count, 1);
+        p.wave_a_off.assign(entries, 0);
+        p.wave_v_off.assign(entries, 0);
+        int64_t a_total = 0, v_total = 0;
+        for (int64_t e = 0; e < entries; ++e)
+        {
+            if (p.wave_ring[e])
+                continue;
+            const int64_t k = p.wave[e];
+            p.wave_a_off[e] = a_total;
+            p.wave_v_off[e] = v_total;
+            a_total += p.m[k] * p.n[k];
+            v_total += p.n[k] * p.n[k];
+        }
         JSVD_CHECK(p.wv_a.alloc(a_total));
         JSVD_CHECK(p.wv_v.alloc(v_total));
         JSVD_CHECK(p.wv_flags.alloc(entries * p.cfg.max_sweeps));
@@ -841,12 +859,15 @@ namespace
         for (int64_t e = 0; e < entries; ++e)
         {
             const int64_t k = p.wave[e];
-            mats[e].a = p.wv_a.ptr + p.wave_a_off[e];
-            mats[e].v = p.wv_v.ptr + p.wave_v_off[e];
+            mats[e].a = p.wave_ring[e] ? nullptr : p.wv_a.ptr + p.wave_a_off[e];
+            mats[e].v = p.wave_ring[e] ? nullptr : p.wv_v.ptr + p.wave_v_off[e];
             mats[e].flags = p.wv_flags.ptr + e * p.cfg.max_sweeps;
             mats[e].m = static_cast<int32_t>(p.m[k]);
             mats[e].n = static_cast<int32_t>(p.n[k]);
             mats[e].index = static_cast<int32_t>(k);
+            mats[e].a_off = p.a_off[k];
+            mats[e].v_off = p.v_off[k];
+            mats[e].s_off = p.s_off[k];
         }
         JSVD_CHECK(p.d_wave.alloc(entries));
         JSVD_HIP(hipMemcpy(p.d_wave.ptr, mats.data(), sizeof(jsvd::WaveMatrix) * entries, hipMemcpyHostToDevice));

@@ -186,13 +186,14 @@ namespace jsvd
     // runs `groups` independent solves at once.
     struct WaveMatrix
     {
-        double *a;      // column-major m x n workspace (leading dimension m)
-        double *v;      // column-major n x n workspace, identity at launch
+        double *a;      // column-major m x n workspace (leading dimension m); null in the ring form
+        double *v;      // column-major n x n workspace, identity at launch; null in the ring form
         int32_t *flags; // max_sweeps sweep flags, zero at launch
         int32_t m;
         int32_t n;
         int32_t index;  // sweeps output slot
         int32_t pad;
+        int64_t a_off, v_off, s_off; // the matrix's offsets in the packed batch buffers (ring form)
     };
     struct WaveGroupArgs
     {
@@ -206,10 +207,13 @@ namespace jsvd
         int32_t *stop;          // optional: 0 converged, 1 stopped at max_sweeps, 2 barrier timed out
         unsigned int *fault;    // one word, zero at launch: set when a barrier wait hit spin_limit
         uint64_t spin_limit;    // wall-clock ticks a barrier may wait
-        // Ring form only: column mailboxes, ring_mailbox_words(m, members)
-        // 64-bit words per group, zero at launch.
+        // Ring form only: column mailboxes, ring_mailbox_words(m, n, members)
+        // 64-bit words per group, zero at launch; the packed row-major batch
+        // buffers the kernel reads A from and writes U, sigma and V to.
         unsigned long long *mbox;
         int64_t mbox_group;
+        const double *in_a;
+        double *out_u, *out_s, *out_v;
     };
     int wave_sweeps_max_rows();
     // Ring form of the grouped solves (one wave per column pair, every

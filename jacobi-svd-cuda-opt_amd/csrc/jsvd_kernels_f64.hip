@@ -818,6 +818,11 @@ namespace jsvd
             }
         }
 
+        // sigma_j = ||a_j|| of a column-major m x n matrix, one workgroup per
+        // column: thread t sums rows t, t + BS, ... (FMA chain), each wave's
+        // sums are added by the butterfly of dev::wave_sum, then the waves'
+        // totals in wave order.  The ring form of the wave solves reproduces
+        // this order for BS = 256 from its register-held columns.
         template <int BS>
         __global__ __launch_bounds__(BS) void column_norms_f64(const double *__restrict__ a, double *__restrict__ sigma,
                                                                int64_t m, int64_t lda)
@@ -829,7 +834,7 @@ namespace jsvd
             for (int64_t i = threadIdx.x; i < m; i += BS)
             {
                 const double x = col[i];
-                acc += x * x;
+                acc = fma(x, x, acc);
             }
             acc = dev::wave_sum(acc);
             if ((threadIdx.x & 63) == 0)

@@ -30,8 +30,12 @@
 // order, same reduction, rotation parameters and update roundings) and the
 // pairs come in the same order, so the results equal the grouped kernel's
 // bitwise.  After every whole sweep the columns are back at their starting
-// positions (top(k) = column k, bottom(k) = column E - 1 - k), where they are
-// written back to the workspace.
+// positions (top(k) = column k, bottom(k) = column E - 1 - k).  The kernel
+// reads A straight from the caller's packed row-major buffer, starts V as
+// the identity in registers, and at the end writes sigma, U = A / sigma
+// (zero where sigma <= eps, the reference's rule) and V row-major, with
+// column norms summed in column_norms_f64's order: no workspace, no helper
+// launches.
 #include "jsvd_device.hpp"
 #include "jsvd_kernels.hpp"
 
@@ -150,6 +154,28 @@ namespace jsvd
             return 1;
         }
 
+        // ||x|| of a register-held column in column_norms_f64<256>'s order:
+        // its thread t = lane + 64 w sums rows t, t + 256, ..., i.e. this
+        // lane's row registers j = w, w + 4, ... (FMA chain); each wave's sums
+        // are reduced by dev::wave_sum, then the four totals added in order.
+        // Rows beyond m hold zeros, which leave the chains unchanged.
+        template <int RA>
+        __device__ __forceinline__ double ring_norm(const double (&x)[RA])
+        {
+            double red[4];
+#pragma unroll
+            for (int w = 0; w < 4; ++w)
+            {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = w; j < RA; j += 4)
+                    acc = fma(x[j], x[j], acc);
+                red[w] = dev::wave_sum(acc);
+            }
+            const double t = ((red[0] + red[1]) + red[2]) + red[3];
+            return sqrt(fmax(t, 0.0));
+        }
+
         template <int RA, int RV>
         __device__ __forceinline__ void lds_put(double *slot, const double (&a)[RA], const double (&v)[RV], int lane)
         {
@@ -201,20 +227,23 @@ namespace jsvd
                 const int rounds = n < 2 ? 0 : even - 1;
                 const bool active = k < half;
                 const int ct = k, cb = even - 1 - k;
+                // Columns ct and cb of the row-major A (each load touches 64
+                // rows; once per solve), and of V = I.
+                const double *a_in = g.in_a + w.a_off;
                 double tA[RA], tV[RV], bA[RA], bV[RV];
 #pragma unroll
                 for (int j = 0; j < RA; ++j)
                 {
                     const int64_t i = lane + 64 * j;
-                    tA[j] = (active && ct < n && i < m) ? w.a[int64_t(ct) * m + i] : 0.0;
-                    bA[j] = (active && cb < n && i < m) ? w.a[int64_t(cb) * m + i] : 0.0;
+                    tA[j] = (active && ct < n && i < m) ? a_in[i * n + ct] : 0.0;
+                    bA[j] = (active && cb < n && i < m) ? a_in[i * n + cb] : 0.0;
                 }
 #pragma unroll
                 for (int j = 0; j < RV; ++j)
                 {
                     const int i = lane + 64 * j;
-                    tV[j] = (active && ct < n && i < n) ? w.v[int64_t(ct) * n + i] : 0.0;
-                    bV[j] = (active && cb < n && i < n) ? w.v[int64_t(cb) * n + i] : 0.0;
+                    tV[j] = (active && ct < n && i == ct) ? 1.0 : 0.0;
+                    bV[j] = (active && cb < n && i == cb) ? 1.0 : 0.0;
                 }
                 // Message slots of this workgroup's boundaries.
                 unsigned long long *fwd_out = mb + int64_t(member * 2 + 0) * 2 * MSG;       // top to member + 1
@@ -309,15 +338,27 @@ namespace jsvd
                 }
                 if (!failed && active)
                 {
+                    // sigma, then U = A / sigma (zero where sigma <= eps,
+                    // src/domain/jacobi_rotation_kernels.cuh:370-413) and V,
+                    // row-major like the transposes of the barrier forms.
+                    const double st = ring_norm<RA>(tA), sb = ring_norm<RA>(bA);
+                    double *u = g.out_u + w.a_off;
+                    double *vv = g.out_v + w.v_off;
+                    if (lane == 0)
+                    {
+                        g.out_s[w.s_off + ct] = st;
+                        if (cb < n)
+                            g.out_s[w.s_off + cb] = sb;
+                    }
 #pragma unroll
                     for (int j = 0; j < RA; ++j)
                     {
                         const int64_t i = lane + 64 * j;
                         if (i < m)
                         {
-                            w.a[int64_t(ct) * m + i] = tA[j];
+                            u[i * n + ct] = (st > g.eps) ? (tA[j] / st) : 0.0;
                             if (cb < n)
-                                w.a[int64_t(cb) * m + i] = bA[j];
+                                u[i * n + cb] = (sb > g.eps) ? (bA[j] / sb) : 0.0;
                         }
                     }
 #pragma unroll
@@ -326,9 +367,9 @@ namespace jsvd
                         const int i = lane + 64 * j;
                         if (i < n)
                         {
-                            w.v[int64_t(ct) * n + i] = tV[j];
+                            vv[int64_t(i) * n + ct] = tV[j];
                             if (cb < n)
-                                w.v[int64_t(cb) * n + i] = bV[j];
+                                vv[int64_t(i) * n + cb] = bV[j];
                         }
                     }
                 }
@@ -446,7 +487,8 @@ namespace jsvd
                                            hipStream_t stream)
     {
         const void *fn = wave_ring_sweeps_f64_symbol(m, n);
-        if (!fn || groups < 1 || args.members < 1 || args.count < 1 || !args.mbox)
+        if (!fn || groups < 1 || args.members < 1 || args.count < 1 || !args.mbox || !args.in_a || !args.out_u ||
+            !args.out_s || !args.out_v)
             return hipErrorInvalidValue;
         const unsigned int grid = static_cast<unsigned int>(groups) * static_cast<unsigned int>(args.members);
         void *params[] = {const_cast<WaveGroupArgs *>(&args)};
