@@ -1243,7 +1243,8 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                                        &b.grid_qr));
                 JSVD_CHECK(resident_of(jsvd::precond_finish_symbol(b.max_m), jsvd::precond_finish_block(), b.lds_fin,
                                        &b.grid_fin));
-                JSVD_CHECK(resident_of(jsvd::polar_bq_symbol(b.max_n), 256, 0, &b.grid_polar));
+                JSVD_CHECK(resident_of(jsvd::polar_bq_symbol(b.max_n), 256, jsvd::polar_bq_lds_bytes(b.max_n),
+                                       &b.grid_polar));
             }
         }
         if (b.cls >= 0 && jsvd::class_uses_vlog(b.cls))
@@ -1460,11 +1461,8 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
             JSVD_HIP(jsvd::launch_mixed_sweeps_f32(bk.f32cls, bk.grid32, bk.lds32, margs, st));
             if (pre)
             {
-                // Q by two Newton-Schulz steps and B = X Q, on MFMA; E and the
-                // intermediate Q live in the B and Ux buffers meanwhile (B is
-                // written last, Ux only by the refinement).
-                JSVD_HIP(jsvd::launch_polar_bq_f64(bk.max_n, bk.grid_polar, pargs, margs, p->pc_b.ptr, p->pc_ux.ptr,
-                                                   st));
+                // Q = polar factor of V32 and B = X Q, on MFMA.
+                JSVD_HIP(jsvd::launch_polar_bq_f64(bk.max_n, bk.grid_polar, pargs, margs, st));
             }
             else
             {

@@ -134,12 +134,12 @@ namespace jsvd
     const void *precond_finish_symbol(int64_t m); // kernel instance for a bucket's largest m
     int precond_finish_block();
     hipError_t launch_qrcp_f64(int variant, int grid, int lds_bytes, const PrecondArgs &args, hipStream_t stream);
-    // Q = orthonormal factor of V32 (two Newton-Schulz steps) and B = X Q on
-    // FP64 MFMA for a preconditioned bucket (ma: its square entries, q, u =
-    // B buffer, counters 2-3); tmp_e / tmp_q: n x n scratch per matrix.
+    // Q = orthonormal factor of V32 (polar factor by a polynomial in
+    // Q^T Q - I) and B = X Q on FP64 MFMA for a preconditioned bucket (ma:
+    // its square entries, q, u = B buffer, counters 2-3).
     const void *polar_bq_symbol(int64_t n); // instance for a bucket's largest n (<= 128)
-    hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, double *tmp_e,
-                                   double *tmp_q, hipStream_t stream);
+    int polar_bq_lds_bytes(int64_t n);
+    hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, hipStream_t stream);
     hipError_t launch_precond_finish_f64(int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
                                          hipStream_t stream);
 

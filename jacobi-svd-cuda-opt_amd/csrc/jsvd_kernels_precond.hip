@@ -360,112 +360,74 @@ namespace jsvd
 
         typedef double f64x4 __attribute__((ext_vector_type(4)));
 
-        // C = A B for n x n operands (n <= NT, NT = 64 or 128) on FP64 MFMA
-        // (v_mfma_f64_16x16x4f64) by one workgroup of four waves: wave w
-        // computes the (NT/2) x (NT/2) quadrant (w / 2, w % 2) as T x T tiles
-        // of 16 x 16 (T = NT / 32).  K runs in panels of 16: A's NT x 16 and
-        // B's 16 x NT panels are staged through LDS (k-major, rows padded to
-        // NT + 8 doubles so the four k rows of an MFMA operand read start on
-        // different banks), so every operand is read from L2 once per
-        // workgroup.  a(i, k) / b(k, j) give the operands; KFAST says that
-        // consecutive k are contiguous in memory (the staging loop then walks
-        // k fastest).  Values outside n x n are zero (indices clamped, values
-        // selected, so no load sits under a branch).  Lane l supplies
-        // A[i0 + l%16][k + l/16] and B[k + l/16][j0 + l%16] and receives
-        // C[i0 + l/16 + 4r][j0 + l%16], r = 0..3.
-        template <int NT, bool A_KFAST, bool B_KFAST, typename FA, typename FB, typename FOUT>
-        __device__ __forceinline__ void lds_gemm(FA a, FB b, FOUT out, int n, double *As, double *Bs)
+        // One n x n product C = A B (n <= NT) on FP64 MFMA
+        // (v_mfma_f64_16x16x4f64) by one workgroup of four waves: wave w owns
+        // the (NT/2) x (NT/2) quadrant (w / 2, w % 2) of C as T x T tiles of
+        // 16 x 16 (T = NT / 32), accumulated in `acc`.  K runs in blocks of
+        // 16; in step s of a block, lane l supplies A[i0 + l%16][k_l] and
+        // B[k_l][j0 + l%16] with k_l = k0 + 4 (l / 16) + s (any assignment of
+        // the block's k to the four lane groups sums the same products, and
+        // this one gives each lane four consecutive k, so row-major A and
+        // column-major B operands are read as 32 contiguous bytes).  a(i, k)
+        // and b(k, j) return the operands (zero outside n x n); lane l then
+        // holds C[i0 + l/16 + 4 r][j0 + l%16] in acc[x][y][r].
+        template <int NT, typename FA, typename FB>
+        __device__ __forceinline__ void mfma_nn(f64x4 (&acc)[NT / 32][NT / 32], FA a, FB b, int n, int wr, int wc,
+                                                int lr, int lk)
         {
-            constexpr int T = NT / 32, LD = NT + 8, PANEL = 16;
-            const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-            const int lr = lane & 15, lk = lane >> 4;
-            const int wr = (wave >> 1) * (NT / 2), wc = (wave & 1) * (NT / 2);
-            f64x4 acc[T][T];
+            constexpr int T = NT / 32;
 #pragma unroll
             for (int x = 0; x < T; ++x)
 #pragma unroll
                 for (int y = 0; y < T; ++y)
                     acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
-            for (int k0 = 0; k0 < n; k0 += PANEL)
+            for (int k0 = 0; k0 < n; k0 += 16)
             {
+                double av[T][4], bv[T][4];
 #pragma unroll
-                for (int e = tid; e < NT * PANEL; e += 256)
-                {
-                    int i, kk;
-                    if constexpr (A_KFAST)
-                    {
-                        kk = e % PANEL;
-                        i = e / PANEL;
-                    }
-                    else
-                    {
-                        i = e % NT;
-                        kk = e / NT;
-                    }
-                    const int k = k0 + kk;
-                    const double v = a(min(i, n - 1), min(k, n - 1));
-                    As[kk * LD + i] = (i < n && k < n) ? v : 0.0;
-                    int j;
-                    if constexpr (B_KFAST)
-                    {
-                        kk = e % PANEL;
-                        j = e / PANEL;
-                    }
-                    else
-                    {
-                        j = e % NT;
-                        kk = e / NT;
-                    }
-                    const int kb = k0 + kk;
-                    const double u = b(min(kb, n - 1), min(j, n - 1));
-                    Bs[kk * LD + j] = (j < n && kb < n) ? u : 0.0;
-                }
-                __syncthreads();
+                for (int t = 0; t < T; ++t)
 #pragma unroll
-                for (int kk = 0; kk < PANEL; kk += 4)
-                {
-                    double af[T], bf[T];
-#pragma unroll
-                    for (int x = 0; x < T; ++x)
+                    for (int s = 0; s < 4; ++s)
                     {
-                        af[x] = As[(kk + lk) * LD + wr + 16 * x + lr];
-                        bf[x] = Bs[(kk + lk) * LD + wc + 16 * x + lr];
+                        const int k = k0 + 4 * lk + s;
+                        av[t][s] = a(wr + 16 * t + lr, k);
+                        bv[t][s] = b(k, wc + 16 * t + lr);
                     }
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
 #pragma unroll
                     for (int x = 0; x < T; ++x)
 #pragma unroll
                         for (int y = 0; y < T; ++y)
-                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
-                }
-                __syncthreads();
+                            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x][s], bv[y][s], acc[x][y], 0, 0, 0);
             }
-#pragma unroll
-            for (int x = 0; x < T; ++x)
-#pragma unroll
-                for (int y = 0; y < T; ++y)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                    {
-                        const int i = wr + 16 * x + lk + 4 * r, j = wc + 16 * y + lr;
-                        if (i < n && j < n)
-                            out(i, j, acc[x][y][r]);
-                    }
         }
 
-        // Q <- orthonormal factor of V32 by two Newton-Schulz steps
-        // Q <- Q - Q (Q^T Q - I) / 2 (V32 is orthogonal to about 1e-6, so the
-        // error goes 1e-6 -> 1e-12 -> rounding), then B = X Q; every product
-        // on FP64 MFMA, one workgroup of four waves per matrix, operands from
-        // L2: Q (column-major) at q, X (row-major) at x, E = Q^T Q - I in the
-        // B buffer and the intermediate Q in the Ux buffer (both free until
-        // the refinement).  Replaces the Gram-Schmidt and B = A Q stages for
-        // the preconditioned buckets.
+        // Q <- orthonormal factor of V32 (column-major n x n at q), then
+        // B = X Q (X row-major at x, B column-major at b); one workgroup of
+        // four waves per matrix, every product on FP64 MFMA.  With
+        // E = Q^T Q - I (V32 is orthogonal to about 1e-6),
+        //   Q (Q^T Q)^(-1/2) = Q (I - E/2 + 3 E^2 / 8 - ...),
+        // whose truncation after E^2 leaves about (5/16) |E|^3 in Q^T Q - I:
+        // one pass forms E, F = E E and Q <- Q M with M = I - E/2 + 3 F / 8
+        // (three products instead of the four of two Newton-Schulz steps),
+        // and a second pass runs only when max |E| > 2^-14 (truncation then
+        // above about 7e-14; the FP32 sweeps leave |E| of 1e-6 .. 1e-4).  The
+        // NT x (NT + 1) LDS buffer (odd stride: rows on distinct banks) holds
+        // E, then M, then the new Q, each the B operand of the next product;
+        // Q and X are read from L2.
         template <int NT>
-        __global__ __launch_bounds__(256) void polar_bq_f64(PrecondArgs pa, MixedArgs ma, double *tmp_e,
-                                                          double *tmp_q)
+        __global__ __launch_bounds__(256) void polar_bq_f64(PrecondArgs pa, MixedArgs ma)
         {
-            __shared__ double As[16 * (NT + 8)], Bs[16 * (NT + 8)];
+            constexpr int T = NT / 32, LD = NT + 1;
+            extern __shared__ double2 polar_raw[];
+            double *S = reinterpret_cast<double *>(polar_raw);
+            __shared__ double s_max[4];
             __shared__ int s_item;
+            const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+            const int lr = lane & 15, lk = lane >> 4;
+            const int wr = (wave >> 1) * (NT / 2), wc = (wave & 1) * (NT / 2);
+            f64x4 acc[T][T];
             for (;;)
             {
                 claim(&ma.counters[2], &s_item);
@@ -475,31 +437,101 @@ namespace jsvd
                 const WorkItem w = ma.items[item]; // square entry: m = n, a_off = v_off
                 const int n = w.n;
                 double *q = ma.q + w.v_off;
-                double *e = tmp_e + w.v_off;
-                double *q1 = tmp_q + w.v_off;
                 const double *x = pa.x + w.v_off;
                 double *b = ma.u + w.a_off;
-                for (int step = 0; step < 2; ++step)
+                // Column-major Q: Q[r][c] = q[c n + r].
+                auto qt = [=](int i, int k) { return (i < n && k < n) ? q[i * n + k] : 0.0; }; // (Q^T)[i][k]
+                auto qc = [=](int k, int j) { return (k < n && j < n) ? q[j * n + k] : 0.0; }; // Q[k][j]
+                auto qr = [=](int i, int k) { return (i < n && k < n) ? q[k * n + i] : 0.0; }; // Q[i][k]
+                auto sm = [=](int k, int j) { return S[k * LD + j]; };                        // LDS operand
+                for (int pass = 0; pass < 2; ++pass)
                 {
-                    const double *qs = step == 0 ? q : q1;
-                    double *qd = step == 0 ? q1 : q;
-                    // E = Q^T Q - I (all operands column-major n x n).
-                    lds_gemm<NT, true, true>([=](int i, int k) { return qs[i * n + k]; },
-                                             [=](int k, int j) { return qs[j * n + k]; },
-                                             [=](int i, int j, double c) { e[j * n + i] = i == j ? c - 1.0 : c; },
-                                             n, As, Bs);
+                    // E = Q^T Q - I into LDS (zero outside n x n), and max |E|.
+                    mfma_nn<NT>(acc, qt, qc, n, wr, wc, lr, lk);
+                    double emax = 0.0;
+#pragma unroll
+                    for (int xx = 0; xx < T; ++xx)
+#pragma unroll
+                        for (int yy = 0; yy < T; ++yy)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                            {
+                                const int i = wr + 16 * xx + lk + 4 * r, j = wc + 16 * yy + lr;
+                                const double e = (i < n && j < n) ? acc[xx][yy][r] - (i == j ? 1.0 : 0.0) : 0.0;
+                                S[i * LD + j] = e;
+                                emax = fmax(emax, fabs(e));
+                            }
+                    for (int o = 32; o > 0; o >>= 1)
+                        emax = fmax(emax, __shfl_xor(emax, o, 64));
+                    if (lane == 0)
+                        s_max[wave] = emax;
                     __syncthreads();
-                    // Q' = Q - (Q E) / 2.
-                    lds_gemm<NT, false, true>(
-                        [=](int i, int k) { return qs[k * n + i]; }, [=](int k, int j) { return e[j * n + k]; },
-                        [=](int i, int j, double c) { qd[j * n + i] = fma(-0.5, c, qs[j * n + i]); }, n, As, Bs);
+                    const double e_all = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+                    // F = E E (both operands from LDS), then this wave's part of
+                    // M = I - E/2 + 3 F / 8 in registers.
+                    mfma_nn<NT>(acc, sm, sm, n, wr, wc, lr, lk);
+#pragma unroll
+                    for (int xx = 0; xx < T; ++xx)
+#pragma unroll
+                        for (int yy = 0; yy < T; ++yy)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                            {
+                                const int i = wr + 16 * xx + lk + 4 * r, j = wc + 16 * yy + lr;
+                                const double e = S[i * LD + j];
+                                acc[xx][yy][r] = fma(0.375, acc[xx][yy][r], fma(-0.5, e, i == j ? 1.0 : 0.0));
+                            }
+                    __syncthreads(); // every wave has read E
+#pragma unroll
+                    for (int xx = 0; xx < T; ++xx)
+#pragma unroll
+                        for (int yy = 0; yy < T; ++yy)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                            {
+                                const int i = wr + 16 * xx + lk + 4 * r, j = wc + 16 * yy + lr;
+                                S[i * LD + j] = (i < n && j < n) ? acc[xx][yy][r] : 0.0;
+                            }
                     __syncthreads();
+                    // Q <- Q M; the new Q goes to global memory (the refinement's
+                    // starting V) and to LDS (the operand of B = X Q).
+                    mfma_nn<NT>(acc, qr, sm, n, wr, wc, lr, lk);
+                    __syncthreads(); // every wave has read Q and M
+#pragma unroll
+                    for (int xx = 0; xx < T; ++xx)
+#pragma unroll
+                        for (int yy = 0; yy < T; ++yy)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                            {
+                                const int i = wr + 16 * xx + lk + 4 * r, j = wc + 16 * yy + lr;
+                                const bool in = i < n && j < n;
+                                S[i * LD + j] = in ? acc[xx][yy][r] : 0.0;
+                                if (in)
+                                    q[j * n + i] = acc[xx][yy][r];
+                            }
+                    // The second pass reads the new Q from global memory: its
+                    // stores must be visible to the whole workgroup.
+                    __threadfence_block();
+                    __syncthreads();
+                    if (!(e_all > 0x1p-14))
+                        break;
                 }
-                // B = X Q (X row-major, B column-major into the B buffer).
-                lds_gemm<NT, true, true>([=](int i, int k) { return x[i * n + k]; },
-                                         [=](int k, int j) { return q[j * n + k]; },
-                                         [=](int i, int j, double c) { b[j * n + i] = c; }, n, As, Bs);
-                __syncthreads();
+                // B = X Q (X row-major, B column-major).
+                mfma_nn<NT>(acc, [=](int i, int k) { return (i < n && k < n) ? x[i * n + k] : 0.0; }, sm, n, wr, wc,
+                            lr, lk);
+#pragma unroll
+                for (int xx = 0; xx < T; ++xx)
+#pragma unroll
+                    for (int yy = 0; yy < T; ++yy)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                        {
+                            const int i = wr + 16 * xx + lk + 4 * r, j = wc + 16 * yy + lr;
+                            if (i < n && j < n)
+                                b[j * n + i] = acc[xx][yy][r];
+                        }
+                __syncthreads(); // LDS free for the next matrix
             }
             if (threadIdx.x == 0)
             {
@@ -602,13 +634,18 @@ namespace jsvd
                        : reinterpret_cast<const void *>(&polar_bq_f64<128>);
     }
 
-    hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, double *tmp_e,
-                                   double *tmp_q, hipStream_t stream)
+    int polar_bq_lds_bytes(int64_t n)
+    {
+        const int nt = n <= 64 ? 64 : 128;
+        return nt * (nt + 1) * 8;
+    }
+
+    hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, hipStream_t stream)
     {
         if (n <= 64)
-            polar_bq_f64<64><<<grid, 256, 0, stream>>>(pa, ma, tmp_e, tmp_q);
+            polar_bq_f64<64><<<grid, 256, polar_bq_lds_bytes(n), stream>>>(pa, ma);
         else
-            polar_bq_f64<128><<<grid, 256, 0, stream>>>(pa, ma, tmp_e, tmp_q);
+            polar_bq_f64<128><<<grid, 256, polar_bq_lds_bytes(n), stream>>>(pa, ma);
         return hipGetLastError();
     }
 
