@@ -1,9 +1,11 @@
 """GPU tests on the stress mix (jsvd.gen.stress_mix): the reference
 generator's scenario kinds (dense uniform / normal / log-uniform,
 diagonal-dominant, low-rank, ill-conditioned, sparse, zero columns) through
-every solver path -- LDS work-list classes, the grouped wave kernel (a lone
-300 x 150 runs four-wave workgroups with rows in registers) and the blocked
-path -- against the oracle.
+every solver path -- LDS work-list classes, the ring form of the wave
+kernels (a lone 300 x 150 runs eight-wave workgroups with its column pairs
+in registers), the grouped barrier form (JSVD_WAVE_RING=0: four-wave
+workgroups with rows in registers) and the blocked path -- against the
+oracle.
 
 Bar: sigma sorted within 1e-12 of the oracle's largest singular value;
 residual ||A - U S V^T|| / ||A|| <= 1e-12; ||V^T V - I||_F <= 1e-11; U
@@ -54,10 +56,11 @@ def test_stress_mix_lds_classes():
         _check(a, g, r, kind)
 
 
-@pytest.mark.parametrize("bw", [0, 16])
-def test_stress_mix_large_paths(bw):
+@pytest.mark.parametrize("bw,ring", [(0, "1"), (0, "0"), (16, "1")], ids=["ring", "grouped-barrier", "blocked16"])
+def test_stress_mix_large_paths(bw, ring, monkeypatch):
+    monkeypatch.setenv("JSVD_WAVE_RING", ring)
     mats, kinds = gen.stress_mix(8, [(300, 150)], seed=91)
-    _note(f"stress mix 300x150, block_width={bw}")
+    _note(f"stress mix 300x150, block_width={bw}, ring={ring}")
     refs = oracle.solve_batch(mats, EPS, SWEEPS)
     for a, r, kind in zip(mats, refs, kinds):
         g = jsvd.svd(a, epsilon=EPS, max_sweeps=SWEEPS, block_width=bw)
