@@ -305,10 +305,12 @@ namespace
 // orthonormalisation, B = A Q, FP64 sweeps, then the QR preconditioner's two
 // kernels); plain buckets use the FP64 pair.
 constexpr int kCountersPerBucket = 12;
-// Mixed-precision buckets are split into up to kMixedSplitParts parts of at
-// least kMixedSplitMin matrices, each on a stream of its own.
+// Mixed-precision buckets of matrices wider than kMixedSplitMinColumns are
+// split into up to kMixedSplitParts parts of at least kMixedSplitMin
+// matrices, each on a stream of its own.
 constexpr size_t kMixedSplitMin = 64;
 constexpr size_t kMixedSplitParts = 4;
+constexpr int64_t kMixedSplitMinColumns = 64;
 
 // A bucket is one persistent launch of the LDS-resident work-list kernel:
 // matrices of one shape class whose LDS needs allow the same number of
@@ -1123,19 +1125,26 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     // Longest-processing-time-first inside each bucket (a sweep costs about
     // n^2 (m + n)), and the buckets with the most work are launched first.
     auto cost = [](const jsvd::WorkItem &x) { return double(x.n) * x.n * (x.m + x.n); };
-    // A mixed-precision bucket is four dependent launches (FP32 sweeps,
-    // Gram-Schmidt, B = AQ, FP64 refinement), and each ends in a tail where
-    // few workgroups still run.  A large one is split into parts (items of
-    // its longest-first order dealt out in turn) that the stream assignment
-    // below puts on streams of their own: one part's stages then fill the
-    // others' tails.  Each matrix's arithmetic does not depend on its
-    // bucket.
+    // A mixed-precision bucket is a chain of dependent launches (QR, FP32
+    // sweeps, orthonormal factor and B, FP64 refinement, back-transform), and
+    // each ends in a tail where few workgroups still run.  A large bucket of
+    // wide matrices is split into parts (items of its longest-first order
+    // dealt out in turn) that the stream assignment below puts on streams of
+    // their own: one part's stages then fill the others' tails (1024 x
+    // 128x128: 11.66 -> 11.46 ms).  Buckets of at most 64 columns stay
+    // whole: their stages are short, and four chains of five launches on
+    // each stream cost more than the tails (uneven 4096 batch: 4.12 ms
+    // split, 2.98 ms whole).  Each matrix's arithmetic does not depend on
+    // its bucket.
     {
         const size_t initial = groups.size();
         for (size_t i = 0; i < initial; ++i)
         {
+            int64_t widest = 0;
+            for (const jsvd::WorkItem &w : groups[i].second)
+                widest = std::max<int64_t>(widest, w.n);
             const size_t parts = std::min(kMixedSplitParts, groups[i].second.size() / kMixedSplitMin);
-            if (std::get<2>(groups[i].first) < 0 || parts < 2)
+            if (std::get<2>(groups[i].first) < 0 || parts < 2 || widest <= kMixedSplitMinColumns)
                 continue;
             std::vector<jsvd::WorkItem> items;
             items.swap(groups[i].second);
