@@ -396,6 +396,7 @@ struct jsvd_plan
         int waves = 8;   // waves per workgroup of the grouped form
         bool ring = false;      // ring form (wave_ring_sweeps_f64)
         int64_t mbox_group = 0; // ring form: mailbox words per group
+        int xcd_spread = 0;     // ring form: groups placed one XCD each (0: group-major grid)
     };
     std::vector<int64_t> wave;                 // batch index per entry
     std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry (barrier forms)
@@ -406,6 +407,7 @@ struct jsvd_plan
     DevBuf<int32_t> wv_flags;
     DevBuf<unsigned int> wv_bars;
     DevBuf<unsigned long long> wv_mbox; // ring-form column mailboxes
+    DevBuf<unsigned int> wv_xcc;        // ring form: XCC id of every member
     // Blocked Jacobi (cfg.block_width 16 or 32): per-round partial Grams,
     // rotations Q and flags, sized for the largest blocked matrix; the sweep
     // flag is read back through pinned host memory once per sweep.
@@ -637,6 +639,17 @@ namespace
         return !(env && env[0] == '0');
     }
 
+    // MI355X: 8 XCDs; the dispatcher hands workgroup b of a launch to XCD
+    // b % 8.  The ring form places each group on one XCD (its kernel checks
+    // the placement and falls back to device-scope mailbox stores when a
+    // group is spread); JSVD_WAVE_RING_XCD=0 keeps the group-major grid.
+    constexpr int kXcds = 8;
+    bool xcd_ring_enabled()
+    {
+        const char *env = std::getenv("JSVD_WAVE_RING_XCD");
+        return !(env && env[0] == '0');
+    }
+
     bool wave_block_mode(int64_t count, int64_t max_half)
     {
         return max_half <= 256 && count > 4 + max_half * max_half / 200;
@@ -702,6 +715,9 @@ namespace
                 args.out_u = d_u;
                 args.out_s = d_sigma;
                 args.out_v = d_v;
+                args.ring_groups = l.groups;
+                args.xcd_spread = l.xcd_spread;
+                args.xcc = p.wv_xcc.ptr;
                 JSVD_HIP(jsvd::launch_wave_ring_sweeps_f64(l.m_max, l.n_max, l.groups, args, stream));
             }
             else
@@ -758,7 +774,7 @@ namespace
         const int64_t entries = static_cast<int64_t>(p.wave.size());
         for (int64_t e = 0; e < entries; ++e)
             p.wave_bytes += 8.0 * double(p.m[p.wave[e]]) * double(p.n[p.wave[e]]);
-        int64_t max_flags = 1, max_mbox = 0;
+        int64_t max_flags = 1, max_mbox = 0, max_xcc = 0;
         for (int64_t e = 0; e < entries;)
         {
             jsvd_plan::WaveLaunch l;
@@ -793,6 +809,17 @@ namespace
                     l.waves = waves;
                     l.members = static_cast<int>(members);
                     l.groups = static_cast<int>(std::min<int64_t>(l.count, resident / members));
+                    // One XCD per group when the GPU's CUs split into kXcds
+                    // equal XCDs and every XCD has room for its share of the
+                    // groups: mailbox words then meet in that XCD's L2.
+                    const int64_t per_xcd = cus % kXcds == 0 ? int64_t(occupancy) * (cus / kXcds) : 0;
+                    if (xcd_ring_enabled() && members <= per_xcd)
+                    {
+                        const int64_t g8 = std::min<int64_t>(l.count, (per_xcd / members) * kXcds);
+                        l.groups = static_cast<int>(g8);
+                        l.xcd_spread = kXcds;
+                    }
+                    max_xcc = std::max<int64_t>(max_xcc, int64_t(l.groups) * l.members);
                     l.mbox_group = jsvd::ring_mailbox_words(l.m_max, l.n_max, l.members);
                     max_flags = std::max<int64_t>(max_flags, int64_t(l.groups) * l.members);
                     max_mbox = std::max<int64_t>(max_mbox, int64_t(l.groups) * l.mbox_group);
@@ -857,6 +884,7 @@ namespace
         JSVD_CHECK(p.wv_flags.alloc(entries * p.cfg.max_sweeps));
         JSVD_CHECK(p.wv_bars.alloc(max_flags));
         JSVD_CHECK(p.wv_mbox.alloc(max_mbox));
+        JSVD_CHECK(p.wv_xcc.alloc(max_xcc));
         std::vector<jsvd::WaveMatrix> mats(entries);
         for (int64_t e = 0; e < entries; ++e)
         {
@@ -898,7 +926,7 @@ namespace
         std::vector<int64_t> m, n;
         jsvd_config cfg{};
         std::vector<int> tune; // the shape-class table the plan was built with
-        bool ring = true;      // whether the ring form of the wave solves was enabled
+        int ring = 3;          // ring form of the wave solves enabled (bit 0), its XCD placement (bit 1)
         jsvd_plan *plan = nullptr;
         uint64_t last_use = 0;
     };
@@ -1755,7 +1783,7 @@ jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_
     if (cfg)
         c = *cfg;
     const std::vector<int> tune = tune_table();
-    const bool ring = ring_enabled();
+    const int ring = (ring_enabled() ? 1 : 0) | (xcd_ring_enabled() ? 2 : 0);
     jsvd_plan *plan = nullptr;
     for (CachedPlan &e : ctx->plans)
         if (e.m == mm && e.n == nn && same_config(e.cfg, c) && e.tune == tune && e.ring == ring)

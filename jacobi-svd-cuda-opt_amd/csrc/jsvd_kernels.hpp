@@ -214,6 +214,14 @@ namespace jsvd
         int64_t mbox_group;
         const double *in_a;
         double *out_u, *out_s, *out_v;
+        // Ring form: number of groups; with xcd_spread = X > 0 the grid holds
+        // X * ceil(groups / X) * members workgroups, workgroup b runs on XCD
+        // b % X (the dispatcher deals workgroups out to the XCDs in turn),
+        // group q's members sit on XCD q % X and slots without a group exit
+        // at once; xcc gets one word per member (its XCC id).
+        int ring_groups;
+        int xcd_spread;
+        unsigned int *xcc;
     };
     int wave_sweeps_max_rows();
     // Ring form of the grouped solves (one wave per column pair, every

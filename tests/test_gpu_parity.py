@@ -219,6 +219,28 @@ def test_ring_form_equals_barrier_forms(barrier_forms):
     _assert_parity(a, jsvd.svd(a, epsilon=1e-14, max_sweeps=30)[:4], oracle.solve(a, 1e-14, 30), 1e-14, "300x151")
 
 
+def test_ring_xcd_placement_is_bitwise_neutral(monkeypatch):
+    # The ring form places each group's workgroups on one XCD and, after
+    # checking the placement, stores the mailbox words at workgroup scope
+    # (they meet the readers in that XCD's L2).  Only the grid and the store
+    # scope change, so the results must equal the group-major grid's
+    # bitwise: 16 of 256 x 256 (two groups per XCD), a lone 300 x 151 and 24
+    # tall ones (four-wave groups).
+    _note("ring form, XCD placement vs group-major grid")
+    rng = np.random.default_rng(777)
+    batches = [[rng.standard_normal((256, 256)) for _ in range(16)], [rng.standard_normal((300, 151))],
+               [rng.standard_normal((700, 90)) for _ in range(24)]]
+    for mats in batches:
+        monkeypatch.setenv("JSVD_WAVE_RING_XCD", "1")
+        spread = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
+        monkeypatch.setenv("JSVD_WAVE_RING_XCD", "0")
+        major = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
+        for k, (x, y) in enumerate(zip(spread, major)):
+            for a, b in zip(x[:3], y[:3]):
+                np.testing.assert_array_equal(a, b, err_msg=f"xcd vs group-major {mats[k].shape} #{k}")
+            assert x[3] == y[3] and x[4] == y[4] == jsvd.STOP_CONVERGED
+
+
 def test_tall_single_streamed_held_rows():
     # A lone plain-path matrix of more than 1024 rows and at most 512 columns
     # runs four-wave groups with its rows streamed; each pair keeps its first
