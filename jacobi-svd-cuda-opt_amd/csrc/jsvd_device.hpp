@@ -168,6 +168,40 @@ namespace jsvd::dev
                                     (static_cast<long long>(static_cast<unsigned int>(lo))));
     }
 
+    // x + x[lane ^ 32] and x + x[lane ^ 16] for doubles without an LDS
+    // permute: gfx950's v_permlane32_swap (v_permlane16_swap) exchanges the
+    // upper half (odd rows of 16) of its first operand with the lower half
+    // (even rows) of its second, so with both operands a copy of x the two
+    // results hold x and its partner in every lane, and their sum is the
+    // butterfly sum (the same value as x + __shfl_xor(x, 32 or 16): addition
+    // commutes).
+    __device__ __forceinline__ double xor32_sum(double x)
+    {
+        const long long b = __double_as_longlong(x);
+        const auto lo = __builtin_amdgcn_permlane32_swap(static_cast<unsigned int>(b), static_cast<unsigned int>(b),
+                                                         false, false);
+        const auto hi = __builtin_amdgcn_permlane32_swap(static_cast<unsigned int>(b >> 32),
+                                                         static_cast<unsigned int>(b >> 32), false, false);
+        const double a = __longlong_as_double(static_cast<long long>(static_cast<unsigned int>(lo[0])) |
+                                              (static_cast<long long>(hi[0]) << 32));
+        const double c = __longlong_as_double(static_cast<long long>(static_cast<unsigned int>(lo[1])) |
+                                              (static_cast<long long>(hi[1]) << 32));
+        return a + c;
+    }
+    __device__ __forceinline__ double xor16_sum(double x)
+    {
+        const long long b = __double_as_longlong(x);
+        const auto lo = __builtin_amdgcn_permlane16_swap(static_cast<unsigned int>(b), static_cast<unsigned int>(b),
+                                                         false, false);
+        const auto hi = __builtin_amdgcn_permlane16_swap(static_cast<unsigned int>(b >> 32),
+                                                         static_cast<unsigned int>(b >> 32), false, false);
+        const double a = __longlong_as_double(static_cast<long long>(static_cast<unsigned int>(lo[0])) |
+                                              (static_cast<long long>(hi[0]) << 32));
+        const double c = __longlong_as_double(static_cast<long long>(static_cast<unsigned int>(lo[1])) |
+                                              (static_cast<long long>(hi[1]) << 32));
+        return a + c;
+    }
+
     // All-reduce over aligned groups of G lanes (compile-time G <= 64).  The
     // first four levels are DPP moves (no LDS round trip): after the xor-1 and
     // xor-2 levels every lane of a quad holds the quad sum, so the mirror
@@ -185,9 +219,9 @@ namespace jsvd::dev
         if constexpr (G >= 16)
             x += dpp_f64<0x140>(x);
         if constexpr (G >= 32)
-            x += __shfl_xor(x, 16, 64);
+            x = xor16_sum(x);
         if constexpr (G >= 64)
-            x += __shfl_xor(x, 32, 64);
+            x = xor32_sum(x);
         return x;
     }
 
@@ -253,15 +287,15 @@ namespace jsvd::dev
         }
         if constexpr (G >= 32)
         {
-            a += __shfl_xor(a, 16, 64);
-            b += __shfl_xor(b, 16, 64);
-            c += __shfl_xor(c, 16, 64);
+            a = xor16_sum(a);
+            b = xor16_sum(b);
+            c = xor16_sum(c);
         }
         if constexpr (G >= 64)
         {
-            a += __shfl_xor(a, 32, 64);
-            b += __shfl_xor(b, 32, 64);
-            c += __shfl_xor(c, 32, 64);
+            a = xor32_sum(a);
+            b = xor32_sum(b);
+            c = xor32_sum(c);
         }
     }
 
@@ -270,6 +304,18 @@ namespace jsvd::dev
     {
 #pragma unroll
         for (int mask = 32; mask > 0; mask >>= 1)
+            x += __shfl_xor(x, mask, 64);
+        return x;
+    }
+
+    // wave_sum for doubles with the two cross-row levels on permlane swaps.
+    template <>
+    __device__ __forceinline__ double wave_sum<double>(double x)
+    {
+        x = xor32_sum(x);
+        x = xor16_sum(x);
+#pragma unroll
+        for (int mask = 8; mask > 0; mask >>= 1)
             x += __shfl_xor(x, mask, 64);
         return x;
     }
