@@ -336,11 +336,12 @@ struct JsvdBucket
     int f32cls = -1;
     int lds32 = 0, grid32 = 0;
     int lds_ortho = 0, grid_ortho = 0, grid_gemm = 0;
-    // Mixed precision on X = R^T of a pivoted QR (the stages above and the
-    // FP64 refinement then run on n x n matrices): the QR and back-transform
-    // launch shapes.
+    // Mixed precision on X = R2^T of two QRs, A = Q1 R and R^T = Q2 R2 (the
+    // stages above and the FP64 refinement then run on n x n matrices): the
+    // launch shapes of both QRs and of the back-transform.
     bool precond = false;
     int qr_variant = -1, lds_qr = 0, grid_qr = 0, lds_fin = 0, grid_fin = 0, grid_polar = 0;
+    int qr2_variant = -1, lds_qr2 = 0, grid_qr2 = 0;
 };
 
 struct jsvd_plan
@@ -361,7 +362,7 @@ struct jsvd_plan
     // scratch of the preconditioner.
     std::vector<jsvd::WorkItem> pre_items;
     DevBuf<jsvd::WorkItem> d_pre_items;
-    DevBuf<double> pc_refl, pc_tau, pc_x, pc_b, pc_ux, pc_vx;
+    DevBuf<double> pc_refl, pc_tau, pc_x, pc_b, pc_ux, pc_vx, pc_refl2, pc_tau2, pc_t1, pc_t2;
     DevBuf<int32_t> pc_perm;
     // Every matrix in batch order (the adjoint kernel's work-list) and the
     // largest LDS need of the adjoint in FP64 / FP32.
@@ -1082,9 +1083,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                     slots = sl;
                     break;
                 }
-            // Mixed precision prefers the QR preconditioner (pivoted QR,
-            // then the FP32 sweeps, orthonormalisation, B = X Q and the FP64
-            // refinement on X = R^T, n x n); without it the stages run on A.
+            // Mixed precision prefers the QR preconditioner (QR of A and of
+            // R^T, then the FP32 sweeps, polar factor, B = X Q and the FP64
+            // refinement on X = R2^T, n x n); without it the stages run on A.
             // Either needs an FP32 class, a refinement class keeping V in
             // LDS and Q (n x n FP64) in LDS; other matrices of a mixed plan
             // take the plain FP64 path.
@@ -1200,7 +1201,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
         b.count = static_cast<int64_t>(g.second.size());
         for (const auto &w : g.second)
         {
-            // A preconditioned bucket's sweeps run on X = R^T (n x n).
+            // A preconditioned bucket's sweeps run on X = R2^T (n x n).
             const int64_t rows = b.precond ? w.n : w.m;
             const int64_t need = b.cls >= 0 ? jsvd::class_lds_bytes(b.cls, rows, w.n) : jsvd::lds_bytes_f64(w.m, w.n);
             b.lds_bytes = std::max<int>(b.lds_bytes, static_cast<int>(need));
@@ -1218,7 +1219,10 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             // Both preconditioner kernels size their rows for the bucket's
             // tallest matrix.
             b.lds_qr = static_cast<int>(jsvd::qrcp_lds_bytes(b.qr_variant));
-            b.lds_fin = static_cast<int>(jsvd::precond_finish_lds_bytes(b.max_m, b.max_n));
+            // The second QR factors the n x n R^T of every matrix.
+            b.qr2_variant = jsvd::qrcp_variant(b.max_n, b.max_n);
+            b.lds_qr2 = static_cast<int>(jsvd::qrcp_lds_bytes(b.qr2_variant));
+            b.lds_fin = static_cast<int>(jsvd::precond_finish_lds_bytes(b.qr_variant, b.max_m, b.max_n));
         }
         plan->buckets.push_back(b);
     }
@@ -1278,7 +1282,9 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
             {
                 JSVD_CHECK(resident_of(jsvd::qrcp_symbol(b.qr_variant), jsvd::qrcp_block(b.qr_variant), b.lds_qr,
                                        &b.grid_qr));
-                JSVD_CHECK(resident_of(jsvd::precond_finish_symbol(b.max_m), jsvd::precond_finish_block(), b.lds_fin,
+                JSVD_CHECK(resident_of(jsvd::qrcp_symbol(b.qr2_variant), jsvd::qrcp_block(b.qr2_variant), b.lds_qr2,
+                                       &b.grid_qr2));
+                JSVD_CHECK(resident_of(jsvd::precond_finish_symbol(b.qr_variant, b.max_m), jsvd::precond_finish_block(b.qr_variant), b.lds_fin,
                                        &b.grid_fin));
                 JSVD_CHECK(resident_of(jsvd::polar_bq_symbol(b.max_n), 256, jsvd::polar_bq_lds_bytes(b.max_n),
                                        &b.grid_polar));
@@ -1322,8 +1328,12 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                                sizeof(jsvd::WorkItem) * plan->pre_items.size(), hipMemcpyHostToDevice));
             JSVD_CHECK(plan->pc_refl.alloc(plan->a_elems));
             JSVD_CHECK(plan->pc_tau.alloc(plan->s_elems));
+            JSVD_CHECK(plan->pc_tau2.alloc(plan->s_elems));
+            // The blocked QR's panel factors T for Q1 and Q2.
+            JSVD_CHECK(plan->pc_t1.alloc(jsvd::qrcp_t_elems(plan->s_elems, plan->count)));
+            JSVD_CHECK(plan->pc_t2.alloc(jsvd::qrcp_t_elems(plan->s_elems, plan->count)));
             JSVD_CHECK(plan->pc_perm.alloc(plan->s_elems));
-            for (DevBuf<double> *buf : {&plan->pc_x, &plan->pc_b, &plan->pc_ux, &plan->pc_vx})
+            for (DevBuf<double> *buf : {&plan->pc_x, &plan->pc_b, &plan->pc_ux, &plan->pc_vx, &plan->pc_refl2})
                 JSVD_CHECK(buf->alloc(plan->v_elems));
         }
         const size_t streams = std::min<size_t>(plan->buckets.size(), 4);
@@ -1457,9 +1467,9 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
         const jsvd::WorkItem *items = p->d_items.ptr + bk.first;
         unsigned int *ctr = p->d_counters.ptr + kCountersPerBucket * b;
         const bool mixed = bk.f32cls >= 0;
-        // A preconditioned bucket: pivoted QR of each A, then every mixed
-        // stage on X = R^T (square entries, preconditioner scratch), then
-        // U = Q1 Vx, V = P Ux.
+        // A preconditioned bucket: QR of each A (A = Q1 R), QR of R^T in
+        // place (R^T = Q2 R2), then every mixed stage on X = R2^T (square
+        // entries, preconditioner scratch), then U = Q1 Ux, V = P Q2 Vx.
         const bool pre = bk.precond;
         const jsvd::WorkItem *sq_items = pre ? p->d_pre_items.ptr + bk.first : items;
         jsvd::PrecondArgs pargs{};
@@ -1472,6 +1482,10 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
             pargs.tau = p->pc_tau.ptr;
             pargs.perm = p->pc_perm.ptr;
             pargs.x = p->pc_x.ptr;
+            pargs.refl2 = p->pc_refl2.ptr;
+            pargs.tau2 = p->pc_tau2.ptr;
+            pargs.tq = p->pc_t1.ptr;
+            pargs.tq2 = p->pc_t2.ptr;
             pargs.ux = p->pc_ux.ptr;
             pargs.vx = p->pc_vx.ptr;
             pargs.sigma = d_sigma;
@@ -1480,6 +1494,16 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
             pargs.eps = p->cfg.epsilon;
             pargs.counters = ctr + 8;
             JSVD_HIP(jsvd::launch_qrcp_f64(bk.qr_variant, bk.grid_qr, bk.lds_qr, pargs, st));
+            // The second QR reads R^T from X and overwrites it with R2^T.
+            jsvd::PrecondArgs qargs = pargs;
+            qargs.items = sq_items;
+            qargs.a = p->pc_x.ptr;
+            qargs.refl = p->pc_refl2.ptr;
+            qargs.tau = p->pc_tau2.ptr;
+            qargs.tq = p->pc_t2.ptr;
+            qargs.perm = nullptr;
+            qargs.counters = ctr + 4; // the B = A Q pair, unused by preconditioned buckets
+            JSVD_HIP(jsvd::launch_qrcp_f64(bk.qr2_variant, bk.grid_qr2, bk.lds_qr2, qargs, st));
         }
         if (mixed)
         {
@@ -1532,13 +1556,13 @@ jsvd_status jsvd_plan_execute_status_f64(jsvd_plan *p, const double *d_a, double
                 args.q_init = p->q_scratch.ptr;
                 args.add_sweeps = 1;
                 args.max_sweeps = p->cfg.refine_sweeps > 0 ? p->cfg.refine_sweeps : p->cfg.max_sweeps;
-                // X's left vectors become A's V: every nonzero column is
-                // normalised (the U rule applies to A's U, in the back-transform).
+                // X's left vectors give A's U: every nonzero column is
+                // normalised here, the U rule applies in the back-transform.
                 args.u_all_nonzero = pre ? 1 : 0;
             }
             JSVD_HIP(jsvd::launch_lds_class_f64(bk.cls, bk.grid, bk.lds_bytes, args, st));
             if (pre)
-                JSVD_HIP(jsvd::launch_precond_finish_f64(bk.max_m, bk.grid_fin, bk.lds_fin, pargs, st));
+                JSVD_HIP(jsvd::launch_precond_finish_f64(bk.qr_variant, bk.max_m, bk.grid_fin, bk.lds_fin, pargs, st));
         }
         else
             JSVD_HIP(jsvd::launch_lds_batch_f64(bk.block, bk.grid, bk.lds_bytes, items, static_cast<int>(bk.count),

@@ -106,17 +106,27 @@ namespace jsvd
     int gemm_aq_block();
     hipError_t launch_gemm_aq_f64(int grid, const MixedArgs &args, hipStream_t stream);
 
-    // QR preconditioner of the mixed path: A P = Q1 R (pivoted Householder
-    // QR, FP64), Jacobi on X = R^T, then U = Q1 Vx and V = P Ux.
+    // QR preconditioner of the mixed path: A P = Q1 R and R^T = Q2 R2
+    // (Householder QR in FP64, twice), Jacobi on X = R2^T, then U = Q1 Ux
+    // and V = P Q2 Vx.  The same QR kernel runs both factorisations: on A
+    // (items with A's shapes) and in place on X = R^T (square items).
     struct PrecondArgs
     {
         const WorkItem *items;  // the batch's own shapes and offsets
         int count;
-        const double *a;        // packed row-major A (a_off)
-        double *refl;           // packed column-major m x n reflectors (a_off)
+        const double *a;        // packed row-major input of the QR (a_off)
+        double *refl;           // packed column-major m x n reflectors of Q1 (a_off)
         double *tau;            // n per matrix (s_off)
-        int32_t *perm;          // n per matrix (s_off): column k of A P is column perm[k] of A
-        double *x;              // packed row-major n x n X = R^T (v_off)
+        int32_t *perm;          // n per matrix (s_off): column k of A P is column perm[k] of A; null: not written
+        double *x;              // packed row-major n x n output R^T of the QR (v_off)
+        const double *refl2;    // packed column-major n x n reflectors of Q2 (v_off)
+        const double *tau2;     // n per matrix (s_off)
+        // Blocked QR: the 16 x 16 triangular factor T of each panel (Q1 =
+        // prod (I - V_p T_p V_p^T)), written by the QR at tq and read by the
+        // back-transform from tq (Q1) and tq2 (Q2); matrix k's panels start
+        // at 16 s_off + 240 k (index k), 256 doubles each.
+        double *tq;
+        const double *tq2;
         const double *ux;       // packed row-major n x n left vectors of X (v_off)
         const double *vx;       // packed row-major n x n right vectors of X (v_off)
         const double *sigma;    // sigma of X = sigma of A (s_off)
@@ -127,12 +137,14 @@ namespace jsvd
     };
     bool precond_fits(int64_t m, int64_t n);
     int qrcp_variant(int64_t m, int64_t n); // QR kernel shape for an m x n matrix, -1 if none
+    bool qrcp_blocked(int variant);         // blocked QR on MFMA (writes T; needs the blocked back-transform)
+    int64_t qrcp_t_elems(int64_t s_elems, int64_t count); // T buffer of a blocked-QR plan
     int64_t qrcp_lds_bytes(int variant);
-    int64_t precond_finish_lds_bytes(int64_t m, int64_t n);
+    int64_t precond_finish_lds_bytes(int variant, int64_t m, int64_t n);
     const void *qrcp_symbol(int variant);
     int qrcp_block(int variant);
-    const void *precond_finish_symbol(int64_t m); // kernel instance for a bucket's largest m
-    int precond_finish_block();
+    const void *precond_finish_symbol(int variant, int64_t m); // for the bucket's QR variant and largest m
+    int precond_finish_block(int variant);
     hipError_t launch_qrcp_f64(int variant, int grid, int lds_bytes, const PrecondArgs &args, hipStream_t stream);
     // Q = orthonormal factor of V32 (polar factor by a polynomial in
     // Q^T Q - I) and B = X Q on FP64 MFMA for a preconditioned bucket (ma:
@@ -140,7 +152,7 @@ namespace jsvd
     const void *polar_bq_symbol(int64_t n); // instance for a bucket's largest n (<= 128)
     int polar_bq_lds_bytes(int64_t n);
     hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, hipStream_t stream);
-    hipError_t launch_precond_finish_f64(int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
+    hipError_t launch_precond_finish_f64(int variant, int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
                                          hipStream_t stream);
 
     // SVD adjoint: gA from (U, s, V) and upstream (gU, gs, gV); any upstream

@@ -2,25 +2,28 @@
 // path (no reference counterpart: the reference is plain FP64 Jacobi).
 //
 // One-sided Jacobi applied to X = R^T, where A P = Q1 R is a QR
-// factorisation, converges in about half the sweeps it needs on A (Drmac and
-// Veselic, "New fast and accurate Jacobi SVD algorithm", SIAM J. Matrix Anal.
-// Appl. 29 (2008), use column pivoting; measured on the 128 x 128 matrices of
-// kappa 1e3..1e8 with the reference's pair order, Jacobi on R^T takes 10-11
-// sweeps with pivoting, 11-12 without and 17-26 on A, so the QR here does not
-// pivot: P = I, and the permutation output is the identity).  With
+// factorisation, converges in about half the sweeps it needs on A, and a
+// second QR, R^T = Q2 R2, with Jacobi on X = R2^T saves about a quarter of
+// the rest (Drmac and Veselic, "New fast and accurate Jacobi SVD algorithm",
+// SIAM J. Matrix Anal. Appl. 29 (2008)).  Measured with the reference's pair
+// order on the 128 x 128 matrices of kappa 1e3..1e8 at a cosine threshold of
+// 1e-7 (the FP32 stage's): 15-27 sweeps on A, 10-11 on R^T, 7-8 on R2^T;
+// column pivoting in the first QR saves under half a sweep more, so the QR
+// here does not pivot (P = I, the permutation output is the identity).  With
 // X = Ux S Vx^T,
-//     A = (Q1 Vx) S (P Ux)^T,
-// so U = Q1 Vx and V = P Ux.  Two kernels, one workgroup per matrix each:
+//     A = Q1 R P^T = Q1 R2^T Q2^T P^T = (Q1 Ux) S (P Q2 Vx)^T,
+// so U = Q1 Ux and V = P Q2 Vx.  Two kernels, one workgroup per matrix each:
 //  * qr_f64: Householder QR in FP64, every column in the registers of a lane
 //    group, two workgroup barriers per step (see the kernel).  Writes
-//    X = R^T (row-major n x n), the reflectors (column-major m x n, explicit
-//    zeros above the unit diagonal), tau and the permutation.
-//  * precond_finish_f64: the reflectors in LDS; lane groups of eight carry
-//    one column of [Vx; 0] in registers through H_{n-1} ... H_0 (U = Q1 Vx),
-//    apply the U rule (zero columns where sigma <= eps), and write V = P Ux
-//    by rows.  A column of Ux that is exactly zero (sigma = 0) is completed to
-//    an orthonormal basis from unit vectors, so V stays orthogonal for
-//    rank-deficient inputs.
+//    R^T (row-major n x n), the reflectors (column-major m x n, explicit
+//    zeros above the unit diagonal), tau and the permutation.  It runs on A,
+//    then in place on X = R^T (every column is read before any is written).
+//  * precond_finish_f64: the reflectors of Q1 in LDS; lane groups of eight
+//    carry one column of [Ux; 0] in registers through H_0 ... H_{n-1}
+//    (U = Q1 Ux) and apply the U rule (zero columns where sigma <= eps);
+//    then the reflectors of Q2 carry the columns of Vx (V = P Q2 Vx).
+#include <utility>
+
 #include "jsvd_device.hpp"
 #include "jsvd_kernels.hpp"
 
@@ -190,7 +193,8 @@ namespace jsvd
                     }
                     if (lane == 0)
                     {
-                        args.perm[w.s_off + j] = j;
+                        if (args.perm)
+                            args.perm[w.s_off + j] = j;
                         args.tau[w.s_off + j] = s_tau[j];
                     }
                 }
@@ -199,14 +203,75 @@ namespace jsvd
             finish_list(args.counters);
         }
 
+        // The reflectors H_k (unit diagonal, column-major with `rows` rows at
+        // rg) and their tau into LDS: column k at Hs + k LD with LD = G R >=
+        // rows, zero padding rows (branch-free loops in apply_reflectors).
+        template <int R>
+        __device__ __forceinline__ void load_reflectors(double *Hs, double *taus, const double *rg, const double *tg,
+                                                        int rows, int n)
+        {
+            constexpr int LD = kFinishGroup * R;
+            for (int idx = threadIdx.x; idx < LD * n; idx += kFinishBlock)
+            {
+                const int k = idx / LD, i = idx - k * LD;
+                Hs[idx] = i < rows ? rg[k * rows + i] : 0.0;
+            }
+            for (int j = threadIdx.x; j < n; j += kFinishBlock)
+                taus[j] = tg[j];
+        }
+
+        // dst = H_0 H_1 ... H_{n-1} [src; 0] for the row-major n x n src,
+        // one column per lane group (lane l holds rows l + G t), written
+        // row-major `rows` x n.  U (PERM false): row i to row i, columns j
+        // with sigma[j] <= eps as zeros (the reference's U rule).  V (PERM
+        // true): row i to row row_of[i], every column kept.
+        template <int R, bool PERM>
+        __device__ __forceinline__ void apply_reflectors(const double *Hs, const double *taus, const double *src,
+                                                         double *dst, int rows, int n, const int32_t *row_of,
+                                                         const double *sigma, double eps)
+        {
+            constexpr int G = kFinishGroup, GROUPS = kFinishBlock / G, LD = G * R;
+            const int gid = threadIdx.x / G, lane = threadIdx.x & (G - 1);
+            for (int j = gid; j < n; j += GROUPS)
+            {
+                double y[R];
+#pragma unroll
+                for (int t = 0; t < R; ++t)
+                {
+                    const int i = lane + G * t;
+                    y[t] = i < n ? src[i * n + j] : 0.0;
+                }
+                for (int k = n - 1; k >= 0; --k)
+                {
+                    const double *h = Hs + k * LD;
+                    // Four partial sums: a single chain would wait on every
+                    // FMA's latency.
+                    double dd[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                    for (int t = 0; t < R; ++t)
+                        dd[t & 3] = fma(h[lane + G * t], y[t], dd[t & 3]);
+                    const double d = lane_group_sum<G>((dd[0] + dd[1]) + (dd[2] + dd[3])) * taus[k];
+#pragma unroll
+                    for (int t = 0; t < R; ++t)
+                        y[t] = fma(-d, h[lane + G * t], y[t]);
+                }
+                const bool keep = PERM || sigma[j] > eps;
+#pragma unroll
+                for (int t = 0; t < R; ++t)
+                {
+                    const int i = lane + G * t;
+                    if (i < rows)
+                        dst[(PERM ? row_of[i] : i) * n + j] = keep ? y[t] : 0.0;
+                }
+            }
+        }
+
         template <int R>
         __global__ __launch_bounds__(kFinishBlock) void precond_finish_f64(PrecondArgs args)
         {
             extern __shared__ double2 lds_raw[];
-            double *lds = reinterpret_cast<double *>(lds_raw);
+            double *Hs = reinterpret_cast<double *>(lds_raw);
             __shared__ int s_item;
-            constexpr int G = kFinishGroup, GROUPS = kFinishBlock / G, LD = G * R;
-            const int tid = threadIdx.x, gid = tid / G, lane = tid & (G - 1);
             for (;;)
             {
                 claim(&args.counters[2], &s_item);
@@ -215,144 +280,20 @@ namespace jsvd
                     break;
                 const WorkItem w = args.items[item];
                 const int m = w.m, n = w.n;
-                // Reflectors column-major with LD >= m rows, zero padding rows
-                // (branch-free loops below).
-                double *Hs = lds;
-                double *taus = Hs + LD * n;
-                const double *rg = args.refl + w.a_off;
-                for (int idx = tid; idx < LD * n; idx += kFinishBlock)
-                {
-                    const int k = idx / LD, i = idx - k * LD;
-                    Hs[idx] = i < m ? rg[k * m + i] : 0.0;
-                }
-                for (int j = tid; j < n; j += kFinishBlock)
-                    taus[j] = args.tau[w.s_off + j];
+                double *taus = Hs + kFinishGroup * R * n;
+                // U = Q1 [Ux; 0] with the U rule.
+                load_reflectors<R>(Hs, taus, args.refl + w.a_off, args.tau + w.s_off, m, n);
                 __syncthreads();
-                const double *vx = args.vx + w.v_off;
-                const double *sg = args.sigma + w.s_off;
-                double *ug = args.u + w.a_off;
-                // U = H_0 H_1 ... H_{n-1} [Vx; 0], one column per lane group;
-                // lane l holds rows l + G t.
-                for (int j = gid; j < n; j += GROUPS)
-                {
-                    double y[R];
-#pragma unroll
-                    for (int t = 0; t < R; ++t)
-                    {
-                        const int i = lane + G * t;
-                        y[t] = i < n ? vx[i * n + j] : 0.0;
-                    }
-                    for (int k = n - 1; k >= 0; --k)
-                    {
-                        const double *h = Hs + k * LD;
-                        // Four partial sums: a single chain would wait on every
-                        // FMA's latency.
-                        double dd[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                        for (int t = 0; t < R; ++t)
-                            dd[t & 3] = fma(h[lane + G * t], y[t], dd[t & 3]);
-                        const double d = lane_group_sum<G>((dd[0] + dd[1]) + (dd[2] + dd[3])) * taus[k];
-#pragma unroll
-                        for (int t = 0; t < R; ++t)
-                            y[t] = fma(-d, h[lane + G * t], y[t]);
-                    }
-                    // The reference's U rule: a column whose sigma is not above
-                    // eps is written as zeros.
-                    const bool keep = sg[j] > args.eps;
-#pragma unroll
-                    for (int t = 0; t < R; ++t)
-                    {
-                        const int i = lane + G * t;
-                        if (i < m)
-                            ug[i * n + j] = keep ? y[t] : 0.0;
-                    }
-                }
-                // V = P Ux (row i of Ux is row perm[i] of V), assembled in LDS
-                // over the reflectors, which are no longer needed.
+                apply_reflectors<R, false>(Hs, taus, args.ux + w.v_off, args.u + w.a_off, m, n, nullptr,
+                                    args.sigma + w.s_off, args.eps);
                 __syncthreads();
-                const double *ux = args.ux + w.v_off;
-                const int32_t *pm = args.perm + w.s_off;
-                double *Vs = lds; // n x n row-major (m >= n, so it fits)
-                for (int idx = tid; idx < n * n; idx += kFinishBlock)
-                {
-                    const int i = idx / n, j = idx - i * n;
-                    Vs[pm[i] * n + j] = ux[idx];
-                }
-                int zero = 0;
-                for (int j = tid; j < n; j += kFinishBlock)
-                    zero |= sg[j] == 0.0;
-                zero = __syncthreads_or(zero);
-                // An exactly zero singular value leaves a zero column in Ux.
-                // Wave 0 fills each such column with the unit vector least
-                // covered by the other columns (smallest sum_c V[i][c]^2),
-                // projected off them twice and normalised, so V stays
-                // orthogonal for rank-deficient inputs.
-                if (zero && tid < 64)
-                {
-                    double *r = Vs + n * n; // n scratch values past V (taus are no longer needed)
-                    for (int j = 0; j < n; ++j)
-                    {
-                        if (!(sg[j] == 0.0))
-                            continue;
-                        double best = 1e300;
-                        int bi = n;
-                        for (int i = tid; i < n; i += 64)
-                        {
-                            double cov = 0.0;
-                            for (int c = 0; c < n; ++c)
-                                if (c != j)
-                                    cov = fma(Vs[i * n + c], Vs[i * n + c], cov);
-                            if (cov < best)
-                            {
-                                best = cov;
-                                bi = i;
-                            }
-                        }
-                        for (int o = 32; o > 0; o >>= 1)
-                        {
-                            const double ob = __shfl_xor(best, o, 64);
-                            const int oi = __shfl_xor(bi, o, 64);
-                            if (ob < best || (ob == best && oi < bi))
-                            {
-                                best = ob;
-                                bi = oi;
-                            }
-                        }
-                        for (int i = tid; i < n; i += 64)
-                            r[i] = i == bi ? 1.0 : 0.0;
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                        for (int pass = 0; pass < 2; ++pass)
-                            for (int c = 0; c < n; ++c)
-                            {
-                                if (c == j)
-                                    continue;
-                                double dot = 0.0;
-                                for (int i = tid; i < n; i += 64)
-                                    dot = fma(Vs[i * n + c], r[i], dot);
-                                for (int o = 32; o > 0; o >>= 1)
-                                    dot += __shfl_xor(dot, o, 64);
-                                for (int i = tid; i < n; i += 64)
-                                    r[i] = fma(-dot, Vs[i * n + c], r[i]);
-                                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                                __builtin_amdgcn_wave_barrier();
-                            }
-                        double nr = 0.0;
-                        for (int i = tid; i < n; i += 64)
-                            nr = fma(r[i], r[i], nr);
-                        for (int o = 32; o > 0; o >>= 1)
-                            nr += __shfl_xor(nr, o, 64);
-                        const double inv = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
-                        for (int i = tid; i < n; i += 64)
-                            Vs[i * n + j] = r[i] * inv;
-                        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                        __builtin_amdgcn_wave_barrier();
-                    }
-                }
+                // V = P Q2 Vx: row i of Q2 Vx is row perm[i] of V.  Vx is
+                // orthogonal (accumulated rotations), so V is orthogonal for
+                // every input, rank-deficient ones included.
+                load_reflectors<R>(Hs, taus, args.refl2 + w.v_off, args.tau2 + w.s_off, n, n);
                 __syncthreads();
-                double *vg = args.v + w.v_off;
-                for (int idx = tid; idx < n * n; idx += kFinishBlock)
-                    vg[idx] = Vs[idx];
+                apply_reflectors<R, true>(Hs, taus, args.vx + w.v_off, args.v + w.v_off, n, n, args.perm + w.s_off,
+                                    nullptr, 0.0);
                 __syncthreads();
             }
             finish_list(args.counters + 2);
@@ -543,47 +484,540 @@ namespace jsvd
                 }
             }
         }
+
+        // ------------------------------------------------------------------
+        // Blocked Householder QR and back-transform on FP64 MFMA, one
+        // workgroup of four waves per matrix, rows padded with zeros to
+        // 16 MT, columns in blocks of 16.
+        //
+        // The matrix lives in registers as v_mfma_f64_16x16x4f64 result
+        // tiles: column block b (columns 16 b .. 16 b + 15) belongs to wave
+        // b % 4 as its slot b / 4, and lane l = 16 q + c holds rows
+        // 16 t + q + 4 r of column 16 b + c as element r of tile t.  The
+        // product's operands are A[i][k] from lane 16 k + i and B[k][j] from
+        // lane 16 k + j, so element r of a tile is also the B operand of the
+        // K-step over rows 16 t + 4 r .. 16 t + 4 r + 3: a product V^T Y reads
+        // Y straight from the registers that hold it.
+        //
+        // QR (LAPACK dgeqrf with dlarft's compact WY form): one wave factors
+        // panel p (the 16 columns of block p, rows 16 p ..) in registers,
+        // Householder step by step (dlarfg's beta, tau and v), forming T from
+        // the reflectors' inner products as it goes; then every wave applies
+        // Q_p^T = I - V T^T V^T to its blocks: W = V^T C, W <- T^T W,
+        // C <- C - V W, all on MFMA with V and T in LDS, and the owner of
+        // panel p + 1 factors it.  V and T are double-buffered, so one
+        // workgroup barrier per panel suffices.
+        template <int MT>
+        struct BlkShape
+        {
+            static constexpr int ROWS = 16 * MT; // padded rows
+            static constexpr int LDV = ROWS + 4; // LDS stride of a V column (spreads the 16 columns over banks)
+            static constexpr int S = 4 * MT;     // rows per lane of a column
+        };
+
+        __device__ __forceinline__ double readlane_f64(double x, int l)
+        {
+            const long long bits = __double_as_longlong(x);
+            const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffll), l);
+            const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), l);
+            return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                        static_cast<long long>(static_cast<unsigned int>(lo)));
+        }
+
+        __device__ __forceinline__ void wave_lds_sync()
+        {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+
+        // Y <- (I - V T V^T) Y (TRANS false) or (I - V T^T V^T) Y (TRANS
+        // true) for one column block y of MT tiles.  V: LDS, column-major with
+        // stride LDV, rows from 16 t0 on (V's rows above are zero: the tiles
+        // below t0 take zero operands, selected rather than branched around
+        // so that no tile's registers need copies); T: LDS, 16 x 16
+        // row-major.
+        template <int MT, bool TRANS>
+        __device__ __forceinline__ void apply_block_reflector(f64x4 (&y)[MT], const double *V, const double *T, int t0,
+                                                              int c, int q)
+        {
+            constexpr int LDV = BlkShape<MT>::LDV;
+            // W = V^T Y: A[a][row] = V[row][a] from lane (c = a, q = row % 4),
+            // one accumulator per K-step r (four independent chains), at most
+            // one tile's operands loaded ahead.
+            f64x4 wr[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                wr[r] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+            {
+                const bool on = t >= t0;
+                const int tt = on ? t - t0 : 0;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                {
+                    const double a = V[c * LDV + 16 * tt + 4 * r + q];
+                    wr[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? a : 0.0, y[t][r], wr[r], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            const f64x4 w = (wr[0] + wr[1]) + (wr[2] + wr[3]);
+            // W2 = T^T W or T W: A[a][k] = T[k][a] or T[a][k], B = W's elements.
+            f64x4 w2 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                w2 = __builtin_amdgcn_mfma_f64_16x16x4f64(TRANS ? T[(4 * r + q) * 16 + c] : T[c * 16 + 4 * r + q], w[r],
+                                                          w2, 0, 0, 0);
+            // Y <- Y - V W2: A[i][k] = -V[16 t + i][4 r + k]; the tiles'
+            // products are independent, one K-step of all tiles at a time.
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+            {
+#pragma unroll
+                for (int t = 0; t < MT; ++t)
+                {
+                    const bool on = t >= t0;
+                    const int tt = on ? t - t0 : 0;
+                    const double a = V[(4 * r + q) * LDV + 16 * tt + c];
+                    y[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(on ? -a : 0.0, w2[r], y[t], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+
+        // Lane J of each row of 16 lanes, to every lane of the row (DPP
+        // row_newbcast, two 32-bit moves).
+        template <int J>
+        __device__ __forceinline__ double row_bcast(double x)
+        {
+            const long long bits = __double_as_longlong(x);
+            const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(bits & 0xffffffffll), 0x150 + J, 0xF, 0xF, false);
+            const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(bits >> 32), 0x150 + J, 0xF, 0xF, false);
+            return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                                        static_cast<long long>(static_cast<unsigned int>(lo)));
+        }
+
+        // Householder step J of a panel held by one wave: lane (c, q) holds
+        // rows q + 4 s of panel column c in x[s]; trow is row c of T and
+        // tau_c the tau of column c.  Every lane forms the reflector vector
+        // of its own column and row J's lane broadcasts column J's to its
+        // row of 16 lanes (the lanes of one q hold the same rows), so the
+        // step has no divergent branch and no LDS round trip.
+        template <int J, int S>
+        __device__ __forceinline__ void qr_step(double (&x)[S], double (&trow)[16], double &tau_c, int c, int q)
+        {
+            constexpr int sj = J / 4, qj = J % 4;
+            // Head (row J) and squared tail (rows below J) of each column.
+            double h = q == qj ? x[sj] : 0.0;
+            double tl0 = q > qj ? x[sj] * x[sj] : 0.0, tl1 = 0.0;
+#pragma unroll
+            for (int s = sj + 1; s < S; ++s)
+            {
+                if (s & 1)
+                    tl1 = fma(x[s], x[s], tl1);
+                else
+                    tl0 = fma(x[s], x[s], tl0);
+            }
+            h = dev::xor32_sum(dev::xor16_sum(h));
+            const double tl = dev::xor32_sum(dev::xor16_sum(tl0 + tl1));
+            const double alpha = readlane_f64(h, J), x2 = readlane_f64(tl, J);
+            // LAPACK dlarfg: beta = -sign(alpha) ||(alpha, x)||,
+            // tau = (beta - alpha) / beta, v = x / (alpha - beta); tau = 0
+            // (no reflection) when x = 0.
+            double beta = alpha, tj = 0.0, scale = 0.0;
+            if (x2 != 0.0)
+            {
+                beta = -copysign(sqrt(fma(alpha, alpha, x2)), alpha);
+                tj = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            // v (zeros above row J, 1 on it) of this lane's column, column J's
+            // broadcast; column J keeps its R entries, takes beta on row J and
+            // v below.
+            double vb[S];
+#pragma unroll
+            for (int s = sj; s < S; ++s)
+            {
+                const int o = q + 4 * s;
+                const double xs = x[s] * scale;
+                const double v = s > sj ? xs : (o > J ? xs : (o == J ? 1.0 : 0.0));
+                vb[s] = row_bcast<J>(v);
+                const double nx = s > sj ? v : (o > J ? v : (o == J ? beta : x[s]));
+                x[s] = c == J ? nx : x[s];
+            }
+            double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+            for (int s = sj; s < S; ++s)
+            {
+                if (s & 1)
+                    d1 = fma(vb[s], x[s], d1);
+                else
+                    d0 = fma(vb[s], x[s], d0);
+            }
+            const double d = dev::xor32_sum(dev::xor16_sum(d0 + d1));
+            // Column J of T (dlarft, forward, columnwise): T[c][J] =
+            // -tau_J sum_l T[c][l] G[l][J] for c < J, with G[l][J] = v_l^T v_J
+            // the d of column l (its rows from J on hold v_l); T[J][J] = tau_J.
+            double acc = 0.0;
+#pragma unroll
+            for (int l = 0; l < J; ++l)
+                acc = fma(trow[l], readlane_f64(d, l), acc);
+            trow[J] = c < J ? -tj * acc : (c == J ? tj : 0.0);
+            tau_c = c == J ? tj : tau_c;
+            // Later columns: x <- x - tau v (v^T x).
+            const double f = c > J ? tj * d : 0.0;
+#pragma unroll
+            for (int s = sj; s < S; ++s)
+                x[s] = fma(-f, vb[s], x[s]);
+        }
+
+        template <int S, int... J>
+        __device__ __forceinline__ void qr_steps(double (&x)[S], double (&trow)[16], double &tau_c, int c, int q,
+                                                 std::integer_sequence<int, J...>)
+        {
+            ((qr_step<J, S>(x, trow, tau_c, c, q), __builtin_amdgcn_sched_barrier(0)), ...);
+        }
+
+        // Panel p of the QR, by its owning wave: the block's tiles from t = p
+        // on are staged in LDS (rows relative to 16 p, zeros beyond the padded
+        // matrix) into registers x[s] (rows q + 4 s of column c), factored by
+        // 16 Householder steps, and leave behind the explicit V (unit
+        // diagonal, zeros above) in Vp, T in Tp and the outputs: R's rows (as
+        // X = R^T), the reflectors, tau and T.  Tiles above p hold R rows that
+        // earlier panels finished.
+        template <int MT, int NBW>
+        __device__ __forceinline__ void qr_panel(const f64x4 (&acc)[NBW][MT], int slot, int p, double *Vp, double *Tp,
+                                                 const PrecondArgs &args, const WorkItem &w, int64_t t_off, int c,
+                                                 int q)
+        {
+            constexpr int LDV = BlkShape<MT>::LDV, S = BlkShape<MT>::S;
+            const int m = w.m, n = w.n, col = 16 * p + c;
+            const bool live = col < n;
+            double *xg = args.x + w.v_off;
+            double *rg = args.refl + w.a_off;
+            // Stage the block from tile p on; write the finished R rows above.
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+            {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                {
+                    const int row = 16 * t + q + 4 * r;
+                    double val = acc[0][t][r];
+                    if constexpr (NBW > 1)
+                        val = slot == 1 ? acc[1][t][r] : val;
+                    if (t >= p)
+                        Vp[c * LDV + row - 16 * p] = val;
+                    else if (live)
+                    {
+                        if (row < n)
+                            xg[int64_t(col) * n + row] = val;
+                        rg[int64_t(col) * m + row] = 0.0;
+                    }
+                }
+                if (t < p)
+                {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        Vp[c * LDV + 16 * (MT - 1 - t) + q + 4 * r] = 0.0; // rows past the padded matrix
+                }
+            }
+            wave_lds_sync();
+            double x[S];
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                x[s] = Vp[c * LDV + q + 4 * s];
+            double trow[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                trow[j] = 0.0;
+            double tau_c = 0.0;
+            qr_steps<S>(x, trow, tau_c, c, q, std::make_integer_sequence<int, 16>{});
+            wave_lds_sync();
+            // V for the other waves, T, and the outputs of column col: R's
+            // rows 16 p .. as X = R^T (zeros below the diagonal), the
+            // reflector (zeros above, unit diagonal), tau.
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+            {
+                const int o = q + 4 * s;
+                Vp[c * LDV + o] = o < c ? 0.0 : (o == c ? 1.0 : x[s]);
+            }
+            if (q == 0)
+            {
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                {
+                    Tp[c * 16 + j] = trow[j];
+                    args.tq[t_off + 256 * p + c * 16 + j] = trow[j];
+                }
+            }
+            if (live)
+            {
+#pragma unroll
+                for (int s = 0; s < S; ++s)
+                {
+                    const int o = q + 4 * s, row = 16 * p + o;
+                    if (row < n)
+                        xg[int64_t(col) * n + row] = o <= c ? x[s] : 0.0;
+                    if (row < m)
+                        rg[int64_t(col) * m + row] = o < c ? 0.0 : (o == c ? 1.0 : x[s]);
+                }
+                if (q == 0)
+                    args.tau[w.s_off + col] = tau_c;
+            }
+            wave_lds_sync();
+        }
+
+        template <int MT, int NBW>
+        __global__ __launch_bounds__(256) void qr_blocked_f64(PrecondArgs args)
+        {
+            constexpr int LDV = BlkShape<MT>::LDV;
+            __shared__ double Vbuf[2][16 * LDV];
+            __shared__ double Tbuf[2][256];
+            __shared__ int s_item;
+            const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+            const int c = lane & 15, q = lane >> 4;
+            for (;;)
+            {
+                claim(&args.counters[0], &s_item);
+                const int item = s_item;
+                if (item >= args.count)
+                    break;
+                const WorkItem w = args.items[item];
+                const int m = w.m, n = w.n, nb = (n + 15) / 16;
+                const int64_t t_off = 16 * w.s_off + 240 * int64_t(w.index);
+                const double *ag = args.a + w.a_off;
+                f64x4 acc[NBW][MT];
+#pragma unroll
+                for (int sl = 0; sl < NBW; ++sl)
+                {
+                    const int col = 16 * (4 * sl + wave) + c;
+#pragma unroll
+                    for (int t = 0; t < MT; ++t)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                        {
+                            const int row = 16 * t + q + 4 * r;
+                            acc[sl][t][r] = (row < m && col < n) ? ag[int64_t(row) * n + col] : 0.0;
+                        }
+                }
+                // Panel p + 1 is factored by its owner after every wave has
+                // applied panel p to all its blocks (finished blocks are never
+                // read again and padding columns stay zero, so every slot is
+                // updated without a test).
+                for (int p = -1; p + 1 < nb; ++p)
+                {
+                    if (p >= 0)
+                    {
+#pragma unroll
+                        for (int sl = 0; sl < NBW; ++sl)
+                            apply_block_reflector<MT, true>(acc[sl], Vbuf[p & 1], Tbuf[p & 1], p, c, q);
+                    }
+                    const int np = p + 1;
+                    if (wave == (np & 3))
+                        qr_panel<MT, NBW>(acc, np >> 2, np, Vbuf[np & 1], Tbuf[np & 1], args, w, t_off, c, q);
+                    __syncthreads();
+                }
+            }
+            finish_list(args.counters);
+        }
+
+        // Y <- Q Y for the Q = Q_0 Q_1 ... Q_{nb-1} of a blocked QR (panel p:
+        // columns 16 p .. of the column-major `rows` x n reflectors at rg, T
+        // at tg + 256 p), over the column blocks y of this wave; panels are
+        // staged in LDS by the whole workgroup, last panel first.
+        template <int MT, int NBW>
+        __device__ __forceinline__ void apply_q_blocked(f64x4 (&y)[NBW][MT], const double *rg, const double *tg,
+                                                        int rows, int n, double (*Vbuf)[16 * BlkShape<MT>::LDV],
+                                                        double (*Tbuf)[256], int wave, int c, int q)
+        {
+            constexpr int LDV = BlkShape<MT>::LDV, ROWS = BlkShape<MT>::ROWS;
+            const int tid = threadIdx.x, nb = (n + 15) / 16;
+            auto stage = [&](int p) {
+                double *Vp = Vbuf[p & 1];
+                for (int idx = tid; idx < 16 * ROWS; idx += 256)
+                {
+                    const int cc = idx / ROWS, o = idx - cc * ROWS, row = 16 * p + o, col = 16 * p + cc;
+                    Vp[cc * LDV + o] = (row < rows && col < n) ? rg[int64_t(col) * rows + row] : 0.0;
+                }
+                Tbuf[p & 1][tid] = tg[256 * p + tid];
+            };
+            stage(nb - 1);
+            __syncthreads();
+            for (int p = nb - 1; p >= 0; --p)
+            {
+                if (p > 0)
+                    stage(p - 1);
+                const double *V = Vbuf[p & 1];
+#pragma unroll
+                for (int sl = 0; sl < NBW; ++sl)
+                    apply_block_reflector<MT, false>(y[sl], V, Tbuf[p & 1], p, c, q);
+                __syncthreads();
+            }
+        }
+
+        // Back-transform of a blocked-QR bucket: U = Q1 [Ux; 0] with the U
+        // rule, then V = Q2 Vx (the QR does not pivot: P = I).
+        template <int MT, int NBW>
+        __global__ __launch_bounds__(256) void finish_blocked_f64(PrecondArgs args)
+        {
+            constexpr int LDV = BlkShape<MT>::LDV;
+            __shared__ double Vbuf[2][16 * LDV];
+            __shared__ double Tbuf[2][256];
+            __shared__ int s_item;
+            const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+            const int c = lane & 15, q = lane >> 4;
+            for (;;)
+            {
+                claim(&args.counters[2], &s_item);
+                const int item = s_item;
+                if (item >= args.count)
+                    break;
+                const WorkItem w = args.items[item];
+                const int m = w.m, n = w.n;
+                const int64_t t_off = 16 * w.s_off + 240 * int64_t(w.index);
+                f64x4 y[NBW][MT];
+                for (int part = 0; part < 2; ++part)
+                {
+                    // part 0: Ux (n x n) -> U (m x n); part 1: Vx -> V (n x n).
+                    const double *src = (part == 0 ? args.ux : args.vx) + w.v_off;
+                    const int rows = part == 0 ? m : n;
+#pragma unroll
+                    for (int sl = 0; sl < NBW; ++sl)
+                    {
+                        const int col = 16 * (4 * sl + wave) + c;
+#pragma unroll
+                        for (int t = 0; t < MT; ++t)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                            {
+                                const int row = 16 * t + q + 4 * r;
+                                y[sl][t][r] = (row < n && col < n) ? src[row * n + col] : 0.0;
+                            }
+                    }
+                    if (part == 0)
+                        apply_q_blocked<MT, NBW>(y, args.refl + w.a_off, args.tq + t_off, rows, n, Vbuf, Tbuf, wave, c,
+                                                 q);
+                    else
+                        apply_q_blocked<MT, NBW>(y, args.refl2 + w.v_off, args.tq2 + t_off, rows, n, Vbuf, Tbuf, wave,
+                                                 c, q);
+                    double *dst = (part == 0 ? args.u + w.a_off : args.v + w.v_off);
+#pragma unroll
+                    for (int sl = 0; sl < NBW; ++sl)
+                    {
+                        const int col = 16 * (4 * sl + wave) + c;
+                        if (col < n)
+                        {
+                            const bool keep = part == 1 || args.sigma[w.s_off + col] > args.eps;
+#pragma unroll
+                            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                                for (int r = 0; r < 4; ++r)
+                                {
+                                    const int row = 16 * t + q + 4 * r;
+                                    if (row < rows)
+                                        dst[row * n + col] = keep ? y[sl][t][r] : 0.0;
+                                }
+                        }
+                    }
+                }
+            }
+            finish_list(args.counters + 2);
+        }
     } // namespace
 
     namespace
     {
-        // QR shapes: (workgroup, lanes per column, rows per lane); a matrix
-        // takes the first that fits (n <= BS / G columns, m <= G RW rows).
+        // QR shapes; a matrix takes the first that fits.  Unblocked
+        // (mt = 0): qr_f64<block, group, rows> for n <= block / group columns
+        // and m <= group rows rows.  Blocked (mt > 0): qr_blocked_f64<mt, nbw>
+        // for m <= 16 mt rows and n <= 64 nbw columns.  Up to 64 columns the
+        // unblocked kernel is faster (measured on the 4096 uneven batch:
+        // 2.97 vs 3.15 ms for the mixed execution, the blocked panels' 16
+        // dependent steps then outweigh their cheaper updates); from 65 to
+        // 128 columns the blocked one (1024 x 128x128: 10.0 vs 11.7 ms).
         struct QrSpec
         {
-            int block, group, rows;
+            int block, group, rows, mt, nbw;
         };
-        constexpr QrSpec kQr[] = {{256, 4, 8}, {256, 4, 16}, {256, 4, 32}, {512, 4, 32}, {512, 8, 32}};
+        constexpr QrSpec kQr[] = {{256, 4, 8, 0, 0}, {256, 4, 16, 0, 0}, {256, 4, 32, 0, 0},
+                                  {256, 0, 0, 8, 2}, {512, 4, 32, 0, 0}, {512, 8, 32, 0, 0}};
         constexpr int kQrCount = sizeof(kQr) / sizeof(kQr[0]);
         template <int I>
         const void *qr_symbol_t()
         {
-            return reinterpret_cast<const void *>(&qr_f64<kQr[I].block, kQr[I].group, kQr[I].rows>);
+            if constexpr (kQr[I].mt > 0)
+                return reinterpret_cast<const void *>(&qr_blocked_f64<kQr[I].mt, kQr[I].nbw>);
+            else
+                return reinterpret_cast<const void *>(&qr_f64<kQr[I].block, kQr[I].group, kQr[I].rows>);
         }
         template <int I>
         void qr_launch_t(int grid, int lds, const PrecondArgs &args, hipStream_t stream)
         {
-            qr_f64<kQr[I].block, kQr[I].group, kQr[I].rows><<<grid, kQr[I].block, lds, stream>>>(args);
+            if constexpr (kQr[I].mt > 0)
+                qr_blocked_f64<kQr[I].mt, kQr[I].nbw><<<grid, kQr[I].block, 0, stream>>>(args);
+            else
+                qr_f64<kQr[I].block, kQr[I].group, kQr[I].rows><<<grid, kQr[I].block, lds, stream>>>(args);
         }
-        const void *(*const kQrSymbol[])() = {qr_symbol_t<0>, qr_symbol_t<1>, qr_symbol_t<2>, qr_symbol_t<3>,
-                                             qr_symbol_t<4>};
+        template <int I>
+        const void *finish_blocked_symbol_t()
+        {
+            if constexpr (kQr[I].mt > 0)
+                return reinterpret_cast<const void *>(&finish_blocked_f64<kQr[I].mt, kQr[I].nbw>);
+            else
+                return nullptr;
+        }
+        template <int I>
+        void finish_blocked_launch_t(int grid, const PrecondArgs &args, hipStream_t stream)
+        {
+            if constexpr (kQr[I].mt > 0)
+                finish_blocked_f64<kQr[I].mt, kQr[I].nbw><<<grid, 256, 0, stream>>>(args);
+        }
+        const void *(*const kQrSymbol[])() = {qr_symbol_t<0>, qr_symbol_t<1>, qr_symbol_t<2>,
+                                             qr_symbol_t<3>, qr_symbol_t<4>, qr_symbol_t<5>};
         void (*const kQrLaunch[])(int, int, const PrecondArgs &, hipStream_t) = {
-            qr_launch_t<0>, qr_launch_t<1>, qr_launch_t<2>, qr_launch_t<3>, qr_launch_t<4>};
+            qr_launch_t<0>, qr_launch_t<1>, qr_launch_t<2>, qr_launch_t<3>, qr_launch_t<4>, qr_launch_t<5>};
+        const void *(*const kFinishBlockedSymbol[])() = {finish_blocked_symbol_t<0>, finish_blocked_symbol_t<1>,
+                                                         finish_blocked_symbol_t<2>, finish_blocked_symbol_t<3>,
+                                                         finish_blocked_symbol_t<4>, finish_blocked_symbol_t<5>};
+        void (*const kFinishBlockedLaunch[])(int, const PrecondArgs &, hipStream_t) = {
+            finish_blocked_launch_t<0>, finish_blocked_launch_t<1>, finish_blocked_launch_t<2>,
+            finish_blocked_launch_t<3>, finish_blocked_launch_t<4>, finish_blocked_launch_t<5>};
         static_assert(sizeof(kQrLaunch) / sizeof(kQrLaunch[0]) == kQrCount, "one launcher per shape");
+        static_assert(sizeof(kQrSymbol) / sizeof(kQrSymbol[0]) == kQrCount, "one symbol per shape");
+        static_assert(sizeof(kFinishBlockedLaunch) / sizeof(kFinishBlockedLaunch[0]) == kQrCount,
+                      "one back-transform launcher per shape");
     } // namespace
 
     int qrcp_variant(int64_t m, int64_t n)
     {
         for (int v = 0; v < kQrCount; ++v)
-            if (n <= kQr[v].block / kQr[v].group && m <= int64_t(kQr[v].group) * kQr[v].rows)
+        {
+            const QrSpec &k = kQr[v];
+            const bool fits = k.mt > 0 ? (m <= 16 * k.mt && n <= 64 * k.nbw && n <= m)
+                                       : (n <= k.block / k.group && m <= int64_t(k.group) * k.rows);
+            if (fits)
                 return v;
+        }
         return -1;
+    }
+
+    bool qrcp_blocked(int variant)
+    {
+        return variant >= 0 && variant < kQrCount && kQr[variant].mt > 0;
+    }
+
+    int64_t qrcp_t_elems(int64_t s_elems, int64_t count)
+    {
+        return 16 * s_elems + 240 * count;
     }
 
     int64_t qrcp_lds_bytes(int variant)
     {
-        // The reflector of a step over every register row (G RW >= m).
-        return int64_t(kQr[variant].group) * kQr[variant].rows * 8;
+        // Unblocked: the reflector of a step over every register row
+        // (G RW >= m).  Blocked: static LDS only.
+        return kQr[variant].mt > 0 ? 0 : int64_t(kQr[variant].group) * kQr[variant].rows * 8;
     }
 
     namespace
@@ -595,17 +1029,21 @@ namespace jsvd
         }
     } // namespace
 
-    int64_t precond_finish_lds_bytes(int64_t m, int64_t n)
+    int64_t precond_finish_lds_bytes(int variant, int64_t m, int64_t n)
     {
-        // Reflectors (LD = 8 x rows per lane, zero padded) and tau, later
-        // V n x n and one scratch column.
+        // Blocked: static LDS only.  Unblocked: reflectors (LD = 8 x rows per
+        // lane, zero padded) and tau; those of Q2 (n <= m rows) reuse the
+        // space.
+        if (qrcp_blocked(variant))
+            return 0;
         return (int64_t(kFinishGroup) * finish_rows(m) * n + n) * 8;
     }
 
     bool precond_fits(int64_t m, int64_t n)
     {
-        return n <= 128 && qrcp_variant(m, n) >= 0 && m <= int64_t(kFinishGroup) * kFinishRows &&
-               precond_finish_lds_bytes(m, n) <= kMaxLdsBytes;
+        const int v = qrcp_variant(m, n);
+        return n <= 128 && v >= 0 && m <= int64_t(kFinishGroup) * kFinishRows &&
+               precond_finish_lds_bytes(v, m, n) <= kMaxLdsBytes;
     }
 
     const void *qrcp_symbol(int variant)
@@ -616,16 +1054,18 @@ namespace jsvd
     {
         return kQr[variant].block;
     }
-    const void *precond_finish_symbol(int64_t m)
+    const void *precond_finish_symbol(int variant, int64_t m)
     {
+        if (qrcp_blocked(variant))
+            return kFinishBlockedSymbol[variant]();
         const int r = finish_rows(m);
         return r == 8    ? reinterpret_cast<const void *>(&precond_finish_f64<8>)
                : r == 16 ? reinterpret_cast<const void *>(&precond_finish_f64<16>)
                          : reinterpret_cast<const void *>(&precond_finish_f64<32>);
     }
-    int precond_finish_block()
+    int precond_finish_block(int variant)
     {
-        return kFinishBlock;
+        return qrcp_blocked(variant) ? 256 : kFinishBlock;
     }
 
     const void *polar_bq_symbol(int64_t n)
@@ -657,9 +1097,14 @@ namespace jsvd
         return hipGetLastError();
     }
 
-    hipError_t launch_precond_finish_f64(int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
+    hipError_t launch_precond_finish_f64(int variant, int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
                                          hipStream_t stream)
     {
+        if (qrcp_blocked(variant))
+        {
+            kFinishBlockedLaunch[variant](grid, args, stream);
+            return hipGetLastError();
+        }
         const int r = finish_rows(m);
         if (r == 8)
             precond_finish_f64<8><<<grid, kFinishBlock, lds_bytes, stream>>>(args);
