@@ -309,7 +309,7 @@ constexpr int kCountersPerBucket = 12;
 // split into up to kMixedSplitParts parts of at least kMixedSplitMin
 // matrices, each on a stream of its own.
 constexpr size_t kMixedSplitMin = 64;
-constexpr size_t kMixedSplitParts = 4;
+constexpr size_t kMixedSplitParts = 2;
 constexpr int64_t kMixedSplitMinColumns = 64;
 
 // A bucket is one persistent launch of the LDS-resident work-list kernel:
@@ -1159,11 +1159,12 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
     // each ends in a tail where few workgroups still run.  A large bucket of
     // wide matrices is split into parts (items of its longest-first order
     // dealt out in turn) that the stream assignment below puts on streams of
-    // their own: one part's stages then fill the others' tails (1024 x
-    // 128x128: 11.66 -> 11.46 ms).  Buckets of at most 64 columns stay
+    // their own: one part's stages then fill the other's tails (1024 x
+    // 128x128 with the blocked QR: 10.3 ms whole, 9.7 in two parts, 10.2 in
+    // three, 10.0 in four, 10.6 in eight).  Buckets of at most 64 columns stay
     // whole: their stages are short, and four chains of five launches on
-    // each stream cost more than the tails (uneven 4096 batch: 4.12 ms
-    // split, 2.98 ms whole).  Each matrix's arithmetic does not depend on
+    // each stream cost more than the tails (uneven 4096 batch: 4.12 ms in
+    // four parts, 2.98 ms whole).  Each matrix's arithmetic does not depend on
     // its bucket.
     {
         const size_t initial = groups.size();

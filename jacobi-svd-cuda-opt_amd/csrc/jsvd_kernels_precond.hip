@@ -461,18 +461,85 @@ namespace jsvd
                 // B = X Q (X row-major, B column-major).
                 mfma_nn<NT>(acc, [=](int i, int k) { return (i < n && k < n) ? x[i * n + k] : 0.0; }, sm, n, wr, wc,
                             lr, lk);
+                auto each = [&](auto f) {
 #pragma unroll
-                for (int xx = 0; xx < T; ++xx)
+                    for (int xx = 0; xx < T; ++xx)
 #pragma unroll
-                    for (int yy = 0; yy < T; ++yy)
+                        for (int yy = 0; yy < T; ++yy)
 #pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                        {
-                            const int i = wr + 16 * xx + lk + 4 * r, j = wc + 16 * yy + lr;
-                            if (i < n && j < n)
-                                b[j * n + i] = acc[xx][yy][r];
-                        }
-                __syncthreads(); // LDS free for the next matrix
+                            for (int r = 0; r < 4; ++r)
+                                f(acc[xx][yy][r], wr + 16 * xx + lk + 4 * r, wc + 16 * yy + lr);
+                };
+                each([&](double v, int i, int j) {
+                    if (i < n && j < n)
+                        b[j * n + i] = v;
+                });
+                __threadfence_block();
+                __syncthreads();
+                // One simultaneous first-order Jacobi step on B: with
+                // G = B^T B, the rotation generator E_ij = G_ij / (G_jj - G_ii)
+                // (antisymmetric, formed from G's upper triangle) and
+                // W = I + E + E^2/2 + E^3/6 (orthogonal to O(E^4)), B <- B W
+                // and Q <- Q W take the columns' cosines from the FP32
+                // stage's ~5e-6 to ~1e-10, which saves the FP64 refinement
+                // about 1.6 of its ~3.6 sweeps.  Entries above 2^-14 (close
+                // singular values) are left to the sweeps.
+                auto bt = [=](int i, int k) { return (i < n && k < n) ? b[i * n + k] : 0.0; }; // (B^T)[i][k]
+                auto bc = [=](int k, int j) { return (k < n && j < n) ? b[j * n + k] : 0.0; }; // B[k][j]
+                mfma_nn<NT>(acc, bt, bc, n, wr, wc, lr, lk);
+                each([&](double v, int i, int j) { S[i * LD + j] = (i < n && j < n) ? v : 0.0; });
+                __syncthreads();
+                // E in place: the thread of pair i < j reads G_ij, G_ii, G_jj
+                // and writes E_ij and E_ji = -E_ij; the diagonal (read by
+                // every pair) is cleared after a barrier.
+                for (int idx = tid; idx < n * n; idx += 256)
+                {
+                    const int i = idx / n, j = idx - i * n;
+                    if (i < j)
+                    {
+                        const double g = S[i * LD + j], den = S[j * LD + j] - S[i * LD + i];
+                        const double e = (den != 0.0 && fabs(g) * 16384.0 <= fabs(den)) ? g / den : 0.0;
+                        S[i * LD + j] = e;
+                        S[j * LD + i] = -e;
+                    }
+                }
+                __syncthreads();
+                for (int i = tid; i < n; i += 256)
+                    S[i * LD + i] = 0.0;
+                __syncthreads();
+                // P1 = E (I + E/3); X (no longer needed) takes I + P1/2.
+                mfma_nn<NT>(acc, sm,
+                            [=](int k, int j) { return S[k * LD + j] * (1.0 / 3.0) + (k == j ? 1.0 : 0.0); }, n, wr,
+                            wc, lr, lk);
+                double *xs = pa.x + w.v_off;
+                each([&](double v, int i, int j) {
+                    if (i < n && j < n)
+                        xs[i * n + j] = fma(0.5, v, i == j ? 1.0 : 0.0);
+                });
+                __threadfence_block();
+                __syncthreads();
+                // W = I + E (I + P1/2), into LDS once every wave has read E.
+                mfma_nn<NT>(acc, sm, [=](int k, int j) { return (k < n && j < n) ? xs[k * n + j] : 0.0; }, n, wr,
+                            wc, lr, lk);
+                __syncthreads();
+                each([&](double v, int i, int j) { S[i * LD + j] = (i < n && j < n) ? v + (i == j ? 1.0 : 0.0) : 0.0; });
+                __syncthreads();
+                // B <- B W and Q <- Q W, each stored once every wave has read it.
+                mfma_nn<NT>(acc, [=](int i, int k) { return (i < n && k < n) ? b[k * n + i] : 0.0; }, sm, n, wr, wc,
+                            lr, lk);
+                __syncthreads();
+                each([&](double v, int i, int j) {
+                    if (i < n && j < n)
+                        b[j * n + i] = v;
+                });
+                mfma_nn<NT>(acc, qr, sm, n, wr, wc, lr, lk);
+                __syncthreads();
+                each([&](double v, int i, int j) {
+                    if (i < n && j < n)
+                        q[j * n + i] = v;
+                });
+                __threadfence_block();
+                __syncthreads(); // LDS and the global operands free for the next matrix
             }
             if (threadIdx.x == 0)
             {
