@@ -691,12 +691,25 @@ namespace jsvd
             // LAPACK dlarfg: beta = -sign(alpha) ||(alpha, x)||,
             // tau = (beta - alpha) / beta, v = x / (alpha - beta); tau = 0
             // (no reflection) when x = 0.
+            // |beta| = sqrt(s2), s2 = alpha^2 + x2, from the refined 1/sqrt
+            // estimate of s2 scaled by an even power of two into [0.5, 2);
+            // then d = alpha - beta (no cancellation: alpha and -beta share
+            // a sign), scale = 1/d from the refined reciprocal of d scaled
+            // into [0.5, 1), and tau = -d / beta = |d| / |beta|.
             double beta = alpha, tj = 0.0, scale = 0.0;
             if (x2 != 0.0)
             {
-                beta = -copysign(sqrt(fma(alpha, alpha, x2)), alpha);
-                tj = (beta - alpha) / beta;
-                scale = 1.0 / (alpha - beta);
+                const double s2 = fma(alpha, alpha, x2);
+                const int k = __builtin_amdgcn_frexp_exp(s2) & ~1;
+                const double sc = __builtin_amdgcn_ldexp(s2, -k);
+                const double y = dev::rsqrt_refined(sc);
+                const double bmag = __builtin_amdgcn_ldexp(sc * y, k / 2);
+                beta = -copysign(bmag, alpha);
+                const double d = alpha - beta;
+                const int kd = __builtin_amdgcn_frexp_exp(d);
+                const double r = dev::divide_refined(1.0, fabs(__builtin_amdgcn_ldexp(d, -kd)));
+                scale = copysign(__builtin_amdgcn_ldexp(r, -kd), d);
+                tj = fabs(d) * __builtin_amdgcn_ldexp(y, -(k / 2));
             }
             // v (zeros above row J, 1 on it) of this lane's column, column J's
             // broadcast; column J keeps its R entries, takes beta on row J and
