@@ -38,10 +38,13 @@ def _check(a, got, ref_sigma, eps, tag):
     return norm, comp
 
 
-# Up to 128 rows the QR preconditioner runs blocked on MFMA (16-column
-# panels); 200 x 50 and 256 x 64 take the one-column-per-lane-group QR for A
-# and the blocked one for R^T, and the reflector-by-reflector back-transform.
-@pytest.mark.parametrize("shape", [(40, 30), (64, 64), (100, 60), (128, 128), (128, 100), (200, 50), (256, 64)])
+# From 65 to 128 columns (at most 128 rows) the QR preconditioner and the
+# back-transform run blocked on MFMA (16-column panels): 128 x 65 ends in a
+# one-column panel, 120 x 120 and 96 x 80 pad rows and columns to the tiles.
+# Up to 64 columns, and 200 x 50 / 256 x 64, take the one-column-per-lane-group
+# QR and the reflector-by-reflector back-transform.
+@pytest.mark.parametrize("shape", [(40, 30), (64, 64), (100, 60), (128, 128), (128, 100), (128, 65), (120, 120),
+                                   (96, 80), (200, 50), (256, 64)])
 def test_mixed_single_shapes(shape):
     _note(f"mixed shape {shape}")
     rng = np.random.default_rng(3 + shape[0] + shape[1])
