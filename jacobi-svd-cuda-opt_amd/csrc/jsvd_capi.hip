@@ -398,6 +398,7 @@ struct jsvd_plan
         bool ring = false;      // ring form (wave_ring_sweeps_f64)
         int64_t mbox_group = 0; // ring form: mailbox words per group
         int xcd_spread = 0;     // ring form: groups placed one XCD each (0: group-major grid)
+        bool ring_db = false;   // ring form: two LDS slot sets (one barrier per round)
     };
     std::vector<int64_t> wave;                 // batch index per entry
     std::vector<int64_t> wave_a_off, wave_v_off; // workspace offsets per entry (barrier forms)
@@ -651,6 +652,15 @@ namespace
         return !(env && env[0] == '0');
     }
 
+    // The ring form takes two LDS slot sets (one workgroup barrier per round
+    // instead of two) when twice its LDS still leaves room for every group
+    // of the launch where it is placed; JSVD_WAVE_RING_DB=0 keeps one set.
+    bool ring_db_enabled()
+    {
+        const char *env = std::getenv("JSVD_WAVE_RING_DB");
+        return !(env && env[0] == '0');
+    }
+
     bool wave_block_mode(int64_t count, int64_t max_half)
     {
         return max_half <= 256 && count > 4 + max_half * max_half / 200;
@@ -719,6 +729,7 @@ namespace
                 args.ring_groups = l.groups;
                 args.xcd_spread = l.xcd_spread;
                 args.xcc = p.wv_xcc.ptr;
+                args.ring_db = l.ring_db ? 1 : 0;
                 JSVD_HIP(jsvd::launch_wave_ring_sweeps_f64(l.m_max, l.n_max, l.groups, args, stream));
             }
             else
@@ -819,6 +830,22 @@ namespace
                         const int64_t g8 = std::min<int64_t>(l.count, (per_xcd / members) * kXcds);
                         l.groups = static_cast<int>(g8);
                         l.xcd_spread = kXcds;
+                    }
+                    // Two slot sets when the doubled LDS still holds the
+                    // launch's groups (on their XCDs when spread).
+                    const int64_t lds2 = 2 * jsvd::ring_lds_bytes(l.m_max, l.n_max);
+                    if (ring_db_enabled() && lds2 <= jsvd::kMaxLdsBytes)
+                    {
+                        int occ2 = 0;
+                        JSVD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                            &occ2, jsvd::wave_ring_sweeps_f64_symbol(l.m_max, l.n_max), 64 * waves,
+                            static_cast<int>(lds2)));
+                        const int64_t need = int64_t(l.groups) * members;
+                        const bool fits = l.xcd_spread > 0
+                                              ? (int64_t(occ2) * (cus / kXcds) >= members &&
+                                                 ((int64_t(occ2) * (cus / kXcds)) / members) * kXcds >= l.groups)
+                                              : int64_t(occ2) * cus >= need;
+                        l.ring_db = occ2 > 0 && fits;
                     }
                     max_xcc = std::max<int64_t>(max_xcc, int64_t(l.groups) * l.members);
                     l.mbox_group = jsvd::ring_mailbox_words(l.m_max, l.n_max, l.members);
@@ -927,7 +954,7 @@ namespace
         std::vector<int64_t> m, n;
         jsvd_config cfg{};
         std::vector<int> tune; // the shape-class table the plan was built with
-        int ring = 3;          // ring form of the wave solves enabled (bit 0), its XCD placement (bit 1)
+        int ring = 7;          // ring form of the wave solves enabled (bit 0), its XCD placement (bit 1), two slot sets (bit 2)
         jsvd_plan *plan = nullptr;
         uint64_t last_use = 0;
     };
@@ -1808,7 +1835,7 @@ jsvd_status jsvd_solve_batched_status_f64(int64_t count, const double *const *a_
     if (cfg)
         c = *cfg;
     const std::vector<int> tune = tune_table();
-    const int ring = (ring_enabled() ? 1 : 0) | (xcd_ring_enabled() ? 2 : 0);
+    const int ring = (ring_enabled() ? 1 : 0) | (xcd_ring_enabled() ? 2 : 0) | (ring_db_enabled() ? 4 : 0);
     jsvd_plan *plan = nullptr;
     for (CachedPlan &e : ctx->plans)
         if (e.m == mm && e.n == nn && same_config(e.cfg, c) && e.tune == tune && e.ring == ring)

@@ -234,6 +234,10 @@ namespace jsvd
         int ring_groups;
         int xcd_spread;
         unsigned int *xcc;
+        // Ring form: two sets of LDS slots used by round parity (the launch
+        // then takes twice ring_lds_bytes), so a round needs one workgroup
+        // barrier instead of two.
+        int ring_db;
     };
     int wave_sweeps_max_rows();
     // Ring form of the grouped solves (one wave per column pair, every

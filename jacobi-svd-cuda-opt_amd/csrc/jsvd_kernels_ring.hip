@@ -239,8 +239,11 @@ namespace jsvd
             unsigned int *bar = g.bars + group * g.members;
             unsigned long long *mb = g.mbox + group * g.mbox_group;
             const dev::SpinBound bound{g.fault, g.spin_limit};
-            double *my_top = ring_lds + (2 * wave) * SLOT;
-            double *my_bot = my_top + SLOT;
+            // With two slot sets (g.ring_db) round r uses set r % 2: a set is
+            // written two rounds after its last reads, and the barrier of the
+            // round between orders them, so the barrier before the writes
+            // goes.
+            const int set_stride = g.ring_db ? 2 * W * SLOT : 0;
             const int k = member * W + wave;
             unsigned int epoch = 0, tag = 0;
             int failed = 0;
@@ -313,6 +316,8 @@ namespace jsvd
                     {
                         ++tag;
                         const int par = static_cast<int>(tag & 1u) * MSG;
+                        double *my_top = ring_lds + static_cast<int>(tag & 1u) * set_stride + (2 * wave) * SLOT;
+                        double *my_bot = my_top + SLOT;
                         if (active)
                             any |= ring_pair<RA, RV>(tA, bA, tV, bV, g.eps);
                         // The boundary columns leave at once, so they travel
@@ -324,9 +329,10 @@ namespace jsvd
                             ring_send<RA, RV>(fwd_out + par, tag, tA, tV, lane, local);
                         if (send_back)
                             ring_send<RA, RV>(back_out + par, tag, bA, bV, lane, local);
-                        // Every read of the previous round's slots is done
-                        // (and every wave agrees on a failed wait).
-                        if (__syncthreads_or(failed))
+                        // One slot set: every read of the previous round's
+                        // slots is done (and every wave agrees on a failed
+                        // wait) before they are written.
+                        if (!g.ring_db && __syncthreads_or(failed))
                         {
                             failed = 1;
                             break;
@@ -336,7 +342,18 @@ namespace jsvd
                             lds_put<RA, RV>(my_top, tA, tV, lane);
                             lds_put<RA, RV>(my_bot, bA, bV, lane);
                         }
-                        __syncthreads();
+                        // Two sets: a failed wait of the previous round is
+                        // agreed on here, one round later.
+                        if (g.ring_db)
+                        {
+                            if (__syncthreads_or(failed))
+                            {
+                                failed = 1;
+                                break;
+                            }
+                        }
+                        else
+                            __syncthreads();
                         if (active)
                         {
                             // New bottom first: at k = half - 1 it is this
@@ -542,6 +559,7 @@ namespace jsvd
         const unsigned int grid =
             ((static_cast<unsigned int>(groups) + spread - 1) / spread) * spread * static_cast<unsigned int>(args.members);
         void *params[] = {const_cast<WaveGroupArgs *>(&args)};
-        return hipLaunchKernel(fn, dim3(grid), dim3(64 * ring_waves(m, n)), params, ring_lds_bytes(m, n), stream);
+        return hipLaunchKernel(fn, dim3(grid), dim3(64 * ring_waves(m, n)), params,
+                               ring_lds_bytes(m, n) * (args.ring_db ? 2 : 1), stream);
     }
 } // namespace jsvd

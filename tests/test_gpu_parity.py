@@ -241,6 +241,27 @@ def test_ring_xcd_placement_is_bitwise_neutral(monkeypatch):
             assert x[3] == y[3] and x[4] == y[4] == jsvd.STOP_CONVERGED
 
 
+def test_ring_slot_sets_are_bitwise_neutral(monkeypatch):
+    # With room for twice its LDS, the ring form uses two slot sets by round
+    # parity and one workgroup barrier per round instead of two.  Only the
+    # synchronisation changes, so the results must equal the one-set form's
+    # bitwise: a lone 256 x 256, 16 of them (one workgroup per CU either
+    # way) and a lone tall 700 x 90 (four-wave groups).
+    _note("ring form, two slot sets vs one")
+    rng = np.random.default_rng(778)
+    batches = [[rng.standard_normal((256, 256))], [rng.standard_normal((256, 256)) for _ in range(16)],
+               [rng.standard_normal((700, 90))]]
+    for mats in batches:
+        monkeypatch.setenv("JSVD_WAVE_RING_DB", "1")
+        two = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
+        monkeypatch.setenv("JSVD_WAVE_RING_DB", "0")
+        one = jsvd.svd_batched(mats, epsilon=1e-14, max_sweeps=30, with_stop=True)
+        for k, (x, y) in enumerate(zip(two, one)):
+            for a, b in zip(x[:3], y[:3]):
+                np.testing.assert_array_equal(a, b, err_msg=f"two vs one slot set {mats[k].shape} #{k}")
+            assert x[3] == y[3] and x[4] == y[4] == jsvd.STOP_CONVERGED
+
+
 def test_tall_single_streamed_held_rows():
     # A lone plain-path matrix of more than 1024 rows and at most 512 columns
     # runs four-wave groups with its rows streamed; each pair keeps its first
