@@ -1012,7 +1012,9 @@ namespace jsvd
         // QR shapes; a matrix takes the first that fits.  Unblocked
         // (mt = 0): qr_f64<block, group, rows> for n <= block / group columns
         // and m <= group rows rows.  Blocked (mt > 0): qr_blocked_f64<mt, nbw>
-        // for m <= 16 mt rows and n <= 64 nbw columns.  Up to 64 columns the
+        // for m <= 16 mt rows and n <= 64 nbw columns (so 65 to 128 columns
+        // of at most 128 rows; taller matrices of at most 64 columns take the
+        // last unblocked shape).  Up to 64 columns the
         // unblocked kernel is faster (measured on the 4096 uneven batch:
         // 2.97 vs 3.15 ms for the mixed execution, the blocked panels' 16
         // dependent steps then outweigh their cheaper updates); from 65 to
@@ -1022,7 +1024,7 @@ namespace jsvd
             int block, group, rows, mt, nbw;
         };
         constexpr QrSpec kQr[] = {{256, 4, 8, 0, 0}, {256, 4, 16, 0, 0}, {256, 4, 32, 0, 0},
-                                  {256, 0, 0, 8, 2}, {512, 4, 32, 0, 0}, {512, 8, 32, 0, 0}};
+                                  {256, 0, 0, 8, 2}, {512, 8, 32, 0, 0}};
         constexpr int kQrCount = sizeof(kQr) / sizeof(kQr[0]);
         template <int I>
         const void *qr_symbol_t()
@@ -1054,16 +1056,16 @@ namespace jsvd
             if constexpr (kQr[I].mt > 0)
                 finish_blocked_f64<kQr[I].mt, kQr[I].nbw><<<grid, 256, 0, stream>>>(args);
         }
-        const void *(*const kQrSymbol[])() = {qr_symbol_t<0>, qr_symbol_t<1>, qr_symbol_t<2>,
-                                             qr_symbol_t<3>, qr_symbol_t<4>, qr_symbol_t<5>};
+        const void *(*const kQrSymbol[])() = {qr_symbol_t<0>, qr_symbol_t<1>, qr_symbol_t<2>, qr_symbol_t<3>,
+                                             qr_symbol_t<4>};
         void (*const kQrLaunch[])(int, int, const PrecondArgs &, hipStream_t) = {
-            qr_launch_t<0>, qr_launch_t<1>, qr_launch_t<2>, qr_launch_t<3>, qr_launch_t<4>, qr_launch_t<5>};
+            qr_launch_t<0>, qr_launch_t<1>, qr_launch_t<2>, qr_launch_t<3>, qr_launch_t<4>};
         const void *(*const kFinishBlockedSymbol[])() = {finish_blocked_symbol_t<0>, finish_blocked_symbol_t<1>,
                                                          finish_blocked_symbol_t<2>, finish_blocked_symbol_t<3>,
-                                                         finish_blocked_symbol_t<4>, finish_blocked_symbol_t<5>};
+                                                         finish_blocked_symbol_t<4>};
         void (*const kFinishBlockedLaunch[])(int, const PrecondArgs &, hipStream_t) = {
             finish_blocked_launch_t<0>, finish_blocked_launch_t<1>, finish_blocked_launch_t<2>,
-            finish_blocked_launch_t<3>, finish_blocked_launch_t<4>, finish_blocked_launch_t<5>};
+            finish_blocked_launch_t<3>, finish_blocked_launch_t<4>};
         static_assert(sizeof(kQrLaunch) / sizeof(kQrLaunch[0]) == kQrCount, "one launcher per shape");
         static_assert(sizeof(kQrSymbol) / sizeof(kQrSymbol[0]) == kQrCount, "one symbol per shape");
         static_assert(sizeof(kFinishBlockedLaunch) / sizeof(kFinishBlockedLaunch[0]) == kQrCount,
