@@ -1,8 +1,12 @@
 // jsvd_kernels_mixed.hip -- mixed-precision one-sided Jacobi (no reference
 // counterpart: the reference is FP64 only).
 //
-// Four work-list launches on one stream per part of a batch (the plan splits
-// a large batch into up to four parts, each on a stream of its own):
+// Matrices that fit the QR preconditioner (jsvd_kernels_precond.hip: two
+// Householder QRs, the polar factor with B = X Q, the back-transform) run
+// steps 1 and 4 below on X = R2^T in place of A, with the preconditioner's
+// kernels instead of steps 2 and 3.  The others take four work-list
+// launches on one stream per part of a batch (the plan splits a large bucket
+// of wide matrices into two parts, each on a stream of its own):
 //  1. mixed_sweeps_compact_f32: FP32 sweeps with A and V in the compact
 //     LDS layout of the FP64 class kernel (top columns held in registers,
 //     half + 2 bottom slots in LDS: two 128 x 128 matrices per CU).  Dots
