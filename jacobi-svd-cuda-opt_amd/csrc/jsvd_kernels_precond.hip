@@ -12,7 +12,12 @@
 // here does not pivot (P = I, the permutation output is the identity).  With
 // X = Ux S Vx^T,
 //     A = Q1 R P^T = Q1 R2^T Q2^T P^T = (Q1 Ux) S (P Q2 Vx)^T,
-// so U = Q1 Ux and V = P Q2 Vx.  Two kernels, one workgroup per matrix each:
+// so U = Q1 Ux and V = P Q2 Vx.  Kernels, one workgroup per matrix each:
+//  * qr_blocked_f64 / finish_blocked_f64 (65 to 128 columns, at most 128
+//    rows): blocked QR and back-transform on FP64 MFMA (see their section
+//    below).
+//  * polar_bq_f64: the polar factor Q of the FP32 stage's V, B = X Q and one
+//    first-order simultaneous Jacobi step on B (see the kernel).
 //  * qr_f64: Householder QR in FP64, every column in the registers of a lane
 //    group, two workgroup barriers per step (see the kernel).  Writes
 //    R^T (row-major n x n), the reflectors (column-major m x n, explicit
