@@ -361,7 +361,9 @@ namespace jsvd
         // above about 7e-14; the FP32 sweeps leave |E| of 1e-6 .. 1e-4).  The
         // NT x (NT + 1) LDS buffer (odd stride: rows on distinct banks) holds
         // E, then M, then the new Q, each the B operand of the next product;
-        // Q and X are read from L2.
+        // Q and X are read from L2.  Then one simultaneous first-order Jacobi
+        // step rotates B and Q by W (five more products, described at the
+        // step); X, no longer needed, holds one of its operands.
         template <int NT>
         __global__ __launch_bounds__(256) void polar_bq_f64(PrecondArgs pa, MixedArgs ma)
         {
