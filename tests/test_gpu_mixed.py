@@ -1,6 +1,7 @@
-"""Mixed-precision path (FP32 sweeps, FP64 rotation parameters and
-convergence test, FP64 re-orthonormalisation, B = A Q on FP64 MFMA, FP64
-refinement sweeps) against the FP64 CPU oracle, through the C-ABI.
+"""Mixed-precision path (two FP64 QRs, FP32 sweeps with FP64 rotation
+parameters and convergence test, the FP64 polar factor, B = X Q and a
+first-order simultaneous Jacobi step on MFMA, FP64 refinement sweeps, the
+back-transform) against the FP64 CPU oracle, through the C-ABI.
 
 Bar (the FP64 path's own bar against the oracle): normwise sigma error
 max_k |s_k - r_k| / max_k r_k <= 1e-12 on sigma compared sorted (the
@@ -121,3 +122,30 @@ def test_mixed_rank_deficient_keeps_v_orthogonal():
             _check(a, g, r[1], 1e-14, f"rank-deficient {a.shape}")
         else:
             assert np.all(s == 0.0) and np.all(u == 0.0)
+
+
+def test_mixed_stress_kinds_and_clustered_spectra():
+    # The stress mix's kinds, and spectra of repeated and of clustered
+    # singular values (the first-order step before the FP64 refinement leaves
+    # pairs of close singular values to the sweeps), through the blocked QR
+    # (128 x 128, 100 x 70) and the one-column-per-lane-group QR (60 x 40),
+    # against the oracle with the stress suite's bar.
+    mats = []
+    for k, (m, n) in enumerate([(128, 128), (100, 70), (60, 40)]):
+        rng = np.random.default_rng(900 + k)
+        for kind in gen.STRESS_KINDS:
+            mats.append(np.ascontiguousarray(gen.stress_matrix(kind, m, n, rng)))
+        mats.append(gen.constructed(rng, m, n, np.repeat([1.0, 0.5, 1e-3, 1e-6], (n + 3) // 4)[:n]))
+        groups = 10.0 ** -np.arange((n + 7) // 8, dtype=np.float64)
+        mats.append(gen.constructed(rng, m, n, np.repeat(groups, 8)[:n] * (1.0 + 1e-10 * rng.standard_normal(n))))
+    _note(f"mixed stress: {len(mats)} matrices")
+    got = jsvd.svd_batched(mats, epsilon=1e-15, max_sweeps=30, precision=jsvd.PRECISION_MIXED)
+    refs = oracle.solve_batch(mats, 1e-15, 30)
+    for k, (a, g, r) in enumerate(zip(mats, got, refs)):
+        u, s, v, sw = g
+        top = max(r[1].max(), 1e-300)
+        tag = f"mixed stress #{k} {a.shape}"
+        assert np.max(np.abs(np.sort(s) - np.sort(r[1]))) <= 1e-12 * top, tag
+        assert metrics.residual(a, u, s, v) <= 1e-12, tag
+        assert metrics.orthogonality(v) <= 1e-11, tag
+        assert metrics.orthogonality(u, s > 1e-6 * top) <= 1e-9, tag
