@@ -37,6 +37,7 @@ namespace jsvd
     namespace
     {
         constexpr int kFinishBlock = 512;
+        constexpr int kFinishWaves = 8; // waves of the blocked back-transform
         constexpr int kFinishGroup = 8;
         constexpr int kFinishRows = 32; // most rows per lane: m <= 8 * 32 = 256
 
@@ -913,8 +914,8 @@ namespace jsvd
         // Y <- Q Y for the Q = Q_0 Q_1 ... Q_{nb-1} of a blocked QR (panel p:
         // columns 16 p .. of the column-major `rows` x n reflectors at rg, T
         // at tg + 256 p), over the column blocks y of this wave; panels are
-        // staged in LDS by the whole workgroup, last panel first.
-        template <int MT, int NBW>
+        // staged in LDS by the whole workgroup (BS threads), last panel first.
+        template <int MT, int NBW, int BS>
         __device__ __forceinline__ void apply_q_blocked(f64x4 (&y)[NBW][MT], const double *rg, const double *tg,
                                                         int rows, int n, double (*Vbuf)[16 * BlkShape<MT>::LDV],
                                                         double (*Tbuf)[256], int wave, int c, int q)
@@ -923,12 +924,13 @@ namespace jsvd
             const int tid = threadIdx.x, nb = (n + 15) / 16;
             auto stage = [&](int p) {
                 double *Vp = Vbuf[p & 1];
-                for (int idx = tid; idx < 16 * ROWS; idx += 256)
+                for (int idx = tid; idx < 16 * ROWS; idx += BS)
                 {
                     const int cc = idx / ROWS, o = idx - cc * ROWS, row = 16 * p + o, col = 16 * p + cc;
                     Vp[cc * LDV + o] = (row < rows && col < n) ? rg[int64_t(col) * rows + row] : 0.0;
                 }
-                Tbuf[p & 1][tid] = tg[256 * p + tid];
+                for (int e = tid; e < 256; e += BS)
+                    Tbuf[p & 1][e] = tg[256 * p + e];
             };
             stage(nb - 1);
             __syncthreads();
@@ -945,9 +947,12 @@ namespace jsvd
         }
 
         // Back-transform of a blocked-QR bucket: U = Q1 [Ux; 0] with the U
-        // rule, then V = Q2 Vx (the QR does not pivot: P = I).
-        template <int MT, int NBW>
-        __global__ __launch_bounds__(256) void finish_blocked_f64(PrecondArgs args)
+        // rule, then V = Q2 Vx (the QR does not pivot: P = I).  WAVES waves,
+        // column block b in wave b % WAVES as its slot b / WAVES: eight waves
+        // with one block each keep a wave's registers to one block, so two
+        // matrices share a CU.
+        template <int MT, int NBW, int WAVES>
+        __global__ __launch_bounds__(64 * WAVES) void finish_blocked_f64(PrecondArgs args)
         {
             constexpr int LDV = BlkShape<MT>::LDV;
             __shared__ double Vbuf[2][16 * LDV];
@@ -958,7 +963,7 @@ namespace jsvd
             for (;;)
             {
                 claim(&args.counters[2], &s_item);
-                const int item = s_item;
+                const int item = __builtin_amdgcn_readfirstlane(s_item);
                 if (item >= args.count)
                     break;
                 const WorkItem w = args.items[item];
@@ -970,41 +975,44 @@ namespace jsvd
                     // part 0: Ux (n x n) -> U (m x n); part 1: Vx -> V (n x n).
                     const double *src = (part == 0 ? args.ux : args.vx) + w.v_off;
                     const int rows = part == 0 ? m : n;
+                    // Row 16 t + 4 r + q of a lane: the tests compare q with a
+                    // uniform bound, so no per-element row index is held.
 #pragma unroll
                     for (int sl = 0; sl < NBW; ++sl)
                     {
-                        const int col = 16 * (4 * sl + wave) + c;
+                        const int col = 16 * (WAVES * sl + wave) + c;
+                        const double *ps = src + q * n + col;
 #pragma unroll
                         for (int t = 0; t < MT; ++t)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                             {
-                                const int row = 16 * t + q + 4 * r;
-                                y[sl][t][r] = (row < n && col < n) ? src[row * n + col] : 0.0;
+                                const int k = 16 * t + 4 * r;
+                                y[sl][t][r] = (q < n - k && col < n) ? ps[k * n] : 0.0;
                             }
                     }
-                    if (part == 0)
-                        apply_q_blocked<MT, NBW>(y, args.refl + w.a_off, args.tq + t_off, rows, n, Vbuf, Tbuf, wave, c,
-                                                 q);
-                    else
-                        apply_q_blocked<MT, NBW>(y, args.refl2 + w.v_off, args.tq2 + t_off, rows, n, Vbuf, Tbuf, wave,
-                                                 c, q);
+                    // One call site for both parts: the y blocks stay in one
+                    // register set.
+                    const double *rg = part == 0 ? args.refl + w.a_off : args.refl2 + w.v_off;
+                    const double *tg = (part == 0 ? args.tq : args.tq2) + t_off;
+                    apply_q_blocked<MT, NBW, 64 * WAVES>(y, rg, tg, rows, n, Vbuf, Tbuf, wave, c, q);
                     double *dst = (part == 0 ? args.u + w.a_off : args.v + w.v_off);
 #pragma unroll
                     for (int sl = 0; sl < NBW; ++sl)
                     {
-                        const int col = 16 * (4 * sl + wave) + c;
+                        const int col = 16 * (WAVES * sl + wave) + c;
                         if (col < n)
                         {
                             const bool keep = part == 1 || args.sigma[w.s_off + col] > args.eps;
+                            double *pd = dst + q * n + col;
 #pragma unroll
                             for (int t = 0; t < MT; ++t)
 #pragma unroll
                                 for (int r = 0; r < 4; ++r)
                                 {
-                                    const int row = 16 * t + q + 4 * r;
-                                    if (row < rows)
-                                        dst[row * n + col] = keep ? y[sl][t][r] : 0.0;
+                                    const int k = 16 * t + 4 * r;
+                                    if (q < rows - k)
+                                        pd[k * n] = keep ? y[sl][t][r] : 0.0;
                                 }
                         }
                     }
@@ -1053,7 +1061,8 @@ namespace jsvd
         const void *finish_blocked_symbol_t()
         {
             if constexpr (kQr[I].mt > 0)
-                return reinterpret_cast<const void *>(&finish_blocked_f64<kQr[I].mt, kQr[I].nbw>);
+                return reinterpret_cast<const void *>(
+                    &finish_blocked_f64<kQr[I].mt, (4 * kQr[I].nbw + kFinishWaves - 1) / kFinishWaves, kFinishWaves>);
             else
                 return nullptr;
         }
@@ -1061,7 +1070,8 @@ namespace jsvd
         void finish_blocked_launch_t(int grid, const PrecondArgs &args, hipStream_t stream)
         {
             if constexpr (kQr[I].mt > 0)
-                finish_blocked_f64<kQr[I].mt, kQr[I].nbw><<<grid, 256, 0, stream>>>(args);
+                finish_blocked_f64<kQr[I].mt, (4 * kQr[I].nbw + kFinishWaves - 1) / kFinishWaves, kFinishWaves>
+                    <<<grid, 64 * kFinishWaves, 0, stream>>>(args);
         }
         const void *(*const kQrSymbol[])() = {qr_symbol_t<0>, qr_symbol_t<1>, qr_symbol_t<2>, qr_symbol_t<3>,
                                              qr_symbol_t<4>};
@@ -1154,7 +1164,7 @@ namespace jsvd
     }
     int precond_finish_block(int variant)
     {
-        return qrcp_blocked(variant) ? 256 : kFinishBlock;
+        return qrcp_blocked(variant) ? 64 * kFinishWaves : kFinishBlock;
     }
 
     const void *polar_bq_symbol(int64_t n)
