@@ -723,15 +723,22 @@ namespace jsvd
             // broadcast; column J keeps its R entries, takes beta on row J and
             // v below.
             double vb[S];
-#pragma unroll
-            for (int s = sj; s < S; ++s)
             {
-                const int o = q + 4 * s;
-                const double xs = x[s] * scale;
-                const double v = s > sj ? xs : (o > J ? xs : (o == J ? 1.0 : 0.0));
-                vb[s] = row_bcast<J>(v);
-                const double nx = s > sj ? v : (o > J ? v : (o == J ? beta : x[s]));
-                x[s] = c == J ? nx : x[s];
+                const int o = q + 4 * sj;
+                const double xs = x[sj] * scale;
+                const double v = o > J ? xs : (o == J ? 1.0 : 0.0);
+                vb[sj] = row_bcast<J>(v);
+                const double nx = o > J ? v : (o == J ? beta : x[sj]);
+                x[sj] = c == J ? nx : x[sj];
+            }
+            // Registers below row J's: column J's entries become v = x scale,
+            // the other columns' are multiplied by 1 (unchanged).
+            const double g = c == J ? scale : 1.0;
+#pragma unroll
+            for (int s = sj + 1; s < S; ++s)
+            {
+                x[s] *= g;
+                vb[s] = row_bcast<J>(x[s]);
             }
             double d0 = 0.0, d1 = 0.0;
 #pragma unroll
@@ -766,6 +773,82 @@ namespace jsvd
             ((qr_step<J, S>(x, trow, tau_c, c, q), __builtin_amdgcn_sched_barrier(0)), ...);
         }
 
+        // Householder factor of a staged panel over its first SP registers
+        // per lane (rows q + 4 s, s < SP, relative to 16 p; the rows past them
+        // are zero and lie beyond the matrix, so nothing of them is written).
+        template <int MT, int SP>
+        __device__ __forceinline__ void factor_panel(int p, double *Vp, double *Tp, const PrecondArgs &args,
+                                                     const WorkItem &w, int64_t t_off, int c, int q)
+        {
+            constexpr int LDV = BlkShape<MT>::LDV;
+            const int m = w.m, n = w.n, col = 16 * p + c;
+            const bool live = col < n;
+            double *xg = args.x + w.v_off;
+            double *rg = args.refl + w.a_off;
+            double x[SP];
+#pragma unroll
+            for (int s = 0; s < SP; ++s)
+                x[s] = Vp[c * LDV + q + 4 * s];
+            double trow[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                trow[j] = 0.0;
+            double tau_c = 0.0;
+            qr_steps<SP>(x, trow, tau_c, c, q, std::make_integer_sequence<int, 16>{});
+            wave_lds_sync();
+            // V for the other waves, T, and the outputs of column col: R's
+            // rows 16 p .. as X = R^T (zeros below the diagonal), the
+            // reflector (zeros above, unit diagonal), tau.
+#pragma unroll
+            for (int s = 0; s < SP; ++s)
+            {
+                const int o = q + 4 * s;
+                Vp[c * LDV + o] = o < c ? 0.0 : (o == c ? 1.0 : x[s]);
+            }
+            if (q == 0)
+            {
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                {
+                    Tp[c * 16 + j] = trow[j];
+                    args.tq[t_off + 256 * p + c * 16 + j] = trow[j];
+                }
+            }
+            if (live)
+            {
+#pragma unroll
+                for (int s = 0; s < SP; ++s)
+                {
+                    const int o = q + 4 * s, row = 16 * p + o;
+                    if (row < n)
+                        xg[int64_t(col) * n + row] = o <= c ? x[s] : 0.0;
+                    if (row < m)
+                        rg[int64_t(col) * m + row] = o < c ? 0.0 : (o == c ? 1.0 : x[s]);
+                }
+                if (q == 0)
+                    args.tau[w.s_off + col] = tau_c;
+            }
+            wave_lds_sync();
+        }
+
+        // factor_panel over the smallest even tile count K >= tiles.
+        template <int MT, int K>
+        __device__ __forceinline__ void factor_panel_fit(int tiles, int p, double *Vp, double *Tp,
+                                                         const PrecondArgs &args, const WorkItem &w, int64_t t_off,
+                                                         int c, int q)
+        {
+            static_assert(K % 2 == 0, "even tile counts");
+            if constexpr (K > 2)
+            {
+                if (tiles <= K - 2)
+                {
+                    factor_panel_fit<MT, K - 2>(tiles, p, Vp, Tp, args, w, t_off, c, q);
+                    return;
+                }
+            }
+            factor_panel<MT, 4 * K>(p, Vp, Tp, args, w, t_off, c, q);
+        }
+
         // Panel p of the QR, by its owning wave: the block's tiles from t = p
         // on are staged in LDS (rows relative to 16 p, zeros beyond the padded
         // matrix) into registers x[s] (rows q + 4 s of column c), factored by
@@ -778,7 +861,7 @@ namespace jsvd
                                                  const PrecondArgs &args, const WorkItem &w, int64_t t_off, int c,
                                                  int q)
         {
-            constexpr int LDV = BlkShape<MT>::LDV, S = BlkShape<MT>::S;
+            constexpr int LDV = BlkShape<MT>::LDV;
             const int m = w.m, n = w.n, col = 16 * p + c;
             const bool live = col < n;
             double *xg = args.x + w.v_off;
@@ -811,50 +894,10 @@ namespace jsvd
                 }
             }
             wave_lds_sync();
-            double x[S];
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-                x[s] = Vp[c * LDV + q + 4 * s];
-            double trow[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j)
-                trow[j] = 0.0;
-            double tau_c = 0.0;
-            qr_steps<S>(x, trow, tau_c, c, q, std::make_integer_sequence<int, 16>{});
-            wave_lds_sync();
-            // V for the other waves, T, and the outputs of column col: R's
-            // rows 16 p .. as X = R^T (zeros below the diagonal), the
-            // reflector (zeros above, unit diagonal), tau.
-#pragma unroll
-            for (int s = 0; s < S; ++s)
-            {
-                const int o = q + 4 * s;
-                Vp[c * LDV + o] = o < c ? 0.0 : (o == c ? 1.0 : x[s]);
-            }
-            if (q == 0)
-            {
-#pragma unroll
-                for (int j = 0; j < 16; ++j)
-                {
-                    Tp[c * 16 + j] = trow[j];
-                    args.tq[t_off + 256 * p + c * 16 + j] = trow[j];
-                }
-            }
-            if (live)
-            {
-#pragma unroll
-                for (int s = 0; s < S; ++s)
-                {
-                    const int o = q + 4 * s, row = 16 * p + o;
-                    if (row < n)
-                        xg[int64_t(col) * n + row] = o <= c ? x[s] : 0.0;
-                    if (row < m)
-                        rg[int64_t(col) * m + row] = o < c ? 0.0 : (o == c ? 1.0 : x[s]);
-                }
-                if (q == 0)
-                    args.tau[w.s_off + col] = tau_c;
-            }
-            wave_lds_sync();
+            // Rows from 16 ceil(min(m, rows) / 16) on are zero padding: the
+            // factor holds the tiles left (rounded up to an even count).
+            const int tiles = (min(m, BlkShape<MT>::ROWS) + 15) / 16 - p;
+            factor_panel_fit<MT, MT>(tiles, p, Vp, Tp, args, w, t_off, c, q);
         }
 
         template <int MT, int NBW>
