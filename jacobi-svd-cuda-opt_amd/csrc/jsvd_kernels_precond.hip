@@ -607,14 +607,23 @@ namespace jsvd
 
         // Y <- (I - V T V^T) Y (TRANS false) or (I - V T^T V^T) Y (TRANS
         // true) for one column block y of MT tiles.  V: LDS, column-major with
-        // stride LDV, rows from 16 t0 on (V's rows above are zero: the tiles
-        // below t0 take zero operands, selected rather than branched around
-        // so that no tile's registers need copies); T: LDS, 16 x 16
-        // row-major.
-        template <int MT, bool TRANS>
+        // stride LDV, rows from 16 t0 on (V's rows above are zero); T: LDS,
+        // 16 x 16 row-major.  The instance for T0, the even tile count at or
+        // below t0, leaves the tiles below T0 out; the tile between T0 and t0
+        // (if any) takes zero operands, selected rather than branched around
+        // so that no tile's registers need copies.
+        template <int MT, bool TRANS, int T0 = 0>
         __device__ __forceinline__ void apply_block_reflector(f64x4 (&y)[MT], const double *V, const double *T, int t0,
                                                               int c, int q)
         {
+            if constexpr (T0 + 2 < MT)
+            {
+                if (t0 >= T0 + 2)
+                {
+                    apply_block_reflector<MT, TRANS, T0 + 2>(y, V, T, t0, c, q);
+                    return;
+                }
+            }
             constexpr int LDV = BlkShape<MT>::LDV;
             // W = V^T Y: A[a][row] = V[row][a] from lane (c = a, q = row % 4),
             // one accumulator per K-step r (four independent chains), at most
@@ -624,7 +633,7 @@ namespace jsvd
             for (int r = 0; r < 4; ++r)
                 wr[r] = f64x4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int t = 0; t < MT; ++t)
+            for (int t = T0; t < MT; ++t)
             {
                 const bool on = t >= t0;
                 const int tt = on ? t - t0 : 0;
@@ -649,7 +658,7 @@ namespace jsvd
             for (int r = 0; r < 4; ++r)
             {
 #pragma unroll
-                for (int t = 0; t < MT; ++t)
+                for (int t = T0; t < MT; ++t)
                 {
                     const bool on = t >= t0;
                     const int tt = on ? t - t0 : 0;
