@@ -1314,7 +1314,7 @@ jsvd_status jsvd_plan_create(int64_t count, const int64_t *m, const int64_t *n, 
                                        &b.grid_qr2));
                 JSVD_CHECK(resident_of(jsvd::precond_finish_symbol(b.qr_variant, b.max_m), jsvd::precond_finish_block(b.qr_variant), b.lds_fin,
                                        &b.grid_fin));
-                JSVD_CHECK(resident_of(jsvd::polar_bq_symbol(b.max_n), 256, jsvd::polar_bq_lds_bytes(b.max_n),
+                JSVD_CHECK(resident_of(jsvd::polar_bq_symbol(b.max_n), jsvd::polar_bq_block(), jsvd::polar_bq_lds_bytes(b.max_n),
                                        &b.grid_polar));
             }
         }

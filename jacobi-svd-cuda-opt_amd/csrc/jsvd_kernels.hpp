@@ -151,6 +151,7 @@ namespace jsvd
     // its square entries, q, u = B buffer, counters 2-3).
     const void *polar_bq_symbol(int64_t n); // instance for a bucket's largest n (<= 128)
     int polar_bq_lds_bytes(int64_t n);
+    int polar_bq_block(); // threads per workgroup
     hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, hipStream_t stream);
     hipError_t launch_precond_finish_f64(int variant, int64_t m, int grid, int lds_bytes, const PrecondArgs &args,
                                          hipStream_t stream);

@@ -307,10 +307,15 @@ namespace jsvd
 
         typedef double f64x4 __attribute__((ext_vector_type(4)));
 
+        // Waves of the polar kernel: two row halves by four column quarters.
+        constexpr int kPolarWaves = 8;
+
         // One n x n product C = A B (n <= NT) on FP64 MFMA
-        // (v_mfma_f64_16x16x4f64) by one workgroup of four waves: wave w owns
-        // the (NT/2) x (NT/2) quadrant (w / 2, w % 2) of C as T x T tiles of
-        // 16 x 16 (T = NT / 32), accumulated in `acc`.  K runs in blocks of
+        // (v_mfma_f64_16x16x4f64) by one workgroup of eight waves: wave w owns
+        // the (NT/2) x (NT/4) block (w / 4, w % 4) of C as TX x TY tiles of
+        // 16 x 16 (TX = NT / 32, TY = NT / 64), accumulated in `acc` (two
+        // waves per SIMD, each with half the accumulators of a quadrant, so
+        // they stay in registers).  K runs in blocks of
         // 16; in step s of a block, lane l supplies A[i0 + l%16][k_l] and
         // B[k_l][j0 + l%16] with k_l = k0 + 4 (l / 16) + s (any assignment of
         // the block's k to the four lane groups sums the same products, and
@@ -318,34 +323,50 @@ namespace jsvd
         // column-major B operands are read as 32 contiguous bytes).  a(i, k)
         // and b(k, j) return the operands (zero outside n x n); lane l then
         // holds C[i0 + l/16 + 4 r][j0 + l%16] in acc[x][y][r].
+        // v itself, as a value the compiler cannot see through: lane indices
+        // laundered at each product are not hoisted (with everything derived
+        // from them) to the top of the kernel, where they would hold
+        // registers through every other product.
+        __device__ __forceinline__ int opaque_int(int v)
+        {
+            asm volatile("" : "+v"(v));
+            return v;
+        }
+
         template <int NT, typename FA, typename FB>
-        __device__ __forceinline__ void mfma_nn(f64x4 (&acc)[NT / 32][NT / 32], FA a, FB b, int n, int wr, int wc,
+        __device__ __forceinline__ void mfma_nn(f64x4 (&acc)[NT / 32][NT / 64], FA a, FB b, int n, int wr, int wc,
                                                 int lr, int lk)
         {
-            constexpr int T = NT / 32;
+            constexpr int TX = NT / 32, TY = NT / 64;
+            wr = opaque_int(wr);
+            wc = opaque_int(wc);
+            lr = opaque_int(lr);
+            lk = opaque_int(lk);
 #pragma unroll
-            for (int x = 0; x < T; ++x)
+            for (int x = 0; x < TX; ++x)
 #pragma unroll
-                for (int y = 0; y < T; ++y)
+                for (int y = 0; y < TY; ++y)
                     acc[x][y] = f64x4{0.0, 0.0, 0.0, 0.0};
             for (int k0 = 0; k0 < n; k0 += 16)
             {
-                double av[T][4], bv[T][4];
+                double av[TX][4], bv[TY][4];
 #pragma unroll
-                for (int t = 0; t < T; ++t)
+                for (int s = 0; s < 4; ++s)
+                {
+                    const int k = k0 + 4 * lk + s;
 #pragma unroll
-                    for (int s = 0; s < 4; ++s)
-                    {
-                        const int k = k0 + 4 * lk + s;
+                    for (int t = 0; t < TX; ++t)
                         av[t][s] = a(wr + 16 * t + lr, k);
+#pragma unroll
+                    for (int t = 0; t < TY; ++t)
                         bv[t][s] = b(k, wc + 16 * t + lr);
-                    }
+                }
 #pragma unroll
                 for (int s = 0; s < 4; ++s)
 #pragma unroll
-                    for (int x = 0; x < T; ++x)
+                    for (int x = 0; x < TX; ++x)
 #pragma unroll
-                        for (int y = 0; y < T; ++y)
+                        for (int y = 0; y < TY; ++y)
                             acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x][s], bv[y][s], acc[x][y], 0, 0, 0);
             }
         }
@@ -366,17 +387,23 @@ namespace jsvd
         // step rotates B and Q by W (five more products, described at the
         // step); X, no longer needed, holds one of its operands.
         template <int NT>
-        __global__ __launch_bounds__(256) void polar_bq_f64(PrecondArgs pa, MixedArgs ma)
+        __global__ __launch_bounds__(64 * kPolarWaves) void polar_bq_f64(PrecondArgs pa, MixedArgs ma)
         {
-            constexpr int T = NT / 32, LD = NT + 1;
+            constexpr int TX = NT / 32, TY = NT / 64, LD = NT + 1, BS = 64 * kPolarWaves;
             extern __shared__ double2 polar_raw[];
             double *S = reinterpret_cast<double *>(polar_raw);
-            __shared__ double s_max[4];
+            __shared__ double s_max[kPolarWaves];
             __shared__ int s_item;
             const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-            const int lr = lane & 15, lk = lane >> 4;
-            const int wr = (wave >> 1) * (NT / 2), wc = (wave & 1) * (NT / 2);
-            f64x4 acc[T][T];
+            int lr = lane & 15, lk = lane >> 4;
+            int wr = (wave >> 2) * (NT / 2), wc = (wave & 3) * (NT / 4);
+            auto fresh = [&]() {
+                wr = opaque_int(wr);
+                wc = opaque_int(wc);
+                lr = opaque_int(lr);
+                lk = opaque_int(lk);
+            };
+            f64x4 acc[TX][TY];
             for (;;)
             {
                 claim(&ma.counters[2], &s_item);
@@ -397,11 +424,12 @@ namespace jsvd
                 {
                     // E = Q^T Q - I into LDS (zero outside n x n), and max |E|.
                     mfma_nn<NT>(acc, qt, qc, n, wr, wc, lr, lk);
+                    fresh();
                     double emax = 0.0;
 #pragma unroll
-                    for (int xx = 0; xx < T; ++xx)
+                    for (int xx = 0; xx < TX; ++xx)
 #pragma unroll
-                        for (int yy = 0; yy < T; ++yy)
+                        for (int yy = 0; yy < TY; ++yy)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                             {
@@ -415,14 +443,18 @@ namespace jsvd
                     if (lane == 0)
                         s_max[wave] = emax;
                     __syncthreads();
-                    const double e_all = fmax(fmax(s_max[0], s_max[1]), fmax(s_max[2], s_max[3]));
+                    double e_all = s_max[0];
+#pragma unroll
+                    for (int k = 1; k < kPolarWaves; ++k)
+                        e_all = fmax(e_all, s_max[k]);
                     // F = E E (both operands from LDS), then this wave's part of
                     // M = I - E/2 + 3 F / 8 in registers.
                     mfma_nn<NT>(acc, sm, sm, n, wr, wc, lr, lk);
+                    fresh();
 #pragma unroll
-                    for (int xx = 0; xx < T; ++xx)
+                    for (int xx = 0; xx < TX; ++xx)
 #pragma unroll
-                        for (int yy = 0; yy < T; ++yy)
+                        for (int yy = 0; yy < TY; ++yy)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                             {
@@ -431,10 +463,11 @@ namespace jsvd
                                 acc[xx][yy][r] = fma(0.375, acc[xx][yy][r], fma(-0.5, e, i == j ? 1.0 : 0.0));
                             }
                     __syncthreads(); // every wave has read E
+                    fresh();
 #pragma unroll
-                    for (int xx = 0; xx < T; ++xx)
+                    for (int xx = 0; xx < TX; ++xx)
 #pragma unroll
-                        for (int yy = 0; yy < T; ++yy)
+                        for (int yy = 0; yy < TY; ++yy)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                             {
@@ -446,10 +479,11 @@ namespace jsvd
                     // starting V) and to LDS (the operand of B = X Q).
                     mfma_nn<NT>(acc, qr, sm, n, wr, wc, lr, lk);
                     __syncthreads(); // every wave has read Q and M
+                    fresh();
 #pragma unroll
-                    for (int xx = 0; xx < T; ++xx)
+                    for (int xx = 0; xx < TX; ++xx)
 #pragma unroll
-                        for (int yy = 0; yy < T; ++yy)
+                        for (int yy = 0; yy < TY; ++yy)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                             {
@@ -470,10 +504,11 @@ namespace jsvd
                 mfma_nn<NT>(acc, [=](int i, int k) { return (i < n && k < n) ? x[i * n + k] : 0.0; }, sm, n, wr, wc,
                             lr, lk);
                 auto each = [&](auto f) {
+                    fresh();
 #pragma unroll
-                    for (int xx = 0; xx < T; ++xx)
+                    for (int xx = 0; xx < TX; ++xx)
 #pragma unroll
-                        for (int yy = 0; yy < T; ++yy)
+                        for (int yy = 0; yy < TY; ++yy)
 #pragma unroll
                             for (int r = 0; r < 4; ++r)
                                 f(acc[xx][yy][r], wr + 16 * xx + lk + 4 * r, wc + 16 * yy + lr);
@@ -500,7 +535,7 @@ namespace jsvd
                 // E in place: the thread of pair i < j reads G_ij, G_ii, G_jj
                 // and writes E_ij and E_ji = -E_ij; the diagonal (read by
                 // every pair) is cleared after a barrier.
-                for (int idx = tid; idx < n * n; idx += 256)
+                for (int idx = tid; idx < n * n; idx += BS)
                 {
                     const int i = idx / n, j = idx - i * n;
                     if (i < j)
@@ -512,7 +547,7 @@ namespace jsvd
                     }
                 }
                 __syncthreads();
-                for (int i = tid; i < n; i += 256)
+                for (int i = tid; i < n; i += BS)
                     S[i * LD + i] = 0.0;
                 __syncthreads();
                 // P1 = E (I + E/3); X (no longer needed) takes I + P1/2.
@@ -1231,12 +1266,17 @@ namespace jsvd
         return nt * (nt + 1) * 8;
     }
 
+    int polar_bq_block()
+    {
+        return 64 * kPolarWaves;
+    }
+
     hipError_t launch_polar_bq_f64(int64_t n, int grid, const PrecondArgs &pa, const MixedArgs &ma, hipStream_t stream)
     {
         if (n <= 64)
-            polar_bq_f64<64><<<grid, 256, polar_bq_lds_bytes(n), stream>>>(pa, ma);
+            polar_bq_f64<64><<<grid, polar_bq_block(), polar_bq_lds_bytes(n), stream>>>(pa, ma);
         else
-            polar_bq_f64<128><<<grid, 256, polar_bq_lds_bytes(n), stream>>>(pa, ma);
+            polar_bq_f64<128><<<grid, polar_bq_block(), polar_bq_lds_bytes(n), stream>>>(pa, ma);
         return hipGetLastError();
     }
 
