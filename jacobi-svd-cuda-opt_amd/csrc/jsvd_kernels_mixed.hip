@@ -9,11 +9,13 @@
 // of wide matrices into two parts, each on a stream of its own):
 //  1. mixed_sweeps_compact_f32: FP32 sweeps with A and V in the compact
 //     LDS layout of the FP64 class kernel (top columns held in registers,
-//     half + 2 bottom slots in LDS: two 128 x 128 matrices per CU).  Dots
-//     and column updates run in FP32 with FP32 accumulation; the rotation
-//     parameters (zeta, t, c, s) and the off-diagonal test are computed in
-//     FP64 from the FP32 dots.  The FP32 phase stops at the FP32 rounding
-//     floor, tol32 = max(eps, 8 sqrt(m) 2^-24).  V32 is written out as FP64.
+//     half + 2 bottom slots in LDS: two 128 x 128 matrices per CU).  The dot
+//     p.q and the column updates run in FP32 with FP32 accumulation; the
+//     squared column norms are summed afresh at each sweep's start and then
+//     carried through the rotations in FP64; the rotation parameters (zeta,
+//     t, c, s) and the off-diagonal test are computed in FP64.  The FP32
+//     phase stops at the FP32 rounding floor, tol32 = max(eps, 8 sqrt(m)
+//     2^-24).  V32 is written out as FP64.
 //  2. orthonormalize_f64: modified Gram-Schmidt of V32's columns in FP64,
 //     giving an orthogonal Q (V32 is orthogonal only to ~1e-6).
 //  3. gemm_aq_f64: B = A Q in FP64 on MFMA (v_mfma_f64_16x16x4f64), written
@@ -85,29 +87,33 @@ namespace jsvd
 
         // One column pair in FP32 with its top column held in registers (xp
         // for A, yp for V, updated in place) and its bottom column in LDS (aq,
-        // vq): FP32 dots (FP32 accumulation) and updates; the off-diagonal
-        // test and the rotation parameters (zeta, t, c, s) in FP64 from the
-        // FP32 dots.
+        // vq): the FP32 dot p.q (FP32 accumulation) and FP32 updates; the
+        // off-diagonal test and the rotation parameters (zeta, t, c, s) in
+        // FP64.  The squared norms come from the columns' tracked norms (pn
+        // for the top column, *qn for the bottom one), which a rotation by
+        // the FP32 (c, s) carries over in FP64:
+        //   |p'|^2 = c^2 |p|^2 - 2 c s p.q + s^2 |q|^2,
+        //   |q'|^2 = s^2 |p|^2 + 2 c s p.q + c^2 |q|^2
+        // (one dot per pair instead of three; the sweeps start from norms
+        // summed afresh).
         template <int G, int RMAX4, int VMAX4>
         __device__ __forceinline__ int pair_f32_held(float4 (&xp)[RMAX4], float4 (&yp)[VMAX4],
                                                      float4 *__restrict__ aq, float4 *__restrict__ vq, int rows4,
-                                                     int vrows4, int lane, double tol)
+                                                     int vrows4, int lane, double tol, double &pn,
+                                                     double *__restrict__ qn)
         {
             float4 xq[RMAX4];
-            float pp = 0.0f, qq = 0.0f, pq = 0.0f;
+            float pq = 0.0f;
 #pragma unroll
             for (int j = 0; j < RMAX4; ++j)
             {
                 if (j < rows4)
                 {
                     xq[j] = aq[lane + j * G];
-                    pp += xp[j].x * xp[j].x + xp[j].y * xp[j].y + xp[j].z * xp[j].z + xp[j].w * xp[j].w;
-                    qq += xq[j].x * xq[j].x + xq[j].y * xq[j].y + xq[j].z * xq[j].z + xq[j].w * xq[j].w;
                     pq += xp[j].x * xq[j].x + xp[j].y * xq[j].y + xp[j].z * xq[j].z + xp[j].w * xq[j].w;
                 }
             }
-            const double dpp = group_sum_f32<G>(pp);
-            const double dqq = group_sum_f32<G>(qq);
+            const double dpp = pn, dqq = *qn;
             const double dpq = group_sum_f32<G>(pq);
             if (!dev::rotation_needed(dpp, dqq, dpq, tol))
                 return 0;
@@ -162,6 +168,11 @@ namespace jsvd
                         vq[lane + j * G] = nq;
                     }
                 }
+                const double c2 = static_cast<double>(c) * c, s2 = static_cast<double>(s) * s;
+                const double x = 2.0 * static_cast<double>(c) * s * dpq;
+                pn = fmax(c2 * dpp - x + s2 * dqq, 0.0);
+                if (lane == 0)
+                    *qn = fmax(s2 * dpp + x + c2 * dqq, 0.0);
             }
             return 1;
         }
@@ -175,7 +186,7 @@ namespace jsvd
         // Same pairs in the same order as the reference circle, and half the
         // LDS of holding all columns (two 128 x 128 matrices per CU).
         template <int BS, int G, int RMAX4, int VMAX4>
-        __global__ __launch_bounds__(BS) void mixed_sweeps_compact_f32(MixedArgs args)
+        __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(G < 16 ? 4 : 1))) void mixed_sweeps_compact_f32(MixedArgs args)
         {
             extern __shared__ float4 lds_quads[];
             float *lds32 = reinterpret_cast<float *>(lds_quads);
@@ -203,6 +214,7 @@ namespace jsvd
                 float *Vs = As + ns * lda;  // ns slots of ldv rows
                 float4 *A4 = reinterpret_cast<float4 *>(As);
                 float4 *V4 = reinterpret_cast<float4 *>(Vs);
+                double *Ns = reinterpret_cast<double *>(Vs + ns * ldv); // ns squared slot norms
                 const int lda4 = lda / 4, ldv4 = ldv / 4;
                 const int rows4 = lda / (4 * G), vrows4 = ldv / (4 * G);
                 const double *ag = args.a + w.a_off;
@@ -221,6 +233,7 @@ namespace jsvd
                     }
                 };
                 int pos[PASSES], ct[PASSES], cb[PASSES], fresh[PASSES];
+                double tn[PASSES]; // squared norm of the held top column
                 float4 ta[PASSES][RMAX4];
                 float4 tv[PASSES][VMAX4];
                 auto load_top = [&](int i, int sl) {
@@ -256,9 +269,44 @@ namespace jsvd
                 int z = even - 1; // bottom[0]'s column this round
                 int gs = 0;       // global round mod ns
                 int executed = 0;
+                // Squared column norms summed afresh from the FP32 columns:
+                // every slot's (the bottom row's columns and the tops that a
+                // group loads later) and every held top's.
+                auto sq_norm = [&](const float4 *col) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int j = 0; j < RMAX4; ++j)
+                        if (j < rows4)
+                        {
+                            const float4 x = col[lane + j * G];
+                            acc += x.x * x.x + x.y * x.y + x.z * x.z + x.w * x.w;
+                        }
+                    return static_cast<double>(group_sum_f32<G>(acc));
+                };
+                auto fresh_norms = [&]() {
+                    for (int sl = gid; sl < ns; sl += GROUPS)
+                    {
+                        const double nn = sq_norm(A4 + sl * lda4);
+                        if (lane == 0)
+                            Ns[sl] = nn;
+                    }
+#pragma unroll
+                    for (int i = 0; i < PASSES; ++i)
+                    {
+                        float acc = 0.0f;
+#pragma unroll
+                        for (int j = 0; j < RMAX4; ++j)
+                            if (j < rows4)
+                                acc += ta[i][j].x * ta[i][j].x + ta[i][j].y * ta[i][j].y + ta[i][j].z * ta[i][j].z +
+                                       ta[i][j].w * ta[i][j].w;
+                        tn[i] = static_cast<double>(group_sum_f32<G>(acc));
+                    }
+                };
                 for (int sweep = 0; sweep < args.max_sweeps; ++sweep)
                 {
                     int any = 0;
+                    fresh_norms();
+                    __syncthreads();
                     for (int r = 0; r < rounds; ++r)
                     {
                         const int z1 = z - 1 < 1 ? ring : z - 1;
@@ -273,13 +321,17 @@ namespace jsvd
                             {
                                 const int p = ct[i], q = cb[i];
                                 if (fresh[i] & (p < n))
+                                {
                                     load_top(i, ls);
+                                    tn[i] = Ns[ls];
+                                }
                                 int bsl = gs + pos[i];
                                 bsl = half == 1 ? 0 : (bsl >= ns ? bsl - ns : bsl);
                                 int rot = 0;
                                 if ((p < n) & (q < n))
                                     rot = pair_f32_held<G, RMAX4, VMAX4>(ta[i], tv[i], A4 + bsl * lda4,
-                                                                         V4 + bsl * ldv4, rows4, vrows4, lane, tol);
+                                                                         V4 + bsl * ldv4, rows4, vrows4, lane, tol,
+                                                                         tn[i], Ns + bsl);
                                 any |= rot;
                                 // Leaving position half-1: the top column becomes
                                 // bottom[half-1] in slot ws; the group restarts at
@@ -295,6 +347,8 @@ namespace jsvd
                                     for (int j = 0; j < VMAX4; ++j)
                                         if (j < vrows4)
                                             V4[ws * ldv4 + lane + j * G] = tv[i][j];
+                                    if (lane == 0)
+                                        Ns[ws] = tn[i];
                                 }
                                 const int q1 = q - 1 < 1 ? ring : q - 1;
                                 const int q2 = q - 2 < 1 ? q - 2 + ring : q - 2;
@@ -537,10 +591,11 @@ namespace jsvd
 
     int64_t f32_class_lds_bytes(int cls, int64_t m, int64_t n)
     {
-        // half + 2 slots of A and of V (compact layout).
+        // half + 2 slots of A and of V (compact layout) and their squared
+        // norms (FP64).
         const int g = kF32Classes[cls].group;
         const int64_t slots = (n + 1) / 2 + 2;
-        return (padded4(static_cast<int>(m), g) + padded4(static_cast<int>(n), g)) * slots * 4;
+        return (padded4(static_cast<int>(m), g) + padded4(static_cast<int>(n), g)) * slots * 4 + slots * 8;
     }
 
     bool f32_class_fits(int cls, int64_t m, int64_t n)
