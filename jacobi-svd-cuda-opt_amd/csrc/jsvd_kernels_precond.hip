@@ -16,8 +16,8 @@
 //  * qr_blocked_f64 / finish_blocked_f64 (65 to 128 columns, at most 128
 //    rows): blocked QR and back-transform on FP64 MFMA (see their section
 //    below).
-//  * polar_bq_f64: the polar factor Q of the FP32 stage's V, B = X Q and one
-//    first-order simultaneous Jacobi step on B (see the kernel).
+//  * polar_bq_f64: the polar factor Q of the FP32 stage's V, B = X Q and two
+//    first-order simultaneous Jacobi steps on B (see the kernel).
 //  * qr_f64: Householder QR in FP64, every column in the registers of a lane
 //    group, two workgroup barriers per step (see the kernel).  Writes
 //    R^T (row-major n x n), the reflectors (column-major m x n, explicit
@@ -309,6 +309,8 @@ namespace jsvd
 
         // Waves of the polar kernel: two row halves by four column quarters.
         constexpr int kPolarWaves = 8;
+        // First-order simultaneous Jacobi steps after B = X Q.
+        constexpr int kFirstOrderSteps = 2;
 
         // One n x n product C = A B (n <= NT) on FP64 MFMA
         // (v_mfma_f64_16x16x4f64) by one workgroup of eight waves: wave w owns
@@ -383,8 +385,8 @@ namespace jsvd
         // above about 7e-14; the FP32 sweeps leave |E| of 1e-6 .. 1e-4).  The
         // NT x (NT + 1) LDS buffer (odd stride: rows on distinct banks) holds
         // E, then M, then the new Q, each the B operand of the next product;
-        // Q and X are read from L2.  Then one simultaneous first-order Jacobi
-        // step rotates B and Q by W (five more products, described at the
+        // Q and X are read from L2.  Then two simultaneous first-order Jacobi
+        // steps rotate B and Q by W (five more products each, described at the
         // step); X, no longer needed, holds one of its operands.
         template <int NT>
         __global__ __launch_bounds__(64 * kPolarWaves) void polar_bq_f64(PrecondArgs pa, MixedArgs ma)
@@ -519,70 +521,75 @@ namespace jsvd
                 });
                 __threadfence_block();
                 __syncthreads();
-                // One simultaneous first-order Jacobi step on B: with
+                // A simultaneous first-order Jacobi step on B: with
                 // G = B^T B, the rotation generator E_ij = G_ij / (G_jj - G_ii)
                 // (antisymmetric, formed from G's upper triangle) and
                 // W = I + E + E^2/2 + E^3/6 (orthogonal to O(E^4)), B <- B W
                 // and Q <- Q W take the columns' cosines from the FP32
-                // stage's ~5e-6 to ~1e-10, which saves the FP64 refinement
-                // about 1.6 of its ~3.6 sweeps.  Entries above 2^-14 (close
+                // stage's ~5e-6 to ~1e-10 and a second step below the FP64
+                // threshold, which leaves the FP64 refinement (~3.6 sweeps
+                // without the steps) its closing sweep without rotations on
+                // most matrices.  Entries above 2^-14 (close
                 // singular values) are left to the sweeps.
-                auto bt = [=](int i, int k) { return (i < n && k < n) ? b[i * n + k] : 0.0; }; // (B^T)[i][k]
-                auto bc = [=](int k, int j) { return (k < n && j < n) ? b[j * n + k] : 0.0; }; // B[k][j]
-                mfma_nn<NT>(acc, bt, bc, n, wr, wc, lr, lk);
-                each([&](double v, int i, int j) { S[i * LD + j] = (i < n && j < n) ? v : 0.0; });
-                __syncthreads();
-                // E in place: the thread of pair i < j reads G_ij, G_ii, G_jj
-                // and writes E_ij and E_ji = -E_ij; the diagonal (read by
-                // every pair) is cleared after a barrier.
-                for (int idx = tid; idx < n * n; idx += BS)
+                for (int step = 0; step < kFirstOrderSteps; ++step)
                 {
-                    const int i = idx / n, j = idx - i * n;
-                    if (i < j)
+                    auto bt = [=](int i, int k) { return (i < n && k < n) ? b[i * n + k] : 0.0; }; // (B^T)[i][k]
+                    auto bc = [=](int k, int j) { return (k < n && j < n) ? b[j * n + k] : 0.0; }; // B[k][j]
+                    mfma_nn<NT>(acc, bt, bc, n, wr, wc, lr, lk);
+                    each([&](double v, int i, int j) { S[i * LD + j] = (i < n && j < n) ? v : 0.0; });
+                    __syncthreads();
+                    // E in place: the thread of pair i < j reads G_ij, G_ii, G_jj
+                    // and writes E_ij and E_ji = -E_ij; the diagonal (read by
+                    // every pair) is cleared after a barrier.
+                    for (int idx = tid; idx < n * n; idx += BS)
                     {
-                        const double g = S[i * LD + j], den = S[j * LD + j] - S[i * LD + i];
-                        const double e = (den != 0.0 && fabs(g) * 16384.0 <= fabs(den)) ? g / den : 0.0;
-                        S[i * LD + j] = e;
-                        S[j * LD + i] = -e;
+                        const int i = idx / n, j = idx - i * n;
+                        if (i < j)
+                        {
+                            const double g = S[i * LD + j], den = S[j * LD + j] - S[i * LD + i];
+                            const double e = (den != 0.0 && fabs(g) * 16384.0 <= fabs(den)) ? g / den : 0.0;
+                            S[i * LD + j] = e;
+                            S[j * LD + i] = -e;
+                        }
                     }
+                    __syncthreads();
+                    for (int i = tid; i < n; i += BS)
+                        S[i * LD + i] = 0.0;
+                    __syncthreads();
+                    // P1 = E (I + E/3); X (no longer needed) takes I + P1/2.
+                    mfma_nn<NT>(acc, sm,
+                                [=](int k, int j) { return S[k * LD + j] * (1.0 / 3.0) + (k == j ? 1.0 : 0.0); }, n, wr,
+                                wc, lr, lk);
+                    double *xs = pa.x + w.v_off;
+                    each([&](double v, int i, int j) {
+                        if (i < n && j < n)
+                            xs[i * n + j] = fma(0.5, v, i == j ? 1.0 : 0.0);
+                    });
+                    __threadfence_block();
+                    __syncthreads();
+                    // W = I + E (I + P1/2), into LDS once every wave has read E.
+                    mfma_nn<NT>(acc, sm, [=](int k, int j) { return (k < n && j < n) ? xs[k * n + j] : 0.0; }, n, wr,
+                                wc, lr, lk);
+                    __syncthreads();
+                    each([&](double v, int i, int j) { S[i * LD + j] = (i < n && j < n) ? v + (i == j ? 1.0 : 0.0) : 0.0; });
+                    __syncthreads();
+                    // B <- B W and Q <- Q W, each stored once every wave has read it.
+                    mfma_nn<NT>(acc, [=](int i, int k) { return (i < n && k < n) ? b[k * n + i] : 0.0; }, sm, n, wr, wc,
+                                lr, lk);
+                    __syncthreads();
+                    each([&](double v, int i, int j) {
+                        if (i < n && j < n)
+                            b[j * n + i] = v;
+                    });
+                    mfma_nn<NT>(acc, qr, sm, n, wr, wc, lr, lk);
+                    __syncthreads();
+                    each([&](double v, int i, int j) {
+                        if (i < n && j < n)
+                            q[j * n + i] = v;
+                    });
+                    __threadfence_block();
+                    __syncthreads(); // LDS and the global operands free for the next matrix
                 }
-                __syncthreads();
-                for (int i = tid; i < n; i += BS)
-                    S[i * LD + i] = 0.0;
-                __syncthreads();
-                // P1 = E (I + E/3); X (no longer needed) takes I + P1/2.
-                mfma_nn<NT>(acc, sm,
-                            [=](int k, int j) { return S[k * LD + j] * (1.0 / 3.0) + (k == j ? 1.0 : 0.0); }, n, wr,
-                            wc, lr, lk);
-                double *xs = pa.x + w.v_off;
-                each([&](double v, int i, int j) {
-                    if (i < n && j < n)
-                        xs[i * n + j] = fma(0.5, v, i == j ? 1.0 : 0.0);
-                });
-                __threadfence_block();
-                __syncthreads();
-                // W = I + E (I + P1/2), into LDS once every wave has read E.
-                mfma_nn<NT>(acc, sm, [=](int k, int j) { return (k < n && j < n) ? xs[k * n + j] : 0.0; }, n, wr,
-                            wc, lr, lk);
-                __syncthreads();
-                each([&](double v, int i, int j) { S[i * LD + j] = (i < n && j < n) ? v + (i == j ? 1.0 : 0.0) : 0.0; });
-                __syncthreads();
-                // B <- B W and Q <- Q W, each stored once every wave has read it.
-                mfma_nn<NT>(acc, [=](int i, int k) { return (i < n && k < n) ? b[k * n + i] : 0.0; }, sm, n, wr, wc,
-                            lr, lk);
-                __syncthreads();
-                each([&](double v, int i, int j) {
-                    if (i < n && j < n)
-                        b[j * n + i] = v;
-                });
-                mfma_nn<NT>(acc, qr, sm, n, wr, wc, lr, lk);
-                __syncthreads();
-                each([&](double v, int i, int j) {
-                    if (i < n && j < n)
-                        q[j * n + i] = v;
-                });
-                __threadfence_block();
-                __syncthreads(); // LDS and the global operands free for the next matrix
             }
             if (threadIdx.x == 0)
             {
