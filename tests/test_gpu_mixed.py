@@ -1,6 +1,6 @@
 """Mixed-precision path (two FP64 QRs, FP32 sweeps with FP64 rotation
-parameters and convergence test, the FP64 polar factor, B = X Q and a
-first-order simultaneous Jacobi step on MFMA, FP64 refinement sweeps, the
+parameters and convergence test, the FP64 polar factor, B = X Q and two
+first-order simultaneous Jacobi steps on MFMA, FP64 refinement sweeps, the
 back-transform) against the FP64 CPU oracle, through the C-ABI.
 
 Bar (the FP64 path's own bar against the oracle): normwise sigma error
@@ -126,7 +126,7 @@ def test_mixed_rank_deficient_keeps_v_orthogonal():
 
 def test_mixed_stress_kinds_and_clustered_spectra():
     # The stress mix's kinds, and spectra of repeated and of clustered
-    # singular values (the first-order step before the FP64 refinement leaves
+    # singular values (the first-order steps before the FP64 refinement leave
     # pairs of close singular values to the sweeps), through the blocked QR
     # (128 x 128, 100 x 70) and the one-column-per-lane-group QR (60 x 40),
     # against the oracle with the stress suite's bar.
