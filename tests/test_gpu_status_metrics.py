@@ -85,7 +85,17 @@ def test_stop_reason_blocked_and_mixed(bw):
     small = [rng.standard_normal((64, 48)) for _ in range(3)]
     out = jsvd.svd_batched(small, epsilon=1e-13, max_sweeps=40, precision=jsvd.PRECISION_MIXED, with_stop=True)
     assert all(o[4] == jsvd.STOP_CONVERGED for o in out)
-    out = jsvd.svd_batched(small, epsilon=1e-13, max_sweeps=40, refine_sweeps=1, precision=jsvd.PRECISION_MIXED,
+    # Pairs of singular values 1e-9 apart: the FP32 sweeps leave such a pair
+    # at an FP32 cosine and the first-order steps leave it to the FP64
+    # refinement, whose first sweep then rotates; capped at one refinement
+    # sweep, the solve stops at the cap.  (On well-separated spectra the
+    # first-order steps leave the refinement a first sweep without rotations,
+    # so one sweep already converges.)
+    pairs = np.repeat(10.0 ** (-np.arange(24, dtype=np.float64) / 4), 2) * (1.0 + 1e-9 * np.tile([0.0, 1.0], 24))
+    close = [gen.constructed(rng, 64, 48, pairs) for _ in range(3)]
+    out = jsvd.svd_batched(close, epsilon=1e-13, max_sweeps=40, precision=jsvd.PRECISION_MIXED, with_stop=True)
+    assert all(o[4] == jsvd.STOP_CONVERGED for o in out)
+    out = jsvd.svd_batched(close, epsilon=1e-13, max_sweeps=40, refine_sweeps=1, precision=jsvd.PRECISION_MIXED,
                            with_stop=True)
     assert all(o[4] == jsvd.STOP_MAX_SWEEPS for o in out)
 
